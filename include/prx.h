@@ -1,0 +1,212 @@
+/*
+ * prx.h -- C ABI of libprx, the MI355X (gfx950) screened-Poisson solve path.
+ *
+ * The entry points replace the public API of the reference crate
+ * (Avanatiker/PoissonReconstruction); each declaration names the reference
+ * item it stands in for.  All pointers are plain host pointers unless the
+ * name ends in _device; no torch or HIP types appear in the signatures.
+ * Every function returns a status code (PRX_OK = 0) and never unwinds; the
+ * text of the last error on the calling thread is prx_last_error().
+ *
+ * Vector layout: every fine-level vector (solution, RHS, weights) is in the
+ * reference's FEM node index order, i.e. the order of FEMTree::solution
+ * (src/fem_tree.rs:59), which is the Morton code of the node offset with x in
+ * bit 0.  Coarse-level vectors exported by prx_coarse_system use the
+ * reference's row-major coarse order (src/fem_tree.rs:361).
+ */
+#ifndef PRX_H
+#define PRX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define PRX_OK 0
+#define PRX_ERR_INVALID 1       /* bad argument (depth out of range, null pointer, size) */
+#define PRX_ERR_NONFINITE 2     /* NaN/Inf in the input points */
+#define PRX_ERR_NOT_CONVERGED 3 /* iteration cap reached before the tolerance */
+#define PRX_ERR_BREAKDOWN 4     /* CG breakdown: non-finite beta (src/solvers.rs:63) */
+#define PRX_ERR_DEVICE 5        /* HIP runtime / kernel failure, or no GPU */
+#define PRX_ERR_OOM 6           /* device allocation failed */
+#define PRX_ERR_STATE 7         /* call order (e.g. solve before assemble) */
+
+#define PRX_MAX_DEPTH 10 /* 8^10 nodes = 2^30: FEM indices stay 32-bit (octree.rs NodeIndex is i32) */
+
+/* ------------------------------------------------------------ types */
+
+/* OrientedPoint (src/fem_tree.rs:10) with Point3/Vec3 (src/geometry.rs:9-13,
+ * 146-150) flattened; the Rust side marshals into this repr(C) layout. */
+typedef struct prx_point {
+    double px, py, pz;
+    double nx, ny, nz;
+} prx_point;
+
+enum { PRX_PREC_FP64 = 0, PRX_PREC_MIXED = 1 };
+/* Stopping test of the fine solve.  REFERENCE: ||r|| <= eps*||r0|| with r0
+ * the residual after smoothing (src/solvers.rs:36-58).  TRUE: the FP64 true
+ * residual ||b-Ax|| <= eps*||b||. */
+enum { PRX_RESID_REFERENCE = 0, PRX_RESID_TRUE = 1 };
+
+/* ReconstructParams (src/reconstructor.rs:8-15): the reference fields in the
+ * reference order, then extension fields.  prx_params_default() gives the
+ * reference defaults (8, 1.5, 1.1, 1e-3, 500, 0.0, false). */
+typedef struct prx_params {
+    uint32_t depth;
+    double samples_per_node; /* accepted and ignored, as in src/fem_tree.rs:83 */
+    double scale;
+    double cg_accuracy;
+    uint64_t cg_iters;
+    double point_weight;
+    int32_t verbose; /* ignored inside reconstruct, as in the reference */
+    /* extensions */
+    int32_t precision;      /* PRX_PREC_FP64 (default) or PRX_PREC_MIXED */
+    int32_t residual_mode;  /* PRX_RESID_REFERENCE (default) or PRX_RESID_TRUE */
+    int32_t device;         /* HIP device ordinal (default 0) */
+    int32_t gs_iters;       /* smoothing sweeps per level (reference: 20) */
+    int32_t base_depth;     /* -1: D-2 rule of src/reconstructor.rs:51 */
+    int32_t refresh_period; /* CG true-residual refresh period (reference: 10; 0 = never) */
+    double inner_tol;       /* mixed: residual reduction of one FP32 inner solve (default 1e-3) */
+    uint64_t max_outer;     /* mixed: cap on FP64 refinement steps (default 64) */
+} prx_params;
+
+/* ReconstructStats (src/reconstructor.rs:17-23) plus extension fields.
+ * Unlike the reference (which leaves them 0), time_system and matrix_nnz are
+ * filled: time_system is the fine+coarse assembly time, matrix_nnz the number
+ * of stored entries the reference's CSR would hold. */
+typedef struct prx_stats {
+    double time_transform, time_tree, time_system, time_solve, time_extract, time_total;
+    uint64_t fem_nodes, matrix_nnz, mesh_vertices, mesh_triangles;
+    double iso_value;
+    /* extensions */
+    uint64_t coarse_iters, fine_iters, outer_iters;
+    double final_rel_residual; /* FP64 ||b-Ax||/||b|| of the returned solution */
+    double solve_ms;           /* GPU time of the solve (hipEvent) */
+    int32_t solve_status;      /* PRX_OK / PRX_ERR_NOT_CONVERGED / PRX_ERR_BREAKDOWN */
+} prx_stats;
+
+/* Mesh (src/marching_cubes.rs:8-11), in model space; arrays owned by the
+ * library until prx_mesh_free. */
+typedef struct prx_mesh {
+    double* vertices;    /* [3*n_vertices] */
+    uint64_t n_vertices;
+    uint64_t* triangles; /* [3*n_triangles] */
+    uint64_t n_triangles;
+} prx_mesh;
+
+/* Options and report of one cascadic solve (FEMTree::solve_cascadic,
+ * src/fem_tree.rs:458-487). */
+typedef struct prx_solve_opts {
+    int32_t precision;
+    int32_t residual_mode;
+    int32_t refresh_period;
+    int32_t gs_iters;
+    double inner_tol;
+    uint64_t max_outer;
+} prx_solve_opts;
+
+typedef struct prx_solve_report {
+    uint64_t coarse_iters, fine_iters, outer_iters, applies;
+    double coarse_rel;      /* coarse CG sqrt(delta/delta0) at exit */
+    double fine_rel;        /* fine stopping-test value at exit */
+    double true_rel;        /* FP64 ||b-Ax||/||b|| of the final solution */
+    double ms_assemble, ms_coarse, ms_fine_gs, ms_fine_cg, ms_total;
+    int32_t status;
+} prx_solve_report;
+
+typedef struct prx_tree prx_tree;
+
+/* ------------------------------------------------------------ errors */
+const char* prx_last_error(void);
+int prx_device_count(int* out);
+
+/* ------------------------------------------------------------ pipeline */
+void prx_params_default(prx_params* p);
+void prx_solve_opts_default(prx_solve_opts* o);
+
+/* reconstruct (src/reconstructor.rs:25-70).  Empty input gives PRX_OK and an
+ * empty mesh with zeroed stats (src/reconstructor.rs:28). */
+int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params, prx_mesh* out_mesh,
+                    prx_stats* out_stats);
+void prx_mesh_free(prx_mesh* mesh);
+
+/* ------------------------------------------------------------ FEMTree */
+
+/* FEMTree::new + initialize + finalize (src/fem_tree.rs:67-126): the complete
+ * tree to `depth`, nodes numbered in Morton order. */
+int prx_tree_create(uint32_t depth, int device, prx_tree** out);
+void prx_tree_destroy(prx_tree* t);
+/* fem_node_count (src/fem_tree.rs:54) */
+int64_t prx_tree_node_count(const prx_tree* t);
+/* idx_to_offset [3N] and grid_to_idx [res^3], row-major z*res*res+y*res+x
+ * (src/fem_tree.rs:56-58, :117-118); either pointer may be NULL. */
+int prx_tree_export_nodes(prx_tree* t, uint32_t* idx_to_offset, int64_t* grid_to_idx);
+
+/* Upload points already in the unit cube (the output of the transform at
+ * src/reconstructor.rs:36-40) and splat them: FEMTree::splat_normal_field
+ * (src/fem_tree.rs:130-178).  Builds the per-node point lists used by
+ * assembly and the operator. */
+int prx_splat_normal_field(prx_tree* t, const prx_point* unit_points, size_t n);
+/* normal_field [3N, AoS x,y,z] and node_weight [N] (src/fem_tree.rs:60-61). */
+int prx_export_normal_field(prx_tree* t, double* normal_field, double* node_weight);
+
+/* FEMTree::assemble_system + assemble_rhs (src/fem_tree.rs:180-299): sets up
+ * the matrix-free fine operator and computes the RHS on the device. */
+int prx_assemble(prx_tree* t, double point_weight);
+int prx_export_rhs(prx_tree* t, double* rhs);
+/* Verification mode: the CSR the reference would hold (src/sparse.rs:24-33),
+ * row i sorted by column, values summed in the reference's push order.
+ * prx_csr_nnz first, then prx_export_csr with off[N+1], col[nnz], val[nnz]. */
+int prx_csr_nnz(prx_tree* t, int64_t* nnz);
+int prx_export_csr(prx_tree* t, int64_t* off, int64_t* col, double* val);
+
+/* y = A x with the matrix-free operator (SparseMatrix::multiply_vector,
+ * src/sparse.rs:105-117, on the assembled system).  precision
+ * PRX_PREC_MIXED applies the FP32 operator. */
+int prx_apply(prx_tree* t, int precision, const double* x, double* y);
+
+/* build_system_at_depth (src/fem_tree.rs:346-404) at depth cdepth: RHS in
+ * row-major coarse order [8^cdepth] and the operator's constants
+ * coef[4] = {K centre, K edge, K corner, diagonal screening wh3*1.005}. */
+int prx_coarse_system(prx_tree* t, uint32_t cdepth, double* rhs_rowmajor, double* coef);
+/* prolongate (src/fem_tree.rs:310-343): coarse row-major -> fine [N]. */
+int prx_prolongate(prx_tree* t, uint32_t cdepth, const double* coarse_rowmajor, double* fine);
+
+/* `sweeps` multicolour Gauss-Seidel sweeps on the fine system, in place on x
+ * (the reference's gauss_seidel_sweep, src/solvers.rs:164-175, recoloured:
+ * 8 colours by coordinate parity). */
+int prx_gs(prx_tree* t, int precision, uint32_t sweeps, const double* b, double* x);
+/* solve_cg (src/solvers.rs:20-69) on the fine system from x.
+ * `iters` and `rel` may be NULL. */
+int prx_cg(prx_tree* t, int precision, const double* b, double* x, uint64_t max_iters, double eps,
+           int residual_mode, int refresh_period, uint64_t* iters, double* rel);
+
+/* FEMTree::solve_cascadic (src/fem_tree.rs:458-487). The solution stays on
+ * the device; prx_get_solution copies it out. */
+int prx_solve_cascadic(prx_tree* t, uint32_t base_depth, uint32_t gs, uint64_t cg_iters, double eps,
+                       const prx_solve_opts* opts, prx_solve_report* report);
+int prx_get_solution(prx_tree* t, double* x);
+int prx_set_solution(prx_tree* t, const double* x);
+
+/* FEMTree::evaluate (src/fem_tree.rs:490-513) at m query points q[3m]. */
+int prx_evaluate(prx_tree* t, const double* q, size_t m, double* out);
+/* evaluate at every marching-cubes corner k/res, k in [0,res]^3
+ * (src/fem_tree.rs:596-605), x fastest: out[(kz*(res+1)+ky)*(res+1)+kx]. */
+int prx_evaluate_corners(prx_tree* t, double* out);
+/* iso value: mean of evaluate over the uploaded points (src/reconstructor.rs:56). */
+int prx_iso_value(prx_tree* t, double* iso);
+
+/* ------------------------------------------------------------ generic CSR
+ * SparseMatrix::multiply_vector (src/sparse.rs:105-117) and solve_cg
+ * (src/solvers.rs:20-69) on a caller CSR, run on the device. */
+int prx_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* x, double* y);
+int prx_csr_cg(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* b, double* x,
+               uint64_t max_iters, double eps, uint64_t* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRX_H */

@@ -1,0 +1,450 @@
+// Grid-level kernels: node numbering export, vector updates and reductions of
+// CG, coarse RHS and prolongation, field evaluation, generic CSR SpMV.
+//
+// Kernels that must reproduce the reference bit for bit (tree export, coarse
+// RHS, prolongation, evaluate, CSR SpMV) evaluate every expression in the
+// reference's order; contraction to FMA is off in this file so a*b+c rounds
+// twice, as Rust does.
+#pragma clang fp contract(off)
+
+#include "kernels.hpp"
+
+namespace prx {
+
+static inline int nblocks_for(int64_t n, int per_thread = 1, int threads = 256, int cap = 1 << 30) {
+    int64_t b = (n + (int64_t)threads * per_thread - 1) / ((int64_t)threads * per_thread);
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+// Fixed grid of the reduction kernels: the partial count depends on n only,
+// so every reduction is summed in the same order on every run.
+static inline int red_blocks(int64_t n) { return nblocks_for(n, 8, 256, 2048); }
+
+int partial_capacity(int64_t N) {
+    int64_t b = (N + 511) / 512;    // one partial per brick (8^3 or 16^3)
+    int64_t r = red_blocks(N);
+    int64_t nv = (N + 255) / 256;   // naive stencil: one partial per block
+    int64_t m = b > r ? b : r;
+    if (nv > m) m = nv;
+    return (int)(m < 2048 ? 2048 : m);
+}
+
+// Deterministic block sum: 64-lane butterfly, then waves in order.
+__device__ inline double block_sum_256(double v, double* wsum) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wsum[w];
+    return s;
+}
+
+// ------------------------------------------------------------------ tree export
+__global__ void k_export_offsets(uint32_t N, uint32_t* i2o) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t x, y, z;
+    demorton3(i, x, y, z);
+    i2o[3 * (uint64_t)i] = x;
+    i2o[3 * (uint64_t)i + 1] = y;
+    i2o[3 * (uint64_t)i + 2] = z;
+}
+void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s) {
+    uint32_t N = 1u << (3 * D);
+    hipLaunchKernelGGL(k_export_offsets, dim3(nblocks_for(N)), dim3(256), 0, s, N, i2o);
+    PRX_LAUNCH_CHECK();
+}
+
+__global__ void k_export_g2i(uint32_t D, int64_t* g2i) {
+    const uint32_t res = 1u << D;
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (uint64_t)res * res * res) return;
+    uint32_t x = r % res, y = (r / res) % res, z = r / ((uint64_t)res * res);
+    g2i[r] = (int64_t)morton3(x, y, z);
+}
+void launch_export_g2i(uint32_t D, int64_t* g2i, hipStream_t s) {
+    int64_t n = (int64_t)1 << (3 * D);
+    hipLaunchKernelGGL(k_export_g2i, dim3(nblocks_for(n)), dim3(256), 0, s, D, g2i);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ fills
+template <typename T>
+__global__ void k_fill(T* p, T v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+template <typename T>
+void launch_fill(T* p, T v, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, p, v, n);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_fill<double>(double*, double, int64_t, hipStream_t);
+template void launch_fill<float>(float*, float, int64_t, hipStream_t);
+template void launch_fill<uint32_t>(uint32_t*, uint32_t, int64_t, hipStream_t);
+template void launch_fill<int64_t>(int64_t*, int64_t, int64_t, hipStream_t);
+
+// ------------------------------------------------------------------ reductions
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(const T* a, const T* b, int64_t n, double* partial) {
+    __shared__ double ws[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += (double)a[i] * (double)b[i];
+    double s = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+template <typename T>
+void launch_dot(const T* a, const T* b, int64_t n, double* partial, int* nblk, hipStream_t s) {
+    int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, a, b, n, partial);
+    PRX_LAUNCH_CHECK();
+    *nblk = nb;
+}
+template void launch_dot<double>(const double*, const double*, int64_t, double*, int*, hipStream_t);
+template void launch_dot<float>(const float*, const float*, int64_t, double*, int*, hipStream_t);
+
+// One block folds the partials in a fixed order and advances the CG scalars
+// exactly as solve_cg does (src/solvers.rs:36-64).
+__global__ __launch_bounds__(256) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
+    __shared__ double ws[4];
+    if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && st->done) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
+    double sum = block_sum_256(acc, ws);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_SLOT: st->slot[slot] = sum; break;
+        case FIN_INIT_REF:  // delta0 = r.r ; stop when delta <= delta0*eps^2
+        case FIN_INIT_TRUE: // stop when delta <= ||b||^2 eps^2
+            st->delta = sum;
+            st->delta0 = sum;
+            st->it = 0;
+            st->iters = 0;
+            st->status = 0;
+            st->refresh_due = 0;
+            st->thr = ((op == FIN_INIT_REF ? sum : st->bnorm2) * st->eps) * st->eps;
+            st->done = (sum == 0.0 || sum <= st->thr) ? 1 : 0;
+            break;
+        case FIN_ALPHA:
+            st->dq = sum;
+            if (sum == 0.0) {
+                st->done = 1;
+                st->iters = st->it + 1;
+            } else {
+                st->alpha = st->delta / sum;
+            }
+            break;
+        case FIN_BETA: {
+            double dold = st->delta;
+            st->delta = sum;
+            st->it += 1;
+            if (sum <= st->thr) {
+                st->done = 1;
+                st->iters = st->it;
+                break;
+            }
+            double beta = sum / dold;
+            if (beta != beta || beta == INFINITY || beta == -INFINITY) {
+                st->done = 1;
+                st->status = PRX_ERR_BREAKDOWN;
+                st->iters = st->max_iters;
+                break;
+            }
+            st->beta = beta;
+            if (st->it >= st->max_iters) {
+                st->done = 1;
+                st->iters = st->max_iters;
+                st->status = PRX_ERR_NOT_CONVERGED;
+            }
+            break;
+        }
+    }
+}
+void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(256), 0, s, st, partial, np, op, slot);
+    PRX_LAUNCH_CHECK();
+}
+
+// x += alpha d ; r -= alpha q ; partial = r.r   (src/solvers.rs:47, :53, :57)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
+                                                   const T* __restrict__ q, const CgState* st, int64_t n, double* partial) {
+    __shared__ double ws[4];
+    if (st->done) return;
+    const double a = st->alpha;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double di = (double)d[i], qi = (double)q[i];
+        x[i] = (T)((double)x[i] + a * di);
+        T rn = (T)((double)r[i] - a * qi);
+        r[i] = rn;
+        acc += (double)rn * (double)rn;
+    }
+    double s = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+template <typename T>
+void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s) {
+    int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, s, x, r, d, q, st, n, partial);
+    PRX_LAUNCH_CHECK();
+    *nblk = nb;
+}
+template void launch_cg_update<double>(double*, double*, const double*, const double*, const CgState*, int64_t, double*, int*, hipStream_t);
+template void launch_cg_update<float>(float*, float*, const float*, const float*, const CgState*, int64_t, double*, int*, hipStream_t);
+
+template <typename T>
+__global__ void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ d, const CgState* st, int64_t n) {
+    if (st->done) return;
+    const double a = st->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] + a * (double)d[i]);
+}
+template <typename T>
+void launch_axpy_alpha(T* x, const T* d, const CgState* st, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, x, d, st, n);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_axpy_alpha<double>(double*, const double*, const CgState*, int64_t, hipStream_t);
+template void launch_axpy_alpha<float>(float*, const float*, const CgState*, int64_t, hipStream_t);
+
+// d = r + beta d   (src/solvers.rs:64)
+template <typename T>
+__global__ void k_xpby_beta(T* __restrict__ d, const T* __restrict__ r, const CgState* st, int64_t n) {
+    if (st->done) return;
+    const double b = st->beta;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = (T)((double)r[i] + b * (double)d[i]);
+}
+template <typename T>
+void launch_xpby_beta(T* d, const T* r, const CgState* st, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_xpby_beta<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, d, r, st, n);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_xpby_beta<double>(double*, const double*, const CgState*, int64_t, hipStream_t);
+template void launch_xpby_beta<float>(float*, const float*, const CgState*, int64_t, hipStream_t);
+
+// r = b - q ; partial = r.r   (src/solvers.rs:33, :51)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub(T* __restrict__ r, const T* __restrict__ b, const T* __restrict__ q, int64_t n,
+                                             double* partial, const CgState* st) {
+    __shared__ double ws[4];
+    if (st && st->done) return;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T v = b[i] - q[i];
+        r[i] = v;
+        acc += (double)v * (double)v;
+    }
+    double s = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+template <typename T>
+void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s) {
+    int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_sub<T>, dim3(nb), dim3(256), 0, s, r, b, q, n, partial, st);
+    PRX_LAUNCH_CHECK();
+    *nblk = nb;
+}
+template void launch_sub<double>(double*, const double*, const double*, int64_t, double*, int*, const CgState*, hipStream_t);
+template void launch_sub<float>(float*, const float*, const float*, int64_t, double*, int*, const CgState*, hipStream_t);
+
+__global__ void k_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)(src[i] * scale);
+}
+void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cast_scale_d2f, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, dst, src, scale, n);
+    PRX_LAUNCH_CHECK();
+}
+__global__ void k_add_f2d(double* x, const float* e, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += (double)e[i];
+}
+void launch_add_f2d(double* x, const float* e, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_f2d, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, x, e, n);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ coarse level
+// RHS of build_system_at_depth (src/fem_tree.rs:386-402): the fine nodes of
+// one coarse cell are the 8^(D-d) consecutive Morton indices under it, summed
+// in FEM index order as (v.x + v.y) + v.z, then + wh3*0.5.
+__global__ void k_coarse_rhs(uint32_t D, uint32_t d, const double* __restrict__ nf, double add, int has_add, double* rhs) {
+    const uint64_t nc = 1ull << (3 * d), nfine = 1ull << (3 * D);
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t per = 1ull << (3 * (D - d));
+    double s = 0.0;
+    for (uint64_t k = 0; k < per; ++k) {
+        uint64_t i = c * per + k;
+        s += nf[i] + nf[nfine + i] + nf[2 * nfine + i];
+    }
+    if (has_add) s += add;
+    rhs[c] = s;
+}
+void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs, hipStream_t s) {
+    int64_t nc = (int64_t)1 << (3 * d);
+    hipLaunchKernelGGL(k_coarse_rhs, dim3(nblocks_for(nc)), dim3(256), 0, s, D, d, nf, add, add > 0.0 ? 1 : 0, rhs);
+    PRX_LAUNCH_CHECK();
+}
+
+// prolongate (src/fem_tree.rs:310-343) as a gather: fine node f receives
+// v_c * w from every coarse c with f = 2c + delta - 1, in increasing
+// row-major c (z, then y, then x), skipping v == 0.
+__global__ void k_prolongate(uint32_t D, uint32_t d, const double* __restrict__ coarse, double* fine) {
+    const uint32_t N = 1u << (3 * D);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t cres = 1u << d;
+    const double bw[3] = {0.5, 1.0, 0.5};
+    uint32_t f[3];
+    demorton3(i, f[0], f[1], f[2]);
+    uint32_t cand[3][2], delta[3][2];
+    int nc[3];
+    for (int a = 0; a < 3; ++a) {
+        nc[a] = 0;
+        if ((f[a] & 1u) == 0) {
+            uint32_t c = f[a] >> 1;
+            if (c < cres) { cand[a][nc[a]] = c; delta[a][nc[a]] = 1; nc[a]++; }
+        } else {
+            uint32_t c0 = (f[a] - 1) >> 1, c1 = (f[a] + 1) >> 1;
+            if (c0 < cres) { cand[a][nc[a]] = c0; delta[a][nc[a]] = 2; nc[a]++; }
+            if (c1 < cres) { cand[a][nc[a]] = c1; delta[a][nc[a]] = 0; nc[a]++; }
+        }
+    }
+    double acc = 0.0;
+    for (int iz = 0; iz < nc[2]; ++iz)
+        for (int iy = 0; iy < nc[1]; ++iy)
+            for (int ix = 0; ix < nc[0]; ++ix) {
+                double v = coarse[morton3(cand[0][ix], cand[1][iy], cand[2][iz])];
+                if (v == 0.0) continue;
+                double w = bw[delta[0][ix]] * bw[delta[1][iy]] * bw[delta[2][iz]];
+                acc += v * w;
+            }
+    fine[i] = acc;
+}
+void launch_prolongate(uint32_t D, uint32_t d, const double* coarse, double* fine, hipStream_t s) {
+    int64_t n = (int64_t)1 << (3 * D);
+    hipLaunchKernelGGL(k_prolongate, dim3(nblocks_for(n)), dim3(256), 0, s, D, d, coarse, fine);
+    PRX_LAUNCH_CHECK();
+}
+
+__global__ void k_m2r(uint32_t d, const double* src, double* dst, int dir) {
+    const uint32_t res = 1u << d;
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (uint64_t)res * res * res) return;
+    uint32_t x = r % res, y = (r / res) % res, z = r / ((uint64_t)res * res);
+    uint32_t m = morton3(x, y, z);
+    if (dir == 0) dst[r] = src[m];
+    else dst[m] = src[r];
+}
+void launch_morton_to_rowmajor(uint32_t d, const double* src, double* dst, hipStream_t s) {
+    int64_t n = (int64_t)1 << (3 * d);
+    hipLaunchKernelGGL(k_m2r, dim3(nblocks_for(n)), dim3(256), 0, s, d, src, dst, 0);
+    PRX_LAUNCH_CHECK();
+}
+void launch_rowmajor_to_morton(uint32_t d, const double* src, double* dst, hipStream_t s) {
+    int64_t n = (int64_t)1 << (3 * d);
+    hipLaunchKernelGGL(k_m2r, dim3(nblocks_for(n)), dim3(256), 0, s, d, src, dst, 1);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ evaluate
+// FEMTree::evaluate (src/fem_tree.rs:490-513) with bspline1 (:640-642) and
+// contributing_offsets (:645-649); loop order ox, oy, oz and zero-weight skips
+// as there, so the result is bit-identical for the same solution vector.
+__device__ inline double bspline1(double t) {
+    if (t < 0.0 || t > 2.0) return 0.0;
+    if (t <= 1.0) return t;
+    return 2.0 - t;
+}
+__device__ inline int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ double eval_field(uint32_t D, const double* __restrict__ sol, double px, double py, double pz) {
+    const int64_t res = (int64_t)1 << D;
+    const double h = 1.0 / (double)res;
+    const double s[3] = {px / h, py / h, pz / h};
+    int64_t o[3][2];
+    double b[3][2];
+    for (int k = 0; k < 3; ++k) {
+        int64_t f = sat_i64(floor(s[k]));
+        o[k][0] = clampi(wadd(f, -1), 0, res - 1);
+        o[k][1] = clampi(f, 0, res - 1);
+        b[k][0] = bspline1(s[k] - (double)o[k][0]);
+        b[k][1] = bspline1(s[k] - (double)o[k][1]);
+    }
+    double v = 0.0;
+    for (int a = 0; a < 2; ++a) {
+        if (b[0][a] == 0.0) continue;
+        for (int c = 0; c < 2; ++c) {
+            if (b[1][c] == 0.0) continue;
+            for (int e = 0; e < 2; ++e) {
+                if (b[2][e] == 0.0) continue;
+                uint32_t idx = morton3((uint32_t)o[0][a], (uint32_t)o[1][c], (uint32_t)o[2][e]);
+                v += sol[idx] * b[0][a] * b[1][c] * b[2][e];
+            }
+        }
+    }
+    return v;
+}
+
+__global__ void k_evaluate(uint32_t D, const double* __restrict__ sol, const double* __restrict__ q, int64_t m, double* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    out[i] = eval_field(D, sol, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+}
+void launch_evaluate(uint32_t D, const double* sol, const double* q, int64_t m, double* out, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_evaluate, dim3(nblocks_for(m)), dim3(256), 0, s, D, sol, q, m, out);
+    PRX_LAUNCH_CHECK();
+}
+__global__ void k_evaluate_soa(uint32_t D, const double* __restrict__ sol, const double* __restrict__ pos, int64_t P, double* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    out[i] = eval_field(D, sol, pos[i], pos[P + i], pos[2 * P + i]);
+}
+void launch_evaluate_pointsSoA(uint32_t D, const double* sol, const double* pos, int64_t P, double* out, hipStream_t s) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_evaluate_soa, dim3(nblocks_for(P)), dim3(256), 0, s, D, sol, pos, P, out);
+    PRX_LAUNCH_CHECK();
+}
+
+// Corner positions are (off + c) * w with w = 1/res (src/fem_tree.rs:596-602).
+__global__ void k_evaluate_corners(uint32_t D, const double* __restrict__ sol, double* out) {
+    const uint64_t r1 = (1ull << D) + 1;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r1 * r1 * r1) return;
+    const uint64_t kx = i % r1, ky = (i / r1) % r1, kz = i / (r1 * r1);
+    const double w = 1.0 / (double)(1ull << D);
+    out[i] = eval_field(D, sol, (double)kx * w, (double)ky * w, (double)kz * w);
+}
+void launch_evaluate_corners(uint32_t D, const double* sol, double* out, hipStream_t s) {
+    int64_t r1 = ((int64_t)1 << D) + 1;
+    hipLaunchKernelGGL(k_evaluate_corners, dim3(nblocks_for(r1 * r1 * r1)), dim3(256), 0, s, D, sol, out);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ generic CSR
+// multiply_vector (src/sparse.rs:105-117): one row per thread, summed from 0.0
+// in stored column order.
+__global__ void k_csr_spmv(int64_t n, const int64_t* __restrict__ off, const int64_t* __restrict__ col,
+                           const double* __restrict__ val, const double* __restrict__ x, double* y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t k = off[i]; k < off[i + 1]; ++k) s = s + val[k] * x[col[k]];
+    y[i] = s;
+}
+void launch_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* x, double* y, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_csr_spmv, dim3(nblocks_for(n)), dim3(256), 0, s, n, off, col, val, x, y);
+    PRX_LAUNCH_CHECK();
+}
+
+}  // namespace prx

@@ -1,0 +1,506 @@
+// Point-driven kernels: per-node point lists, normal splatting, RHS assembly,
+// the per-point screening gather of the operator, and the verification-mode
+// CSR export.
+//
+// Splat, RHS and CSR values reproduce the reference bit for bit: every node
+// sums its contributions sequentially in input point order, with the
+// reference's expression order, and contraction to FMA is off in this file.
+#pragma clang fp contract(off)
+
+#include <hipcub/hipcub.hpp>
+
+#include "kernels.hpp"
+
+namespace prx {
+
+static inline int nblk(int64_t n, int threads = 256) {
+    int64_t b = (n + threads - 1) / threads;
+    return (int)(b < 1 ? 1 : b);
+}
+
+// Floor cell for the splat (src/fem_tree.rs:138-143): 64-bit cell, corner
+// offsets compared as usize.  Trunc cell for screening (src/fem_tree.rs:207-213):
+// `as isize` then `(c + d) as u32`.
+__device__ inline int64_t cell_floor(double p, double h) { return sat_i64(floor(p / h)); }
+__device__ inline int64_t cell_trunc(double p, double h) { return sat_i64(p / h); }
+
+__global__ void k_point_keys(uint32_t D, const double* __restrict__ pos, int64_t P, uint32_t* key_f, uint32_t* key_t,
+                             uint32_t* val, uint32_t* cell_key, uint32_t* cell_val) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t res = 1u << D, N = 1u << (3 * D);
+    const double h = 1.0 / (double)res;
+    const double q[3] = {pos[p], pos[P + p], pos[2 * P + p]};
+    int64_t cf[3], ct[3];
+    for (int a = 0; a < 3; ++a) { cf[a] = cell_floor(q[a], h); ct[a] = cell_trunc(q[a], h); }
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        uint64_t fx = (uint64_t)wadd(cf[0], dx), fy = (uint64_t)wadd(cf[1], dy), fz = (uint64_t)wadd(cf[2], dz);
+        key_f[8 * p + k] = (fx < res && fy < res && fz < res) ? morton3((uint32_t)fx, (uint32_t)fy, (uint32_t)fz) : N;
+        uint32_t tx = (uint32_t)(uint64_t)wadd(ct[0], dx), ty = (uint32_t)(uint64_t)wadd(ct[1], dy), tz = (uint32_t)(uint64_t)wadd(ct[2], dz);
+        key_t[8 * p + k] = (tx < res && ty < res && tz < res) ? morton3(tx, ty, tz) : N;
+        val[8 * p + k] = (uint32_t)(8 * p + k);
+    }
+    uint32_t c[3];
+    for (int a = 0; a < 3; ++a) c[a] = (uint32_t)(ct[a] < 0 ? 0 : (ct[a] >= (int64_t)res ? res - 1 : ct[a]));
+    cell_key[p] = morton3(c[0], c[1], c[2]);
+    cell_val[p] = (uint32_t)p;
+}
+void launch_point_keys(uint32_t D, const double* pos, int64_t P, uint32_t* key_f, uint32_t* key_t, uint32_t* val,
+                       uint32_t* cell_key, uint32_t* cell_val, hipStream_t s) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_point_keys, dim3(nblk(P)), dim3(256), 0, s, D, pos, P, key_f, key_t, val, cell_key, cell_val);
+    PRX_LAUNCH_CHECK();
+}
+
+__global__ void k_point_sorted(uint32_t D, const double* __restrict__ pos, int64_t P, const uint32_t* __restrict__ perm,
+                               uint32_t* cell, double* t64, float* t32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const double h = 1.0 / (double)(1u << D);
+    const uint32_t p = perm[i];
+    for (int a = 0; a < 3; ++a) {
+        double q = pos[a * P + p];
+        int64_t c = cell_trunc(q, h);
+        double t = q / h - (double)c;
+        cell[a * P + i] = (uint32_t)(uint64_t)c;
+        t64[a * P + i] = t;
+        t32[a * P + i] = (float)t;
+    }
+}
+void launch_point_sorted_data(uint32_t D, const double* pos, int64_t P, const uint32_t* perm, uint32_t* cell, double* t64,
+                              float* t32, hipStream_t s) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_point_sorted, dim3(nblk(P)), dim3(256), 0, s, D, pos, P, perm, cell, t64, t32);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ node lists
+__global__ void k_count_valid(const uint32_t* key, int64_t M8, uint32_t N, uint32_t* out_M) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M8) return;
+    if (key[j] < N && (j + 1 == M8 || key[j + 1] >= N)) *out_M = (uint32_t)(j + 1);
+}
+__global__ void k_heads(const uint32_t* key, int64_t M, uint32_t* head) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    head[j] = (j == 0 || key[j] != key[j - 1]) ? 1u : 0u;
+}
+__global__ void k_scatter_heads(const uint32_t* key, const uint32_t* head, const uint32_t* pos, int64_t M, uint32_t* node,
+                                uint32_t* off) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    if (head[j]) {
+        node[pos[j]] = key[j];
+        off[pos[j]] = (uint32_t)j;
+    }
+}
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// weight of corner k of point p, (bx*by)*bz as in src/fem_tree.rs:155-158 / :215-218
+__device__ inline double corner_weight(uint32_t D, const double* pos, int64_t P, uint32_t p, int k, int mode) {
+    const double h = 1.0 / (double)(1u << D);
+    double b[3];
+    for (int a = 0; a < 3; ++a) {
+        double q = pos[a * P + p];
+        int64_t c = mode == 0 ? cell_floor(q, h) : cell_trunc(q, h);
+        double t = q / h - (double)c;
+        b[a] = ((k >> a) & 1) ? t : 1.0 - t;
+    }
+    return b[0] * b[1] * b[2];
+}
+
+__global__ void k_list_entries(uint32_t D, int mode, const uint32_t* code, int64_t M, const double* __restrict__ pos, int64_t P,
+                               const uint32_t* __restrict__ inv, uint32_t* sp, double* phi, float* phi32) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const uint32_t c = code[j], p = c >> 3;
+    const int k = c & 7;
+    double w = corner_weight(D, pos, P, p, k, mode);
+    phi[j] = w;
+    phi32[j] = (float)w;
+    sp[j] = inv[p];
+}
+__global__ void k_list_diag(const uint32_t* off, int64_t T, const double* phi, double* diag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    double s = 0.0;
+    for (uint32_t j = off[t]; j < off[t + 1]; ++j) s += phi[j] * phi[j];
+    diag[t] = s;
+}
+// brick[b] = first touched node with index >= b << log
+__global__ void k_brick_ranges(const uint32_t* node, int64_t T, int64_t nb, int lg, uint32_t* brick) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    const uint64_t v = (uint64_t)b << lg;
+    int64_t lo = 0, hi = T;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)node[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    brick[b] = (uint32_t)lo;
+}
+__global__ void k_invert_perm(const uint32_t* perm, int64_t P, uint32_t* inv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    inv[perm[i]] = (uint32_t)i;
+}
+
+void launch_invert_perm(const uint32_t* perm, int64_t P, uint32_t* inv, hipStream_t s) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_invert_perm, dim3(nblk(P)), dim3(256), 0, s, perm, P, inv);
+    PRX_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void dmalloc(T** p, int64_t n) {
+    PRX_HIP(hipMalloc((void**)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+}
+
+void build_node_list(NodeList& L, uint32_t D, const uint32_t* key, const uint32_t* val, int64_t M8, const DevPoints& pts,
+                     const uint32_t* inv, void* scratch, size_t scratch_bytes, hipStream_t s) {
+    const uint32_t N = 1u << (3 * D);
+    uint32_t* dM;
+    PRX_HIP(hipMallocAsync((void**)&dM, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, dM, 0u);
+    if (M8 > 0) hipLaunchKernelGGL(k_count_valid, dim3(nblk(M8)), dim3(256), 0, s, key, M8, N, dM);
+    PRX_LAUNCH_CHECK();
+    uint32_t hM = 0;
+    PRX_HIP(hipMemcpyAsync(&hM, dM, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipStreamSynchronize(s));
+    const int64_t M = hM;
+    L.M = M;
+    uint32_t *head, *pos;
+    PRX_HIP(hipMallocAsync((void**)&head, sizeof(uint32_t) * (M + 1), s));
+    PRX_HIP(hipMallocAsync((void**)&pos, sizeof(uint32_t) * (M + 1), s));
+    int64_t T = 0;
+    if (M > 0) {
+        hipLaunchKernelGGL(k_heads, dim3(nblk(M)), dim3(256), 0, s, key, M, head);
+        size_t need = 0;
+        PRX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, head, pos, (int)M, s));
+        if (need > scratch_bytes) throw Error(PRX_ERR_DEVICE, "scan scratch too small");
+        PRX_HIP(hipcub::DeviceScan::ExclusiveSum(scratch, need, head, pos, (int)M, s));
+        uint32_t lastpos = 0, lasthead = 0;
+        PRX_HIP(hipMemcpyAsync(&lastpos, pos + M - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipMemcpyAsync(&lasthead, head + M - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+        T = (int64_t)lastpos + lasthead;
+    }
+    L.T = T;
+    dmalloc(&L.node, T);
+    dmalloc(&L.off, T + 1);
+    dmalloc(&L.code, M);
+    dmalloc(&L.sp, M);
+    dmalloc(&L.phi, M);
+    dmalloc(&L.phi32, M);
+    dmalloc(&L.diag, T);
+    if (M > 0) {
+        hipLaunchKernelGGL(k_scatter_heads, dim3(nblk(M)), dim3(256), 0, s, key, head, pos, M, L.node, L.off);
+        PRX_HIP(hipMemcpyAsync(L.code, val, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_list_entries, dim3(nblk(M)), dim3(256), 0, s, D, L.mode, L.code, M, pts.pos, pts.P, inv, L.sp,
+                           L.phi, L.phi32);
+    }
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, L.off + T, (uint32_t)M);
+    if (T > 0) hipLaunchKernelGGL(k_list_diag, dim3(nblk(T)), dim3(256), 0, s, L.off, T, L.phi, L.diag);
+    L.brick_log = D >= 4 ? 12 : 3 * (int)D;
+    const int64_t nb = (int64_t)N >> L.brick_log;
+    dmalloc(&L.brick, nb + 1);
+    hipLaunchKernelGGL(k_brick_ranges, dim3(nblk(nb + 1)), dim3(256), 0, s, L.node, T, nb, L.brick_log, L.brick);
+    if (D >= 3) {
+        const int64_t nb8 = (int64_t)N >> 9;
+        dmalloc(&L.brick8, nb8 + 1);
+        hipLaunchKernelGGL(k_brick_ranges, dim3(nblk(nb8 + 1)), dim3(256), 0, s, L.node, T, nb8, 9, L.brick8);
+    }
+    PRX_LAUNCH_CHECK();
+    PRX_HIP(hipFreeAsync(head, s));
+    PRX_HIP(hipFreeAsync(pos, s));
+    PRX_HIP(hipFreeAsync(dM, s));
+}
+
+// ------------------------------------------------------------------ splat
+// splat_normal_field (src/fem_tree.rs:130-178): per touched node, sum
+// n*w and w over its points in input order, then nf = -nf * (1/w).
+__global__ void k_splat(int64_t T, const uint32_t* __restrict__ node, const uint32_t* __restrict__ off,
+                        const uint32_t* __restrict__ code, const double* __restrict__ phi, const double* __restrict__ nrm,
+                        int64_t P, int64_t N, double* nf, double* w) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    for (uint32_t j = off[t]; j < off[t + 1]; ++j) {
+        const uint32_t p = code[j] >> 3;
+        const double ww = phi[j];
+        sx += nrm[p] * ww;
+        sy += nrm[P + p] * ww;
+        sz += nrm[2 * P + p] * ww;
+        sw += ww;
+    }
+    if (sw > 0.0) {
+        double inv = 1.0 / sw;
+        sx = -sx * inv;
+        sy = -sy * inv;
+        sz = -sz * inv;
+    }
+    const uint32_t i = node[t];
+    nf[i] = sx;
+    nf[N + i] = sy;
+    nf[2 * N + i] = sz;
+    w[i] = sw;
+}
+void launch_splat(uint32_t D, const NodeList& L, const DevPoints& pts, double* nf, double* w, hipStream_t s) {
+    const int64_t N = (int64_t)1 << (3 * D);
+    launch_fill<double>(nf, 0.0, 3 * N, s);
+    launch_fill<double>(w, 0.0, N, s);
+    if (L.T <= 0) return;
+    hipLaunchKernelGGL(k_splat, dim3(nblk(L.T)), dim3(256), 0, s, L.T, L.node, L.off, L.code, L.phi, pts.nrm, pts.P, N, nf, w);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ RHS
+__device__ inline double mass_1d(int d) { return d == 0 ? 2. / 3. : 1. / 6.; }
+__device__ inline double deriv_mass_1d(int d) { return d == 0 ? 0. : (d == 1 ? -0.5 : 0.5); }
+
+// assemble_rhs Laplacian part (src/fem_tree.rs:258-274): neighbours in
+// dx, dy, dz loop order, h*h*(v.x G M M + v.y M G M + v.z M M G).
+__global__ void k_rhs_laplacian(uint32_t D, const double* __restrict__ nf, double* rhs) {
+    const uint32_t res = 1u << D;
+    const uint64_t N = 1ull << (3 * D);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double h = 1.0 / (double)res;
+    uint32_t ix, iy, iz;
+    demorton3((uint32_t)i, ix, iy, iz);
+    double sum = 0.0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                int64_t jx = (int64_t)ix + dx, jy = (int64_t)iy + dy, jz = (int64_t)iz + dz;
+                if (jx < 0 || jx >= res || jy < 0 || jy >= res || jz < 0 || jz >= res) continue;
+                const uint32_t j = morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
+                const double vx = nf[j], vy = nf[N + j], vz = nf[2 * N + j];
+                sum += h * h * (vx * deriv_mass_1d(dx) * mass_1d(dy) * mass_1d(dz) + vy * mass_1d(dx) * deriv_mass_1d(dy) * mass_1d(dz) +
+                                vz * mass_1d(dx) * mass_1d(dy) * deriv_mass_1d(dz));
+            }
+    rhs[i] = sum;
+}
+void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s) {
+    const int64_t N = (int64_t)1 << (3 * D);
+    hipLaunchKernelGGL(k_rhs_laplacian, dim3(nblk(N)), dim3(256), 0, s, D, nf, rhs);
+    PRX_LAUNCH_CHECK();
+}
+
+// Screening RHS (src/fem_tree.rs:277-297): rhs += wh3*0.5*bx*by*bz per point,
+// in input order.
+__global__ void k_rhs_screening(uint32_t D, int64_t T, const uint32_t* __restrict__ node, const uint32_t* __restrict__ off,
+                                const uint32_t* __restrict__ code, const double* __restrict__ pos, int64_t P, double wh3,
+                                double* rhs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const double h = 1.0 / (double)(1u << D);
+    const double target = 0.5;
+    const uint32_t i = node[t];
+    double acc = rhs[i];
+    for (uint32_t j = off[t]; j < off[t + 1]; ++j) {
+        const uint32_t c = code[j], p = c >> 3;
+        const int k = c & 7;
+        double b[3];
+        for (int a = 0; a < 3; ++a) {
+            double q = pos[a * P + p];
+            int64_t cc = cell_trunc(q, h);
+            double tt = q / h - (double)cc;
+            b[a] = ((k >> a) & 1) ? tt : 1. - tt;
+        }
+        acc += wh3 * target * b[0] * b[1] * b[2];
+    }
+    rhs[i] = acc;
+}
+void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, double wh3, double* rhs, hipStream_t s) {
+    if (L.T <= 0) return;
+    hipLaunchKernelGGL(k_rhs_screening, dim3(nblk(L.T)), dim3(256), 0, s, D, L.T, L.node, L.off, L.code, pts.pos, pts.P, wh3, rhs);
+    PRX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ screening gather
+// s_p = wh3 * sum_k phi_k x(node_k) over the point's in-range trunc corners,
+// per point in sorted order.  The operator then adds sum phi * s_p at every
+// touched node: together they apply sum_p wh3 phi_p phi_p^T
+// (src/fem_tree.rs:206-226).
+template <typename T, int MODE>  // MODE 0: x ; MODE 1: x64 + e32
+__global__ void k_screen_gather(uint32_t D, int64_t P, const uint32_t* __restrict__ cell, const T* __restrict__ tt,
+                                const T* __restrict__ x, const double* __restrict__ x64, const float* __restrict__ e32, T wh3,
+                                T* s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const uint32_t res = 1u << D;
+    const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
+    const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
+    T acc = 0;
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t ox = c[0] + (k & 1), oy = c[1] + ((k >> 1) & 1), oz = c[2] + (k >> 2);
+        if (ox >= res || oy >= res || oz >= res) continue;
+        const T bx = (k & 1) ? t[0] : T(1) - t[0];
+        const T by = ((k >> 1) & 1) ? t[1] : T(1) - t[1];
+        const T bz = (k >> 2) ? t[2] : T(1) - t[2];
+        const uint32_t m = morton3(ox, oy, oz);
+        T v;
+        if (MODE == 0) v = x[m];
+        else v = (T)(x64[m] + (double)e32[m]);
+        acc += bx * by * bz * v;
+    }
+    s[i] = wh3 * acc;
+}
+template <typename T>
+void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s) {
+    const DevPoints& pts = *L.pts;
+    if (pts.P <= 0) return;
+    const T* tt;
+    T wh3;
+    if constexpr (sizeof(T) == 8) { tt = (const T*)pts.t64; wh3 = (T)L.k64.wh3; }
+    else { tt = (const T*)pts.t32; wh3 = (T)L.k32.wh3; }
+    if (x64)
+        hipLaunchKernelGGL((k_screen_gather<T, 1>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf);
+    else
+        hipLaunchKernelGGL((k_screen_gather<T, 0>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_screen_gather<double>(const LevelView&, const double*, const double*, const float*, double*, hipStream_t);
+template void launch_screen_gather<float>(const LevelView&, const float*, const double*, const float*, float*, hipStream_t);
+
+// ------------------------------------------------------------------ verification CSR
+// The CSR of assemble_system (src/fem_tree.rs:180-249) as the reference
+// stores it: columns = Laplacian neighbours with K != 0, the diagonal when
+// pw > 0, and every node that shares a point with the row node (the 64
+// screening pairs are pushed without a zero filter); sorted by column.
+__device__ inline int64_t find_touched(const NodeList& L, uint32_t i) {
+    if (L.T == 0) return -1;
+    const uint32_t b = i >> L.brick_log;
+    int64_t lo = L.brick[b], hi = L.brick[b + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        uint32_t v = L.node[mid];
+        if (v == i) return mid;
+        if (v < i) lo = mid + 1;
+        else hi = mid;
+    }
+    return -1;
+}
+
+__device__ inline uint32_t row_mask(uint32_t D, uint32_t i, const NodeList& L, const DevPoints& pts, int has_diag) {
+    const int64_t res = 1 << D;
+    uint32_t ix, iy, iz;
+    demorton3(i, ix, iy, iz);
+    uint32_t mask = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                int nz = (dx != 0) + (dy != 0) + (dz != 0);
+                int64_t jx = (int64_t)ix + dx, jy = (int64_t)iy + dy, jz = (int64_t)iz + dz;
+                if (jx < 0 || jx >= res || jy < 0 || jy >= res || jz < 0 || jz >= res) continue;
+                int bit = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                if (nz != 1) mask |= 1u << bit;  // K != 0 for centre, edges, corners
+            }
+    if (has_diag) mask |= 1u << 13;
+    const int64_t t = find_touched(L, i);
+    if (t >= 0) {
+        for (uint32_t j = L.off[t]; j < L.off[t + 1]; ++j) {
+            const uint32_t p = L.code[j] >> 3, k = L.code[j] & 7;
+            const int ex = k & 1, ey = (k >> 1) & 1, ez = k >> 2;
+            // the point's cell corner c = i - (ex,ey,ez); partners c + (fx,fy,fz)
+            for (int f = 0; f < 8; ++f) {
+                const int fx = f & 1, fy = (f >> 1) & 1, fz = f >> 2;
+                // the partner exists iff its trunc corner is in range
+                const double h = 1.0 / (double)res;
+                int64_t c[3];
+                for (int a = 0; a < 3; ++a) c[a] = sat_i64(pts.pos[a * pts.P + p] / h);
+                uint32_t ox = (uint32_t)(uint64_t)wadd(c[0], fx), oy = (uint32_t)(uint64_t)wadd(c[1], fy),
+                         oz = (uint32_t)(uint64_t)wadd(c[2], fz);
+                if (ox >= (uint32_t)res || oy >= (uint32_t)res || oz >= (uint32_t)res) continue;
+                int bx = fx - ex, by = fy - ey, bz = fz - ez;
+                mask |= 1u << ((bz + 1) * 9 + (by + 1) * 3 + (bx + 1));
+            }
+        }
+    }
+    return mask;
+}
+
+__global__ void k_csr_count(uint32_t D, NodeList L, DevPoints pts, int has_diag, int64_t* cnt) {
+    const uint64_t N = 1ull << (3 * D);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    cnt[i] = __popc(row_mask(D, (uint32_t)i, L, pts, has_diag));
+}
+
+__global__ void k_csr_fill(uint32_t D, NodeList L, DevPoints pts, int has_diag, double K0, double Ke, double Kc,
+                           double wg, double wh3, const int64_t* off, int64_t* col, double* val) {
+    const uint64_t N = 1ull << (3 * D);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t res = 1 << D;
+    const double h = 1.0 / (double)res;
+    uint32_t mask = row_mask(D, (uint32_t)i, L, pts, has_diag);
+    uint32_t ix, iy, iz;
+    demorton3((uint32_t)i, ix, iy, iz);
+    uint32_t cols[27];
+    int bits[27];
+    int n = 0;
+    for (int b = 0; b < 27; ++b) {
+        if (!(mask & (1u << b))) continue;
+        int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
+        uint32_t j = morton3(ix + dx, iy + dy, iz + dz);
+        int k = n++;
+        while (k > 0 && cols[k - 1] > j) { cols[k] = cols[k - 1]; bits[k] = bits[k - 1]; --k; }
+        cols[k] = j;
+        bits[k] = b;
+    }
+    const int64_t t = find_touched(L, (uint32_t)i);
+    const int64_t base = off[i];
+    for (int e = 0; e < n; ++e) {
+        const int b = bits[e];
+        const int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
+        const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+        double v = 0.0;
+        if (nz != 1) v += (nz == 0 ? K0 : (nz == 2 ? Ke : Kc));
+        if (nz == 0 && has_diag) v += wg;
+        if (t >= 0) {
+            for (uint32_t j = L.off[t]; j < L.off[t + 1]; ++j) {
+                const uint32_t p = L.code[j] >> 3, k = L.code[j] & 7;
+                const int ex = k & 1, ey = (k >> 1) & 1, ez = k >> 2;
+                const int fx = ex + dx, fy = ey + dy, fz = ez + dz;
+                if (fx < 0 || fx > 1 || fy < 0 || fy > 1 || fz < 0 || fz > 1) continue;
+                int64_t c[3];
+                double tt[3];
+                for (int a = 0; a < 3; ++a) {
+                    double q = pts.pos[a * pts.P + p];
+                    c[a] = sat_i64(q / h);
+                    tt[a] = q / h - (double)c[a];
+                }
+                uint32_t ox = (uint32_t)(uint64_t)wadd(c[0], fx), oy = (uint32_t)(uint64_t)wadd(c[1], fy),
+                         oz = (uint32_t)(uint64_t)wadd(c[2], fz);
+                if (ox >= (uint32_t)res || oy >= (uint32_t)res || oz >= (uint32_t)res) continue;
+                const double bxj = fx ? tt[0] : 1. - tt[0], byj = fy ? tt[1] : 1. - tt[1], bzj = fz ? tt[2] : 1. - tt[2];
+                const double phij = bxj * byj * bzj;
+                v += wh3 * L.phi[j] * phij;
+            }
+        }
+        col[base + e] = cols[e];
+        val[base + e] = v;
+    }
+}
+
+void launch_csr_count(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, int64_t* rowcnt, hipStream_t s) {
+    const int64_t N = (int64_t)1 << (3 * D);
+    NodeList Lv = L ? *L : NodeList{};
+    DevPoints Pv = pts ? *pts : DevPoints{};
+    hipLaunchKernelGGL(k_csr_count, dim3(nblk(N)), dim3(256), 0, s, D, Lv, Pv, has_diag, rowcnt);
+    PRX_LAUNCH_CHECK();
+}
+void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, double K0, double Ke, double Kc,
+                     double wg, double wh3, const int64_t* off, int64_t* col, double* val, hipStream_t s) {
+    const int64_t N = (int64_t)1 << (3 * D);
+    NodeList Lv = L ? *L : NodeList{};
+    DevPoints Pv = pts ? *pts : DevPoints{};
+    hipLaunchKernelGGL(k_csr_fill, dim3(nblk(N)), dim3(256), 0, s, D, Lv, Pv, has_diag, K0, Ke, Kc, wg, wh3, off, col, val);
+    PRX_LAUNCH_CHECK();
+}
+
+}  // namespace prx

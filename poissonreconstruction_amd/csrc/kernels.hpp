@@ -1,0 +1,110 @@
+// Host-side launchers of the gfx950 kernels, grouped by translation unit.
+#pragma once
+#include "prx_internal.hpp"
+
+namespace prx {
+
+// ------------------------------------------------------------------ device state of one CG solve
+// Scalars stay on the device so an iteration needs no host round-trip; every
+// kernel of an iteration returns at once when `done` is set.
+struct CgState {
+    double delta, delta0, dq, alpha, beta, thr, eps;
+    double bnorm2;
+    double slot[8];
+    int64_t it, iters, max_iters;
+    int32_t done, status, refresh_due, pad;
+};
+
+enum FinOp { FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4 };
+
+// Points on the device.  Input order: positions/normals (SoA, FP64).  Sorted
+// order (by the Morton code of the screening cell): cell corner + weights.
+struct DevPoints {
+    int64_t P = 0;
+    double* pos = nullptr;     // [3P] SoA px|py|pz, input order
+    double* nrm = nullptr;     // [3P] SoA
+    uint32_t* perm = nullptr;  // [P] sorted -> input
+    uint32_t* cell = nullptr;  // [3P] SoA trunc cell as u32 (wrapping `as u32`, src/fem_tree.rs:212)
+    double* t64 = nullptr;     // [3P] SoA fractional offsets s - trunc(s)
+    float* t32 = nullptr;      // [3P]
+};
+
+// One grid level as the operator kernels see it.
+struct LevelView {
+    uint32_t D = 0, res = 1;
+    int64_t N = 1;
+    Coef<double> k64{};
+    Coef<float> k32{};
+    const NodeList* scr = nullptr;  // fine level with pw > 0 only
+    const DevPoints* pts = nullptr;
+};
+
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2 };
+
+// ---- k_grid.hip
+void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
+void launch_export_g2i(uint32_t D, int64_t* g2i, hipStream_t s);
+template <typename T> void launch_fill(T* p, T v, int64_t n, hipStream_t s);
+template <typename T> void launch_dot(const T* a, const T* b, int64_t n, double* partial, int* nblk, hipStream_t s);
+void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s);
+template <typename T> void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s);
+template <typename T> void launch_axpy_alpha(T* x, const T* d, const CgState* st, int64_t n, hipStream_t s);
+template <typename T> void launch_xpby_beta(T* d, const T* r, const CgState* st, int64_t n, hipStream_t s);
+template <typename T> void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s);
+void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s);
+void launch_add_f2d(double* x, const float* e, int64_t n, hipStream_t s);
+void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs_morton, hipStream_t s);
+void launch_prolongate(uint32_t D, uint32_t d, const double* coarse_morton, double* fine, hipStream_t s);
+void launch_morton_to_rowmajor(uint32_t d, const double* src, double* dst, hipStream_t s);
+void launch_rowmajor_to_morton(uint32_t d, const double* src, double* dst, hipStream_t s);
+void launch_evaluate(uint32_t D, const double* sol, const double* q, int64_t m, double* out, hipStream_t s);
+void launch_evaluate_pointsSoA(uint32_t D, const double* sol, const double* pos, int64_t P, double* out, hipStream_t s);
+void launch_evaluate_corners(uint32_t D, const double* sol, double* out, hipStream_t s);
+void launch_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* x, double* y, hipStream_t s);
+
+// ---- k_points.hip
+// Per point (input order): node keys of the 8 corners for the floor (splat)
+// and trunc (screening) cells; invalid corners get key N.
+void launch_point_keys(uint32_t D, const double* pos, int64_t P, uint32_t* key_floor, uint32_t* key_trunc, uint32_t* val,
+                       uint32_t* cell_key, uint32_t* cell_val, hipStream_t s);
+void launch_point_sorted_data(uint32_t D, const double* pos, int64_t P, const uint32_t* perm, uint32_t* cell, double* t64,
+                              float* t32, hipStream_t s);
+// Run-length encode the sorted keys into a NodeList (node/off), fill code,
+// phi and sp; allocates the NodeList arrays.
+void build_node_list(NodeList& L, uint32_t D, const uint32_t* sorted_key, const uint32_t* sorted_val, int64_t M8,
+                     const DevPoints& pts, const uint32_t* inv_perm, void* scratch, size_t scratch_bytes, hipStream_t s);
+void launch_invert_perm(const uint32_t* perm, int64_t P, uint32_t* inv, hipStream_t s);
+void launch_splat(uint32_t D, const NodeList& L, const DevPoints& pts, double* nf, double* w, hipStream_t s);
+void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s);
+void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, double wh3, double* rhs, hipStream_t s);
+template <typename T>
+void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s);
+void launch_csr_count(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, int64_t* rowcnt, hipStream_t s);
+void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, double K0, double Ke, double Kc,
+                     double wg, double wh3, const int64_t* off, int64_t* col, double* val, hipStream_t s);
+
+// ---- k_stencil.hip
+// One operator pass on level L.  OP_APPLY: y = A x, partial = x.y.
+// OP_RESID: y = b - A x, partial = |y|^2.  OP_RESID_UPD (T = double): x64 +=
+// e32 (written back), r32 = (b - A x64) * rscale, partial = |b - A x64|^2.
+template <typename T>
+struct StencilIO {
+    const T* x = nullptr;
+    const T* b = nullptr;
+    T* y = nullptr;
+    double* x64 = nullptr;
+    const float* e32 = nullptr;
+    float* r32 = nullptr;
+    double rscale = 1.0;
+    const T* s = nullptr;  // per-point screening values wh3*sum(phi*x)
+    double* partial = nullptr;
+    const int32_t* done = nullptr;
+};
+template <typename T>
+void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);
+template <typename T>
+void launch_gs_color(const LevelView& L, int color, T* x, const T* b, const T* sbuf, hipStream_t s);
+
+int partial_capacity(int64_t N);
+
+}  // namespace prx

@@ -1,0 +1,1008 @@
+// Host side of libprx: the FEMTree handle (device buffers, stream, events),
+// the cascadic solve with its CG / multicolour-GS / mixed-precision
+// refinement drivers, and the extern "C" boundary declared in include/prx.h.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "kernels.hpp"
+
+namespace prx {
+
+// ------------------------------------------------------------------ device arrays
+template <typename T>
+struct DArr {
+    T* p = nullptr;
+    int64_t n = 0;
+    DArr() = default;
+    DArr(const DArr&) = delete;
+    DArr& operator=(const DArr&) = delete;
+    ~DArr() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(int64_t cnt) {
+        if (cnt <= n && p) return;
+        release();
+        PRX_HIP(hipMalloc((void**)&p, (size_t)(cnt > 0 ? cnt : 1) * sizeof(T)));
+        n = cnt;
+    }
+};
+
+static void free_list(NodeList& L) {
+    void* ptrs[] = {L.node, L.off, L.code, L.sp, L.phi, L.phi32, L.diag, L.brick, L.brick8};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    int mode = L.mode;
+    L = NodeList{};
+    L.mode = mode;
+}
+
+// Reference stencil constants (src/fem_tree.rs:39-48), same expression order.
+static double mass_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. / 3. : (d == 1 ? 1. / 6. : 0.0); }
+static double stiff_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. : (d == 1 ? -1. : 0.0); }
+static double stiffness_entry(double h, int dx, int dy, int dz) {
+    return (stiff_1d(dx) * mass_1d(dy) * mass_1d(dz) + mass_1d(dx) * stiff_1d(dy) * mass_1d(dz) +
+            mass_1d(dx) * mass_1d(dy) * stiff_1d(dz)) * h;
+}
+
+struct Timer {
+    hipEvent_t a{}, b{};
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        PRX_HIP(hipEventCreate(&a));
+        PRX_HIP(hipEventCreate(&b));
+        PRX_HIP(hipEventRecord(a, s));
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    double ms() {
+        PRX_HIP(hipEventRecord(b, s));
+        PRX_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        PRX_HIP(hipEventElapsedTime(&t, a, b));
+        return (double)t;
+    }
+};
+
+struct CgOut {
+    uint64_t iters = 0;
+    double rel = 0.0;
+    int status = PRX_OK;
+    uint64_t applies = 0;
+};
+
+}  // namespace prx
+
+using namespace prx;
+
+// ------------------------------------------------------------------ the handle
+struct prx_tree {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t D = 0, res = 1;
+    int64_t N = 1;
+    double h = 1.0;
+    int tile = 16;
+
+    // points
+    int64_t P = 0;
+    DArr<double> pos, nrm, t64;
+    DArr<uint32_t> perm, cell;
+    DArr<float> t32;
+    DevPoints pts;
+    NodeList floor_list, trunc_list;
+    bool splatted = false, assembled = false;
+
+    // fields
+    DArr<double> nf, w, b, x;
+    double pw = 0.0;
+    Coef<double> k64{};
+    Coef<float> k32{};
+    double K0 = 0, Ke = 0, Kc = 0, wh3 = 0, wg = 0;
+
+    // solver workspace
+    DArr<double> r64, d64, q64, s64, partial, tmp64;
+    DArr<float> e32, r32, ri32, d32, q32, s32;
+    CgState* st = nullptr;
+    CgState* st_host = nullptr;
+
+    // coarse level (Morton order inside)
+    uint32_t cd = 0;
+    DArr<double> cb, cx, cr, cdd, cq, cpart;
+
+    prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; }
+    ~prx_tree() {
+        free_list(floor_list);
+        free_list(trunc_list);
+        if (st) (void)hipFree(st);
+        if (st_host) (void)hipHostFree(st_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    LevelView fine_view() const {
+        LevelView v;
+        v.D = D;
+        v.res = res;
+        v.N = N;
+        v.k64 = k64;
+        v.k32 = k32;
+        v.scr = (pw > 0.0 && trunc_list.T > 0) ? &trunc_list : nullptr;
+        v.pts = &pts;
+        return v;
+    }
+    void ensure_state() {
+        if (!st) PRX_HIP(hipMalloc((void**)&st, sizeof(CgState)));
+        if (!st_host) PRX_HIP(hipHostMalloc((void**)&st_host, sizeof(CgState), hipHostMallocDefault));
+        partial.ensure(partial_capacity(N));
+    }
+    void set_coefficients(double pweight) {
+        pw = pweight;
+        K0 = stiffness_entry(h, 0, 0, 0);
+        Ke = stiffness_entry(h, 1, 1, 0);
+        Kc = stiffness_entry(h, 1, 1, 1);
+        wh3 = pw > 0.0 ? pw * (h * h * h) / 1.5 : 0.0;
+        wg = pw > 0.0 ? wh3 * 0.005 : 0.0;
+        k64 = {K0, Ke, Kc, wg, wh3};
+        const double ih = 1.0 / h;
+        k32 = {(float)(K0 * ih), (float)(Ke * ih), (float)(Kc * ih), (float)(wg * ih), (float)(wh3 * ih)};
+    }
+};
+
+namespace prx {
+
+static thread_local std::string g_err;
+
+static void set_device(prx_tree* t) { PRX_HIP(hipSetDevice(t->device)); }
+
+// ------------------------------------------------------------------ operator helpers
+template <typename T>
+static T* sbuf_for(prx_tree* t);
+template <>
+double* sbuf_for<double>(prx_tree* t) { t->s64.ensure(std::max<int64_t>(t->P, 1)); return t->s64.p; }
+template <>
+float* sbuf_for<float>(prx_tree* t) { t->s32.ensure(std::max<int64_t>(t->P, 1)); return t->s32.p; }
+
+// y = A x (OP_APPLY) or y = b - A x (OP_RESID); partial sums into t->partial.
+template <typename T>
+static int op_pass(prx_tree* t, const LevelView& L, int op, const T* x, const T* b, T* y, const int32_t* done, double* partial) {
+    StencilIO<T> io;
+    io.x = x;
+    io.b = b;
+    io.y = y;
+    io.partial = partial;
+    io.done = done;
+    if (L.scr) {
+        T* sb = sbuf_for<T>(t);
+        launch_screen_gather<T>(L, x, nullptr, nullptr, sb, t->stream);
+        io.s = sb;
+    }
+    int nb = 0;
+    launch_stencil<T>(L, op, io, &nb, t->tile, t->stream);
+    return nb;
+}
+
+// x64 += e32 ; r32 = (b - A x64) * rscale ; partial = |b - A x64|^2
+static int op_resid_update(prx_tree* t, const LevelView& L, double* x64, const float* e32, const double* b, float* r32, double rscale) {
+    StencilIO<double> io;
+    io.x64 = x64;
+    io.e32 = e32;
+    io.b = b;
+    io.r32 = r32;
+    io.rscale = rscale;
+    io.partial = t->partial.p;
+    if (L.scr) {
+        double* sb = sbuf_for<double>(t);
+        launch_screen_gather<double>(L, nullptr, x64, e32, sb, t->stream);
+        io.s = sb;
+    }
+    int nb = 0;
+    launch_stencil<double>(L, OP_RESID_UPD, io, &nb, t->tile, t->stream);
+    return nb;
+}
+
+template <typename T>
+static void gs_sweeps(prx_tree* t, const LevelView& L, T* x, const T* b, int sweeps) {
+    T* sb = L.scr ? sbuf_for<T>(t) : nullptr;
+    for (int k = 0; k < sweeps; ++k)
+        for (int c = 0; c < 8; ++c) {
+            if (L.scr) launch_screen_gather<T>(L, x, nullptr, nullptr, sb, t->stream);
+            launch_gs_color<T>(L, c, x, b, sb, t->stream);
+        }
+}
+
+static void read_state(prx_tree* t) {
+    PRX_HIP(hipMemcpyAsync(t->st_host, t->st, sizeof(CgState), hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+}
+
+static void init_state(prx_tree* t, double eps, uint64_t max_iters, double bnorm2) {
+    CgState h{};
+    h.eps = eps;
+    h.max_iters = (int64_t)max_iters;
+    h.bnorm2 = bnorm2;
+    *t->st_host = h;
+    PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, t->stream));
+}
+
+// Dot product of two device vectors, folded on the device into st->slot[k].
+template <typename T>
+static double dev_dot(prx_tree* t, const T* a, const T* b, int64_t n) {
+    int nb = 0;
+    launch_dot<T>(a, b, n, t->partial.p, &nb, t->stream);
+    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, t->stream);
+    read_state(t);
+    return t->st_host->slot[0];
+}
+
+// solve_cg (src/solvers.rs:20-69) on level L.  `x_zero`: x is known to be 0,
+// so r = b without an operator pass.  `mode`: PRX_RESID_REFERENCE
+// (||r|| <= eps ||r0||) or PRX_RESID_TRUE (||r|| <= eps ||b||, bnorm2 = ||b||^2,
+// confirmed on the true residual, restarting from it if the recursive one
+// drifted).  Refresh r = b - A x on iterations refresh, 2*refresh, ...
+template <typename T>
+static CgOut run_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d, T* q, uint64_t max_iters, double eps,
+                    int mode, int refresh, double bnorm2, bool x_zero) {
+    CgOut out;
+    const int64_t n = L.N;
+    const hipStream_t s = t->stream;
+    init_state(t, eps, max_iters, bnorm2);
+    int nb;
+    if (x_zero) {
+        PRX_HIP(hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        launch_dot<T>(r, r, n, t->partial.p, &nb, s);
+    } else {
+        nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
+        out.applies++;
+    }
+    launch_fin(t->st, t->partial.p, nb, mode == PRX_RESID_TRUE ? FIN_INIT_TRUE : FIN_INIT_REF, 0, s);
+    PRX_HIP(hipMemcpyAsync(d, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    const int32_t* done = &t->st->done;
+    uint64_t it = 0;
+    int chunk = n >= (1 << 21) ? 8 : 32;
+    bool finished = false;
+    while (!finished) {
+        for (int k = 0; k < chunk && it < max_iters; ++k, ++it) {
+            nb = op_pass<T>(t, L, OP_APPLY, d, nullptr, q, done, t->partial.p);
+            launch_fin(t->st, t->partial.p, nb, FIN_ALPHA, 0, s);
+            if (refresh > 0 && (it + 1) % (uint64_t)refresh == 0) {
+                launch_axpy_alpha<T>(x, d, t->st, n, s);
+                nb = op_pass<T>(t, L, OP_RESID, x, b, r, done, t->partial.p);
+                out.applies++;
+            } else {
+                launch_cg_update<T>(x, r, d, q, t->st, n, t->partial.p, &nb, s);
+            }
+            out.applies++;
+            launch_fin(t->st, t->partial.p, nb, FIN_BETA, 0, s);
+            launch_xpby_beta<T>(d, r, t->st, n, s);
+        }
+        read_state(t);
+        const CgState& h = *t->st_host;
+        if (h.done || it >= max_iters) {
+            if (mode == PRX_RESID_TRUE && h.done && h.status == PRX_OK && h.iters > 0) {
+                // confirm on the true residual; restart from it if it misses
+                nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
+                out.applies++;
+                launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 1, s);
+                read_state(t);
+                const double rt = t->st_host->slot[1];
+                if (rt > (bnorm2 * eps) * eps && it < max_iters) {
+                    CgState hs = *t->st_host;
+                    hs.delta = rt;
+                    hs.done = 0;
+                    *t->st_host = hs;
+                    PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
+                    PRX_HIP(hipMemcpyAsync(d, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+                    continue;
+                }
+                t->st_host->delta = rt;
+            }
+            finished = true;
+        }
+    }
+    const CgState& h = *t->st_host;
+    out.iters = h.done ? (uint64_t)h.iters : max_iters;
+    if (!h.done) out.iters = it;
+    out.status = h.done ? h.status : PRX_ERR_NOT_CONVERGED;
+    const double den = mode == PRX_RESID_TRUE ? bnorm2 : h.delta0;
+    out.rel = den > 0.0 ? std::sqrt(h.delta / den) : 0.0;
+    return out;
+}
+
+// ------------------------------------------------------------------ cascadic solve
+struct SolveCfg {
+    uint32_t base;
+    int gs;
+    uint64_t cg_iters;
+    double eps;
+    prx_solve_opts o;
+};
+
+static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, double eps, prx_solve_report& rep) {
+    const hipStream_t s = t->stream;
+    const int64_t nc = (int64_t)1 << (3 * cdepth);
+    const double hc = 1.0 / (double)(1u << cdepth);
+    const double sf = (double)((uint64_t)1 << (2 * (t->D - cdepth)));
+    const double epw = t->pw * sf;
+    const double wh3c = epw > 0.0 ? epw * (hc * hc * hc) / 1.5 : 0.0;
+    LevelView L;
+    L.D = cdepth;
+    L.res = 1u << cdepth;
+    L.N = nc;
+    L.k64 = {stiffness_entry(hc, 0, 0, 0), stiffness_entry(hc, 1, 1, 0), stiffness_entry(hc, 1, 1, 1), epw > 0.0 ? wh3c * 1.005 : 0.0, 0.0};
+    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(nc); t->cdd.ensure(nc); t->cq.ensure(nc);
+    launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
+    launch_fill<double>(t->cx.p, 0.0, nc, s);
+    gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
+    CgOut c = run_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cq.p, iters, eps, PRX_RESID_REFERENCE, 10, 0.0, false);
+    rep.coarse_iters = c.iters;
+    rep.coarse_rel = c.rel;
+    rep.applies += c.applies;
+}
+
+// FP64 true relative residual ||b - A x|| / ||b|| of the current solution.
+static double true_rel(prx_tree* t) {
+    const LevelView L = t->fine_view();
+    t->r64.ensure(t->N);
+    const double bb = dev_dot<double>(t, t->b.p, t->b.p, t->N);
+    int nb = op_pass<double>(t, L, OP_RESID, t->x.p, t->b.p, t->r64.p, nullptr, t->partial.p);
+    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, t->stream);
+    read_state(t);
+    const double rr = t->st_host->slot[0];
+    return bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+}
+
+static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    t->r64.ensure(n); t->d64.ensure(n); t->q64.ensure(n);
+    {
+        Timer tm(t->stream);
+        gs_sweeps<double>(t, L, t->x.p, t->b.p, c.gs);
+        rep.ms_fine_gs = tm.ms();
+    }
+    Timer tm(t->stream);
+    double bb = 0.0;
+    if (c.o.residual_mode == PRX_RESID_TRUE) bb = dev_dot<double>(t, t->b.p, t->b.p, n);
+    CgOut o = run_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->q64.p, cap, c.eps, c.o.residual_mode,
+                             c.o.refresh_period, bb, false);
+    rep.ms_fine_cg = tm.ms();
+    rep.fine_iters = o.iters;
+    rep.fine_rel = o.rel;
+    rep.applies += o.applies;
+    rep.outer_iters = 1;
+    rep.status = o.status;
+}
+
+// Mixed precision: the fine correction equations A e = r are solved with the
+// FP32 operator (scaled by 1/h), FP32 vectors and FP64 CG scalars; the FP64
+// loop keeps the solution, forms the true residual b - A x and decides
+// convergence.
+static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    t->e32.ensure(n); t->r32.ensure(n); t->ri32.ensure(n); t->d32.ensure(n); t->q32.ensure(n);
+    const double ih = 1.0 / t->h;
+    double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
+    // smoothing on the correction: e = GS^k(0; A/h, r/h), x += e
+    {
+        Timer tm(s);
+        launch_fill<float>(t->e32.p, 0.0f, n, s);
+        op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+        if (c.gs > 0) {
+            gs_sweeps<float>(t, L, t->e32.p, t->r32.p, c.gs);
+        }
+        rep.ms_fine_gs = tm.ms();
+    }
+    Timer tm(s);
+    int nb = op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+    rep.applies += 2;
+    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+    read_state(t);
+    double rr = t->st_host->slot[0];
+    const double r0 = rr;
+    const double thr = c.o.residual_mode == PRX_RESID_TRUE ? (bb * c.eps) * c.eps : (r0 * c.eps) * c.eps;
+    uint64_t used = 0, outer = 0;
+    int status = PRX_OK;
+    while (rr > thr && rr > 0.0) {
+        if (used >= cap || outer >= c.o.max_outer) { status = PRX_ERR_NOT_CONVERGED; break; }
+        launch_fill<float>(t->e32.p, 0.0f, n, s);
+        CgOut o = run_cg<float>(t, L, t->e32.p, t->r32.p, t->ri32.p, t->d32.p, t->q32.p, cap - used, c.o.inner_tol,
+                                PRX_RESID_REFERENCE, 0, 0.0, true);
+        used += o.iters;
+        rep.applies += o.applies;
+        nb = op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+        rep.applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+        read_state(t);
+        rr = t->st_host->slot[0];
+        ++outer;
+        if (o.iters == 0) { status = PRX_ERR_BREAKDOWN; break; }
+    }
+    rep.ms_fine_cg = tm.ms();
+    rep.fine_iters = used;
+    rep.outer_iters = outer;
+    const double den = c.o.residual_mode == PRX_RESID_TRUE ? bb : r0;
+    rep.fine_rel = den > 0.0 ? std::sqrt(rr / den) : 0.0;
+    rep.status = status;
+}
+
+static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
+    std::memset(&rep, 0, sizeof rep);
+    t->ensure_state();
+    const hipStream_t s = t->stream;
+    Timer total(s);
+    if (c.base >= t->D) {
+        launch_fill<double>(t->x.p, 0.0, t->N, s);
+    } else {
+        Timer tm(s);
+        coarse_solve(t, c.base, c.gs, c.cg_iters, c.eps, rep);
+        launch_prolongate(t->D, c.base, t->cx.p, t->x.p, s);
+        rep.ms_coarse = tm.ms();
+    }
+    const uint64_t cap = c.base >= t->D ? c.cg_iters : c.cg_iters / 4;
+    if (c.o.precision == PRX_PREC_MIXED) fine_solve_mixed(t, c, cap, rep);
+    else fine_solve_fp64(t, c, cap, rep);
+    rep.ms_total = total.ms();
+    rep.true_rel = true_rel(t);
+}
+
+// ------------------------------------------------------------------ splat and assembly
+static void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
+    const int64_t P = (int64_t)n;
+    if (P >= ((int64_t)1 << 29)) throw Error(PRX_ERR_INVALID, "at most 2^29 points per tree");
+    std::vector<double> soa(6 * (size_t)std::max<int64_t>(P, 1));
+    for (int64_t i = 0; i < P; ++i) {
+        soa[i] = pts[i].px; soa[P + i] = pts[i].py; soa[2 * P + i] = pts[i].pz;
+        soa[3 * P + i] = pts[i].nx; soa[4 * P + i] = pts[i].ny; soa[5 * P + i] = pts[i].nz;
+    }
+    t->P = P;
+    t->pos.ensure(3 * std::max<int64_t>(P, 1));
+    t->nrm.ensure(3 * std::max<int64_t>(P, 1));
+    if (P > 0) {
+        PRX_HIP(hipMemcpyAsync(t->pos.p, soa.data(), sizeof(double) * 3 * P, hipMemcpyHostToDevice, t->stream));
+        PRX_HIP(hipMemcpyAsync(t->nrm.p, soa.data() + 3 * P, sizeof(double) * 3 * P, hipMemcpyHostToDevice, t->stream));
+    }
+    PRX_HIP(hipStreamSynchronize(t->stream));
+}
+
+static void splat(prx_tree* t) {
+    const hipStream_t s = t->stream;
+    const int64_t P = t->P, M8 = 8 * P;
+    free_list(t->floor_list);
+    free_list(t->trunc_list);
+    t->perm.ensure(std::max<int64_t>(P, 1));
+    t->cell.ensure(3 * std::max<int64_t>(P, 1));
+    t->t64.ensure(3 * std::max<int64_t>(P, 1));
+    t->t32.ensure(3 * std::max<int64_t>(P, 1));
+    t->pts.P = P;
+    t->pts.pos = t->pos.p;
+    t->pts.nrm = t->nrm.p;
+    t->pts.perm = t->perm.p;
+    t->pts.cell = t->cell.p;
+    t->pts.t64 = t->t64.p;
+    t->pts.t32 = t->t32.p;
+    t->nf.ensure(3 * t->N);
+    t->w.ensure(t->N);
+    if (P > 0) {
+        DArr<uint32_t> kf, kt, val, kf2, kt2, val2, val3, ck, cv, ck2, inv;
+        kf.ensure(M8); kt.ensure(M8); val.ensure(M8); kf2.ensure(M8); kt2.ensure(M8); val2.ensure(M8); val3.ensure(M8);
+        ck.ensure(P); cv.ensure(P); ck2.ensure(P); inv.ensure(P);
+        launch_point_keys(t->D, t->pos.p, P, kf.p, kt.p, val.p, ck.p, cv.p, s);
+        const int kbits = 3 * (int)t->D + 1;
+        size_t need = 0, need2 = 0, need3 = 0;
+        PRX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kf.p, kf2.p, val.p, val2.p, (int)M8, 0, kbits, s));
+        PRX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need2, ck.p, ck2.p, cv.p, t->perm.p, (int)P, 0, kbits, s));
+        PRX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need3, val.p, val2.p, (int)M8, s));
+        size_t sb = std::max(need, std::max(need2, need3));
+        DArr<uint8_t> scratch;
+        scratch.ensure((int64_t)sb);
+        // spatial order of the points (screening gathers read neighbouring nodes)
+        PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, ck.p, ck2.p, cv.p, t->perm.p, (int)P, 0, kbits, s));
+        launch_invert_perm(t->perm.p, P, inv.p, s);
+        launch_point_sorted_data(t->D, t->pos.p, P, t->perm.p, t->cell.p, t->t64.p, t->t32.p, s);
+        // floor list (splat), trunc list (screening); radix sort is stable, so
+        // entries of one node stay in input point order
+        PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, kf.p, kf2.p, val.p, val2.p, (int)M8, 0, kbits, s));
+        build_node_list(t->floor_list, t->D, kf2.p, val2.p, M8, t->pts, inv.p, scratch.p, sb, s);
+        PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, kt.p, kt2.p, val.p, val3.p, (int)M8, 0, kbits, s));
+        build_node_list(t->trunc_list, t->D, kt2.p, val3.p, M8, t->pts, inv.p, scratch.p, sb, s);
+        PRX_HIP(hipStreamSynchronize(s));
+    }
+    launch_splat(t->D, t->floor_list, t->pts, t->nf.p, t->w.p, s);
+    PRX_HIP(hipStreamSynchronize(s));
+    t->splatted = true;
+    t->assembled = false;
+}
+
+static void assemble(prx_tree* t, double pw) {
+    if (!t->splatted) throw Error(PRX_ERR_STATE, "assemble before splat_normal_field");
+    const hipStream_t s = t->stream;
+    t->set_coefficients(pw);
+    t->b.ensure(t->N);
+    t->x.ensure(t->N);
+    launch_rhs_laplacian(t->D, t->nf.p, t->b.p, s);
+    if (pw > 0.0) launch_rhs_screening(t->D, t->trunc_list, t->pts, t->wh3, t->b.p, s);
+    launch_fill<double>(t->x.p, 0.0, t->N, s);
+    t->ensure_state();
+    PRX_HIP(hipStreamSynchronize(s));
+    t->assembled = true;
+}
+
+static prx_tree* create_tree(uint32_t depth, int device) {
+    if (depth > PRX_MAX_DEPTH) throw Error(PRX_ERR_INVALID, "depth must be <= 10");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(PRX_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) throw Error(PRX_ERR_INVALID, "device ordinal out of range");
+    std::unique_ptr<prx_tree> t(new prx_tree());
+    t->device = device;
+    PRX_HIP(hipSetDevice(device));
+    PRX_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    t->D = depth;
+    t->res = 1u << depth;
+    t->N = (int64_t)1 << (3 * depth);
+    t->h = 1.0 / (double)t->res;
+    if (const char* e = std::getenv("PRX_TILE")) t->tile = std::atoi(e) == 8 ? 8 : 16;
+    return t.release();
+}
+
+static bool finite_points(const prx_point* p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i].px) || !std::isfinite(p[i].py) || !std::isfinite(p[i].pz) || !std::isfinite(p[i].nx) ||
+            !std::isfinite(p[i].ny) || !std::isfinite(p[i].nz))
+            return false;
+    return true;
+}
+
+}  // namespace prx
+
+// ====================================================================== C ABI
+#define PRX_API_BEGIN try {
+#define PRX_API_END                                   \
+    }                                                 \
+    catch (const prx::Error& e) {                     \
+        prx::g_err = e.what();                        \
+        return e.code;                                \
+    }                                                 \
+    catch (const std::bad_alloc&) {                   \
+        prx::g_err = "host allocation failed";        \
+        return PRX_ERR_OOM;                           \
+    }                                                 \
+    catch (const std::exception& e) {                 \
+        prx::g_err = e.what();                        \
+        return PRX_ERR_DEVICE;                        \
+    }                                                 \
+    return PRX_OK;
+
+#define PRX_REQUIRE(cond, code, msg) \
+    if (!(cond)) throw prx::Error(code, msg)
+
+extern "C" {
+
+const char* prx_last_error(void) { return prx::g_err.c_str(); }
+
+int prx_device_count(int* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(out, PRX_ERR_INVALID, "null pointer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *out = n;
+    PRX_API_END
+}
+
+void prx_params_default(prx_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->depth = 8;
+    p->samples_per_node = 1.5;
+    p->scale = 1.1;
+    p->cg_accuracy = 1e-3;
+    p->cg_iters = 500;
+    p->point_weight = 0.0;
+    p->verbose = 0;
+    p->precision = PRX_PREC_FP64;
+    p->residual_mode = PRX_RESID_REFERENCE;
+    p->device = 0;
+    p->gs_iters = 20;
+    p->base_depth = -1;
+    p->refresh_period = 10;
+    p->inner_tol = 1e-3;
+    p->max_outer = 64;
+}
+
+void prx_solve_opts_default(prx_solve_opts* o) {
+    if (!o) return;
+    o->precision = PRX_PREC_FP64;
+    o->residual_mode = PRX_RESID_REFERENCE;
+    o->refresh_period = 10;
+    o->gs_iters = 20;
+    o->inner_tol = 1e-3;
+    o->max_outer = 64;
+}
+
+int prx_tree_create(uint32_t depth, int device, prx_tree** out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(out, PRX_ERR_INVALID, "null pointer");
+    *out = nullptr;
+    *out = prx::create_tree(depth, device);
+    PRX_API_END
+}
+
+void prx_tree_destroy(prx_tree* t) {
+    if (!t) return;
+    (void)hipSetDevice(t->device);
+    delete t;
+}
+
+int64_t prx_tree_node_count(const prx_tree* t) { return t ? t->N : -1; }
+
+int prx_tree_export_nodes(prx_tree* t, uint32_t* i2o, int64_t* g2i) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    set_device(t);
+    if (i2o) {
+        DArr<uint32_t> d;
+        d.ensure(3 * t->N);
+        launch_export_offsets(t->D, d.p, t->stream);
+        PRX_HIP(hipMemcpyAsync(i2o, d.p, sizeof(uint32_t) * 3 * t->N, hipMemcpyDeviceToHost, t->stream));
+        PRX_HIP(hipStreamSynchronize(t->stream));
+    }
+    if (g2i) {
+        DArr<int64_t> d;
+        d.ensure(t->N);
+        launch_export_g2i(t->D, d.p, t->stream);
+        PRX_HIP(hipMemcpyAsync(g2i, d.p, sizeof(int64_t) * t->N, hipMemcpyDeviceToHost, t->stream));
+        PRX_HIP(hipStreamSynchronize(t->stream));
+    }
+    PRX_API_END
+}
+
+int prx_splat_normal_field(prx_tree* t, const prx_point* pts, size_t n) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    PRX_REQUIRE(pts || n == 0, PRX_ERR_INVALID, "null points");
+    set_device(t);
+    prx::upload_points(t, pts, n);
+    prx::splat(t);
+    PRX_API_END
+}
+
+int prx_export_normal_field(prx_tree* t, double* nf, double* w) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    PRX_REQUIRE(t->splatted, PRX_ERR_STATE, "splat_normal_field not called");
+    set_device(t);
+    if (nf) {
+        std::vector<double> soa(3 * (size_t)t->N);
+        PRX_HIP(hipMemcpyAsync(soa.data(), t->nf.p, sizeof(double) * 3 * t->N, hipMemcpyDeviceToHost, t->stream));
+        PRX_HIP(hipStreamSynchronize(t->stream));
+        for (int64_t i = 0; i < t->N; ++i) {
+            nf[3 * i] = soa[i];
+            nf[3 * i + 1] = soa[t->N + i];
+            nf[3 * i + 2] = soa[2 * t->N + i];
+        }
+    }
+    if (w) {
+        PRX_HIP(hipMemcpyAsync(w, t->w.p, sizeof(double) * t->N, hipMemcpyDeviceToHost, t->stream));
+        PRX_HIP(hipStreamSynchronize(t->stream));
+    }
+    PRX_API_END
+}
+
+int prx_assemble(prx_tree* t, double point_weight) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    set_device(t);
+    prx::assemble(t, point_weight);
+    PRX_API_END
+}
+
+int prx_export_rhs(prx_tree* t, double* rhs) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && rhs, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    PRX_HIP(hipMemcpyAsync(rhs, t->b.p, sizeof(double) * t->N, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_csr_nnz(prx_tree* t, int64_t* nnz) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && nnz, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    DArr<int64_t> cnt, sum;
+    cnt.ensure(t->N);
+    sum.ensure(1);
+    const NodeList* L = t->pw > 0.0 ? &t->trunc_list : nullptr;
+    launch_csr_count(t->D, L, &t->pts, t->pw > 0.0 ? 1 : 0, cnt.p, t->stream);
+    size_t need = 0;
+    PRX_HIP(hipcub::DeviceReduce::Sum(nullptr, need, cnt.p, sum.p, (int)t->N, t->stream));
+    DArr<uint8_t> tmp;
+    tmp.ensure((int64_t)need);
+    PRX_HIP(hipcub::DeviceReduce::Sum(tmp.p, need, cnt.p, sum.p, (int)t->N, t->stream));
+    PRX_HIP(hipMemcpyAsync(nnz, sum.p, sizeof(int64_t), hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_export_csr(prx_tree* t, int64_t* off, int64_t* col, double* val) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && off && col && val, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    const NodeList* L = t->pw > 0.0 ? &t->trunc_list : nullptr;
+    const int diag = t->pw > 0.0 ? 1 : 0;
+    DArr<int64_t> cnt;
+    cnt.ensure(t->N + 1);
+    launch_csr_count(t->D, L, &t->pts, diag, cnt.p, t->stream);
+    std::vector<int64_t> h(t->N);
+    PRX_HIP(hipMemcpyAsync(h.data(), cnt.p, sizeof(int64_t) * t->N, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    off[0] = 0;
+    for (int64_t i = 0; i < t->N; ++i) off[i + 1] = off[i] + h[i];
+    const int64_t nnz = off[t->N];
+    DArr<int64_t> doff, dcol;
+    DArr<double> dval;
+    doff.ensure(t->N + 1);
+    dcol.ensure(nnz);
+    dval.ensure(nnz);
+    PRX_HIP(hipMemcpyAsync(doff.p, off, sizeof(int64_t) * (t->N + 1), hipMemcpyHostToDevice, t->stream));
+    launch_csr_fill(t->D, L, &t->pts, diag, t->K0, t->Ke, t->Kc, t->wg, t->wh3, doff.p, dcol.p, dval.p, t->stream);
+    PRX_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipMemcpyAsync(val, dval.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_apply(prx_tree* t, int precision, const double* x, double* y) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && x && y, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    t->r64.ensure(n);
+    t->q64.ensure(n);
+    PRX_HIP(hipMemcpyAsync(t->r64.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (precision == PRX_PREC_MIXED) {
+        t->d32.ensure(n);
+        t->q32.ensure(n);
+        launch_cast_scale_d2f(t->d32.p, t->r64.p, 1.0, n, s);
+        prx::op_pass<float>(t, L, OP_APPLY, t->d32.p, nullptr, t->q32.p, nullptr, t->partial.p);
+        std::vector<float> hy(n);
+        PRX_HIP(hipMemcpyAsync(hy.data(), t->q32.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) y[i] = (double)hy[i] * t->h;
+    } else {
+        prx::op_pass<double>(t, L, OP_APPLY, t->r64.p, nullptr, t->q64.p, nullptr, t->partial.p);
+        PRX_HIP(hipMemcpyAsync(y, t->q64.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+    }
+    PRX_API_END
+}
+
+int prx_coarse_system(prx_tree* t, uint32_t cdepth, double* rhs_rowmajor, double* coef) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    PRX_REQUIRE(cdepth <= t->D, PRX_ERR_INVALID, "coarse depth above the tree depth");
+    set_device(t);
+    const int64_t nc = (int64_t)1 << (3 * cdepth);
+    const double hc = 1.0 / (double)(1u << cdepth);
+    const double epw = t->pw * (double)((uint64_t)1 << (2 * (t->D - cdepth)));
+    const double wh3c = epw > 0.0 ? epw * (hc * hc * hc) / 1.5 : 0.0;
+    if (coef) {
+        coef[0] = prx::stiffness_entry(hc, 0, 0, 0);
+        coef[1] = prx::stiffness_entry(hc, 1, 1, 0);
+        coef[2] = prx::stiffness_entry(hc, 1, 1, 1);
+        coef[3] = epw > 0.0 ? wh3c * 1.005 : 0.0;
+    }
+    if (rhs_rowmajor) {
+        DArr<double> m, r;
+        m.ensure(nc);
+        r.ensure(nc);
+        launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, m.p, t->stream);
+        launch_morton_to_rowmajor(cdepth, m.p, r.p, t->stream);
+        PRX_HIP(hipMemcpyAsync(rhs_rowmajor, r.p, sizeof(double) * nc, hipMemcpyDeviceToHost, t->stream));
+        PRX_HIP(hipStreamSynchronize(t->stream));
+    }
+    PRX_API_END
+}
+
+int prx_prolongate(prx_tree* t, uint32_t cdepth, const double* coarse, double* fine) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && coarse && fine, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(cdepth <= t->D, PRX_ERR_INVALID, "coarse depth above the tree depth");
+    set_device(t);
+    const int64_t nc = (int64_t)1 << (3 * cdepth);
+    DArr<double> r, m, f;
+    r.ensure(nc);
+    m.ensure(nc);
+    f.ensure(t->N);
+    PRX_HIP(hipMemcpyAsync(r.p, coarse, sizeof(double) * nc, hipMemcpyHostToDevice, t->stream));
+    launch_rowmajor_to_morton(cdepth, r.p, m.p, t->stream);
+    launch_prolongate(t->D, cdepth, m.p, f.p, t->stream);
+    PRX_HIP(hipMemcpyAsync(fine, f.p, sizeof(double) * t->N, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_gs(prx_tree* t, int precision, uint32_t sweeps, const double* b, double* x) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && b && x, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    t->r64.ensure(n);
+    t->d64.ensure(n);
+    PRX_HIP(hipMemcpyAsync(t->r64.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(t->d64.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (precision == PRX_PREC_MIXED) {
+        t->d32.ensure(n);
+        t->q32.ensure(n);
+        launch_cast_scale_d2f(t->q32.p, t->r64.p, 1.0 / t->h, n, s);
+        launch_cast_scale_d2f(t->d32.p, t->d64.p, 1.0, n, s);
+        prx::gs_sweeps<float>(t, L, t->d32.p, t->q32.p, (int)sweeps);
+        std::vector<float> hx(n);
+        PRX_HIP(hipMemcpyAsync(hx.data(), t->d32.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) x[i] = hx[i];
+    } else {
+        prx::gs_sweeps<double>(t, L, t->d64.p, t->r64.p, (int)sweeps);
+        PRX_HIP(hipMemcpyAsync(x, t->d64.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+    }
+    PRX_API_END
+}
+
+int prx_cg(prx_tree* t, int precision, const double* b, double* x, uint64_t max_iters, double eps, int residual_mode,
+           int refresh_period, uint64_t* iters, double* rel) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && b && x, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    PRX_REQUIRE(precision == PRX_PREC_FP64, PRX_ERR_INVALID, "prx_cg runs the FP64 CG; mixed precision is a solve option");
+    set_device(t);
+    t->ensure_state();
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    DArr<double> db, dx;
+    db.ensure(n);
+    dx.ensure(n);
+    t->r64.ensure(n); t->d64.ensure(n); t->q64.ensure(n);
+    PRX_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    double bb = residual_mode == PRX_RESID_TRUE ? prx::dev_dot<double>(t, db.p, db.p, n) : 0.0;
+    prx::CgOut o = prx::run_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->q64.p, max_iters, eps, residual_mode,
+                                       refresh_period, bb, false);
+    PRX_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipStreamSynchronize(s));
+    if (iters) *iters = o.iters;
+    if (rel) *rel = o.rel;
+    PRX_API_END
+}
+
+int prx_solve_cascadic(prx_tree* t, uint32_t base_depth, uint32_t gs, uint64_t cg_iters, double eps, const prx_solve_opts* opts,
+                       prx_solve_report* report) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    prx::SolveCfg c;
+    c.base = base_depth;
+    c.gs = (int)gs;
+    c.cg_iters = cg_iters;
+    c.eps = eps;
+    prx_solve_opts_default(&c.o);
+    if (opts) c.o = *opts;
+    prx_solve_report rep;
+    prx::solve_cascadic(t, c, rep);
+    if (report) *report = rep;
+    PRX_API_END
+}
+
+int prx_get_solution(prx_tree* t, double* x) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && x, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    PRX_HIP(hipMemcpyAsync(x, t->x.p, sizeof(double) * t->N, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_set_solution(prx_tree* t, const double* x) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && x, PRX_ERR_INVALID, "null pointer");
+    set_device(t);
+    t->x.ensure(t->N);
+    PRX_HIP(hipMemcpyAsync(t->x.p, x, sizeof(double) * t->N, hipMemcpyHostToDevice, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_evaluate(prx_tree* t, const double* q, size_t m, double* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && (q || m == 0) && (out || m == 0), PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->x.p, PRX_ERR_STATE, "no solution");
+    set_device(t);
+    if (m == 0) return PRX_OK;
+    DArr<double> dq, dout;
+    dq.ensure(3 * (int64_t)m);
+    dout.ensure((int64_t)m);
+    PRX_HIP(hipMemcpyAsync(dq.p, q, sizeof(double) * 3 * m, hipMemcpyHostToDevice, t->stream));
+    launch_evaluate(t->D, t->x.p, dq.p, (int64_t)m, dout.p, t->stream);
+    PRX_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * m, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_evaluate_corners(prx_tree* t, double* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && out, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->x.p, PRX_ERR_STATE, "no solution");
+    set_device(t);
+    const int64_t r1 = (int64_t)t->res + 1, n = r1 * r1 * r1;
+    DArr<double> d;
+    d.ensure(n);
+    launch_evaluate_corners(t->D, t->x.p, d.p, t->stream);
+    PRX_HIP(hipMemcpyAsync(out, d.p, sizeof(double) * n, hipMemcpyDeviceToHost, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
+int prx_iso_value(prx_tree* t, double* iso) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && iso, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->x.p && t->splatted, PRX_ERR_STATE, "no solution");
+    set_device(t);
+    t->ensure_state();
+    if (t->P == 0) { *iso = 0.0; return PRX_OK; }
+    DArr<double> v;
+    v.ensure(t->P);
+    launch_evaluate_pointsSoA(t->D, t->x.p, t->pos.p, t->P, v.p, t->stream);
+    DArr<double> ones;
+    ones.ensure(t->P);
+    launch_fill<double>(ones.p, 1.0, t->P, t->stream);
+    const double s = prx::dev_dot<double>(t, v.p, ones.p, t->P);
+    *iso = s / (double)t->P;
+    PRX_API_END
+}
+
+int prx_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* x, double* y) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(n >= 0 && off && col && val && x && y, PRX_ERR_INVALID, "null pointer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw prx::Error(PRX_ERR_DEVICE, "no HIP device available");
+    const int64_t nnz = off[n];
+    hipStream_t s;
+    PRX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DArr<int64_t> doff, dcol;
+    DArr<double> dval, dx, dy;
+    doff.ensure(n + 1); dcol.ensure(nnz); dval.ensure(nnz); dx.ensure(n); dy.ensure(n);
+    PRX_HIP(hipMemcpyAsync(doff.p, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(dcol.p, col, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(dval.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    launch_csr_spmv(n, doff.p, dcol.p, dval.p, dx.p, dy.p, s);
+    PRX_HIP(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipStreamSynchronize(s));
+    PRX_HIP(hipStreamDestroy(s));
+    PRX_API_END
+}
+
+}  // extern "C"
