@@ -199,6 +199,23 @@ int prx_evaluate_corners(prx_tree* t, double* out);
 /* iso value: mean of evaluate over the uploaded points (src/reconstructor.rs:56). */
 int prx_iso_value(prx_tree* t, double* iso);
 
+/* BBox + unit_cube_transform + transform_point/vector + normalize
+ * (src/geometry.rs:349-402, :256-269, :49-56) as applied at
+ * src/reconstructor.rs:31-40.  m, minv: column-major 4x4 (m[col*4+row]) of
+ * the model->unit map and its inverse (Mat4::inverse, src/geometry.rs:274-316);
+ * out receives the n transformed points.  Any output may be NULL. */
+int prx_unit_transform(const prx_point* points, size_t n, double scale, double* m, double* minv, prx_point* out);
+
+/* ------------------------------------------------------------ measurement
+ * No counterpart in the reference (it has no benchmarks): device-side timing
+ * used by bench.py.  prx_time_operator launches the operator `reps` times on
+ * the tree's stream and returns the average hipEvent duration of the
+ * screening gather kernel and of the stencil kernel.  prx_copy_bandwidth
+ * times a streaming device copy of `bytes` (read + write) and returns the
+ * average ms per copy. */
+int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
+int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
+
 /* ------------------------------------------------------------ generic CSR
  * SparseMatrix::multiply_vector (src/sparse.rs:105-117) and solve_cg
  * (src/solvers.rs:20-69) on a caller CSR, run on the device. */

@@ -180,13 +180,35 @@ typedef struct { int64_t n; int64_t* off; int64_t* col; double* val; } orc_csr;
 
 typedef struct { int64_t col; double val; } ent;
 
+/* stable merge sort by column (Rust's sort_by_key is a stable merge sort) */
+static void merge_sort(ent* r, ent* tmp, int64_t len) {
+    if (len < 24) {
+        for (int64_t i = 1; i < len; ++i) { /* insertion sort: stable */
+            ent e = r[i];
+            int64_t j = i - 1;
+            while (j >= 0 && r[j].col > e.col) { r[j + 1] = r[j]; --j; }
+            r[j + 1] = e;
+        }
+        return;
+    }
+    int64_t h = len / 2;
+    merge_sort(r, tmp, h);
+    merge_sort(r + h, tmp, len - h);
+    int64_t i = 0, j = h, k = 0;
+    while (i < h && j < len) tmp[k++] = (r[j].col < r[i].col) ? r[j++] : r[i++];
+    while (i < h) tmp[k++] = r[i++];
+    while (j < len) tmp[k++] = r[j++];
+    memcpy(r, tmp, len * sizeof(ent));
+}
+
 /* sort_by_key (stable) then sequential dedupe-sum (src/fem_tree.rs:229-239). */
 static int64_t sort_dedupe(ent* r, int64_t len) {
-    for (int64_t i = 1; i < len; ++i) { /* insertion sort: stable */
-        ent e = r[i];
-        int64_t j = i - 1;
-        while (j >= 0 && r[j].col > e.col) { r[j + 1] = r[j]; --j; }
-        r[j + 1] = e;
+    if (len >= 24) {
+        ent* tmp = (ent*)malloc(len * sizeof(ent));
+        merge_sort(r, tmp, len);
+        free(tmp);
+    } else {
+        merge_sort(r, NULL, len);
     }
     int64_t j = 0, u = 0;
     while (j < len) {

@@ -113,6 +113,9 @@ SIGNATURES = {
     "prx_evaluate": (_i32, [_vp, _vp, _sz, _vp]),
     "prx_evaluate_corners": (_i32, [_vp, _vp]),
     "prx_iso_value": (_i32, [_vp, _vp]),
+    "prx_unit_transform": (_i32, [_vp, _sz, _f64, _vp, _vp, _vp]),
+    "prx_time_operator": (_i32, [_vp, _i32, _i32, _vp, _vp]),
+    "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
     "prx_csr_spmv": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "prx_csr_cg": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _u64, _f64, _vp]),
 }
@@ -168,6 +171,17 @@ def default_solve_opts() -> SolveOpts:
     o = SolveOpts()
     lib().prx_solve_opts_default(ctypes.byref(o))
     return o
+
+
+def unit_transform(points, scale: float = 1.1):
+    """(unit points, model_to_unit, unit_to_model): the point transform at
+    src/reconstructor.rs:31-40 (matrices column-major, m[col*4+row])."""
+    pts = as_points(points)
+    m = np.zeros(16)
+    mi = np.zeros(16)
+    out = np.empty_like(pts)
+    check(lib().prx_unit_transform(ptr(pts), len(pts), float(scale), ptr(m), ptr(mi), ptr(out)))
+    return out, m, mi
 
 
 def as_points(points) -> np.ndarray:

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             double2 xo;
             xo.x = xin[v * VEC];
             xo.y = xin[v * VEC + 1];
-            *(double2*)(io.x64 + base + m) = xo;
+            *(double2*)(io.x_out + base + m) = xo;
             const double r0 = bb.x - tile[m], r1 = bb.y - tile[m + 1];
             float2 rr;
             rr.x = (float)(r0 * io.rscale);
@@ -308,12 +308,10 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
         demorton3(i, ix, iy, iz);
         if constexpr (OP == OP_RESID_UPD) {
             const double a = apply_node<T, 1>(L, nullptr, io.x64, io.e32, io.s, ix, iy, iz, i, nullptr);
-            const double xn = io.x64[i] + (double)io.e32[i];
+            io.x_out[i] = io.x64[i] + (double)io.e32[i];
             const double r = io.b[i] - a;
             io.r32[i] = (float)(r * io.rscale);
             acc = r * r;
-            // x64 is written by a separate pass (neighbours still read the old value)
-            (void)xn;
         } else {
             const T a = apply_node<T, 0>(L, io.x, nullptr, nullptr, io.s, ix, iy, iz, i, nullptr);
             if constexpr (OP == OP_APPLY) {
@@ -343,11 +341,8 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
             case OP_APPLY: hipLaunchKernelGGL((k_stencil_node<T, OP_APPLY>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
             case OP_RESID: hipLaunchKernelGGL((k_stencil_node<T, OP_RESID>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
             default:
-                if constexpr (sizeof(T) == 8) {
+                if constexpr (sizeof(T) == 8)
                     hipLaunchKernelGGL((k_stencil_node<T, OP_RESID_UPD>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N);
-                    PRX_LAUNCH_CHECK();
-                    launch_add_f2d(io.x64, io.e32, N, s);
-                }
         }
         PRX_LAUNCH_CHECK();
         *nblk = nb;

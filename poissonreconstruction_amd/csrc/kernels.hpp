@@ -85,14 +85,16 @@ void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int ha
 
 // ---- k_stencil.hip
 // One operator pass on level L.  OP_APPLY: y = A x, partial = x.y.
-// OP_RESID: y = b - A x, partial = |y|^2.  OP_RESID_UPD (T = double): x64 +=
-// e32 (written back), r32 = (b - A x64) * rscale, partial = |b - A x64|^2.
+// OP_RESID: y = b - A x, partial = |y|^2.  OP_RESID_UPD (T = double):
+// x_out = x64 + e32, r32 = (b - A x_out) * rscale, partial = |b - A x_out|^2.
+// x_out must not alias x64: neighbouring workgroups still read x64 as halo.
 template <typename T>
 struct StencilIO {
     const T* x = nullptr;
     const T* b = nullptr;
     T* y = nullptr;
-    double* x64 = nullptr;
+    const double* x64 = nullptr;
+    double* x_out = nullptr;
     const float* e32 = nullptr;
     float* r32 = nullptr;
     double rscale = 1.0;

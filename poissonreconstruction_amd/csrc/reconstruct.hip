@@ -25,7 +25,7 @@ static inline double rs_max(double a, double b) { return a != a ? b : (b != b ? 
 
 // BBox + unit_cube_transform + inverse (src/geometry.rs:349-402, :274-316);
 // m is column-major m[col*4+row] like Mat4.
-static void unit_transform(const prx_point* p, size_t n, double scale, double m[16], double mi[16]) {
+void host_unit_transform(const prx_point* p, size_t n, double scale, double m[16], double mi[16]) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (size_t i = 0; i < n; ++i) {
         const double q[3] = {p[i].px, p[i].py, p[i].pz};
@@ -57,8 +57,7 @@ static void unit_transform(const prx_point* p, size_t n, double scale, double m[
 
 // transform_point / transform_vector + normalize (src/geometry.rs:256-269,
 // :49-56), as applied at src/reconstructor.rs:36-40.
-static void apply_transform(const prx_point* in, size_t n, const double m[16], std::vector<prx_point>& out) {
-    out.resize(n);
+void host_apply_transform(const prx_point* in, size_t n, const double m[16], prx_point* out) {
     for (size_t i = 0; i < n; ++i) {
         const prx_point p = in[i];
         prx_point o;
@@ -105,10 +104,10 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     const double t_total = prx::now_s();
     double t0 = prx::now_s();
     double m[16], mi[16];
-    prx::unit_transform(points, n, p.scale, m, mi);
+    prx::host_unit_transform(points, n, p.scale, m, mi);
     st.time_transform = prx::now_s() - t0;
-    std::vector<prx_point> unit;
-    prx::apply_transform(points, n, m, unit);
+    std::vector<prx_point> unit(n);
+    prx::host_apply_transform(points, n, m, unit.data());
 
     t0 = prx::now_s();
     prx_tree* t = nullptr;

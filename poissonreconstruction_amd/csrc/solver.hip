@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -26,6 +27,10 @@ struct DArr {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+    }
+    void swap(DArr& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
     }
     void ensure(int64_t cnt) {
         if (cnt <= n && p) return;
@@ -103,7 +108,7 @@ struct prx_tree {
     bool splatted = false, assembled = false;
 
     // fields
-    DArr<double> nf, w, b, x;
+    DArr<double> nf, w, b, x, x2;
     double pw = 0.0;
     Coef<double> k64{};
     Coef<float> k32{};
@@ -190,10 +195,14 @@ static int op_pass(prx_tree* t, const LevelView& L, int op, const T* x, const T*
     return nb;
 }
 
-// x64 += e32 ; r32 = (b - A x64) * rscale ; partial = |b - A x64|^2
-static int op_resid_update(prx_tree* t, const LevelView& L, double* x64, const float* e32, const double* b, float* r32, double rscale) {
+// x_new = x + e32 ; r32 = (b - A x_new) * rscale ; partial = |b - A x_new|^2.
+// x_new goes to the second solution buffer, which then becomes t->x.
+static int op_resid_update(prx_tree* t, const LevelView& L, const float* e32, const double* b, float* r32, double rscale) {
+    t->x2.ensure(t->N);
+    const double* x64 = t->x.p;
     StencilIO<double> io;
     io.x64 = x64;
+    io.x_out = t->x2.p;
     io.e32 = e32;
     io.b = b;
     io.r32 = r32;
@@ -206,6 +215,7 @@ static int op_resid_update(prx_tree* t, const LevelView& L, double* x64, const f
     }
     int nb = 0;
     launch_stencil<double>(L, OP_RESID_UPD, io, &nb, t->tile, t->stream);
+    t->x.swap(t->x2);
     return nb;
 }
 
@@ -397,14 +407,14 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     {
         Timer tm(s);
         launch_fill<float>(t->e32.p, 0.0f, n, s);
-        op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+        op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
         if (c.gs > 0) {
             gs_sweeps<float>(t, L, t->e32.p, t->r32.p, c.gs);
         }
         rep.ms_fine_gs = tm.ms();
     }
     Timer tm(s);
-    int nb = op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+    int nb = op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
     rep.applies += 2;
     launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
     read_state(t);
@@ -420,7 +430,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
                                 PRX_RESID_REFERENCE, 0, 0.0, true);
         used += o.iters;
         rep.applies += o.applies;
-        nb = op_resid_update(t, L, t->x.p, t->e32.p, t->b.p, t->r32.p, ih);
+        nb = op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
         rep.applies++;
         launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
         read_state(t);
@@ -561,6 +571,63 @@ static bool finite_points(const prx_point* p, size_t n) {
             !std::isfinite(p[i].ny) || !std::isfinite(p[i].nz))
             return false;
     return true;
+}
+
+int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil) {
+    try {
+        if (!t->assembled) return PRX_ERR_STATE;
+        set_device(t);
+        t->ensure_state();
+        const LevelView L = t->fine_view();
+        const int64_t n = t->N;
+        const hipStream_t s = t->stream;
+        hipEvent_t e0, e1, e2;
+        PRX_HIP(hipEventCreate(&e0));
+        PRX_HIP(hipEventCreate(&e1));
+        PRX_HIP(hipEventCreate(&e2));
+        double g = 0.0, k = 0.0;
+        auto run = [&](auto* xin, auto* yout, auto* sb) {
+            using T = std::remove_pointer_t<decltype(xin)>;
+            StencilIO<T> io;
+            io.x = xin;
+            io.y = yout;
+            io.partial = t->partial.p;
+            io.s = sb;
+            int nb = 0;
+            for (int r = -1; r < reps; ++r) {  // r = -1 warms up
+                PRX_HIP(hipEventRecord(e0, s));
+                if (L.scr) launch_screen_gather<T>(L, xin, nullptr, nullptr, sb, s);
+                PRX_HIP(hipEventRecord(e1, s));
+                launch_stencil<T>(L, OP_APPLY, io, &nb, t->tile, s);
+                PRX_HIP(hipEventRecord(e2, s));
+                PRX_HIP(hipEventSynchronize(e2));
+                float a = 0.f, b = 0.f;
+                PRX_HIP(hipEventElapsedTime(&a, e0, e1));
+                PRX_HIP(hipEventElapsedTime(&b, e1, e2));
+                if (r >= 0) { g += a; k += b; }
+            }
+        };
+        if (precision == PRX_PREC_MIXED) {
+            t->d32.ensure(n);
+            t->q32.ensure(n);
+            launch_fill<float>(t->d32.p, 1.0f, n, s);
+            run(t->d32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr);
+        } else {
+            t->d64.ensure(n);
+            t->q64.ensure(n);
+            launch_fill<double>(t->d64.p, 1.0, n, s);
+            run(t->d64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        if (ms_gather) *ms_gather = g / reps;
+        if (ms_stencil) *ms_stencil = k / reps;
+        return PRX_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    }
 }
 
 }  // namespace prx

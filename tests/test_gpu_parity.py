@@ -143,7 +143,7 @@ def test_coarse_system_and_prolongation_bit_exact(gpu, D, cd, pw):
         for k in range(off[i], off[i + 1]):
             j = col[k]
             jz, jy, jx = j // (r * r), (j // r) % r, j % r
-            nzc = (jx != x) + (jy != y) + (jz != z)
+            nzc = int(jx != x) + int(jy != y) + int(jz != z)
             expect = {0: coef[0], 2: coef[1], 3: coef[2]}[nzc]
             if nzc == 0 and coef[3] > 0:
                 expect = 0.0 + coef[0] + coef[3]
