@@ -69,8 +69,8 @@ typedef struct prx_params {
     int32_t gs_iters;       /* smoothing sweeps per level (reference: 20) */
     int32_t base_depth;     /* -1: D-2 rule of src/reconstructor.rs:51 */
     int32_t refresh_period; /* CG true-residual refresh period (reference: 10; 0 = never) */
-    double inner_tol;       /* mixed: residual reduction of one FP32 inner solve (default 1e-3) */
-    uint64_t max_outer;     /* mixed: cap on FP64 refinement steps (default 64) */
+    double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.1) */
+    uint64_t max_outer;     /* mixed: cap on FP64 residual replacements (default 64) */
 } prx_params;
 
 /* ReconstructStats (src/reconstructor.rs:17-23) plus extension fields.

@@ -23,7 +23,7 @@ static inline int nblocks_for(int64_t n, int per_thread = 1, int threads = 256, 
 static inline int red_blocks(int64_t n) { return nblocks_for(n, 8, 256, 2048); }
 
 int partial_capacity(int64_t N) {
-    int64_t b = (N + 511) / 512;    // one partial per brick (8^3 or 16^3)
+    int64_t b = (N + 511) / 512;    // one partial per brick (8^3 or 16^3), per 512 nodes of k_cg_update
     int64_t r = red_blocks(N);
     int64_t nv = (N + 255) / 256;   // naive stencil: one partial per block
     int64_t m = b > r ? b : r;
@@ -92,8 +92,17 @@ template void launch_fill<int64_t>(int64_t*, int64_t, int64_t, hipStream_t);
 template <typename T>
 __global__ __launch_bounds__(256) void k_dot(const T* a, const T* b, int64_t n, double* partial) {
     __shared__ double ws[4];
+    using VT = typename V16<T>::type;
+    constexpr int VEC = V16<T>::n;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    const int64_t nv = n / VEC, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const VT av = ((const VT*)a)[i], bv = ((const VT*)b)[i];
+        const T *as = (const T*)&av, *bs = (const T*)&bv;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc += (double)as[e] * (double)bs[e];
+    }
+    for (int64_t i = nv * VEC + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         acc += (double)a[i] * (double)b[i];
     double s = block_sum_256(acc, ws);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -110,12 +119,19 @@ template void launch_dot<float>(const float*, const float*, int64_t, double*, in
 
 // One block folds the partials in a fixed order and advances the CG scalars
 // exactly as solve_cg does (src/solvers.rs:36-64).
-__global__ __launch_bounds__(256) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
-    __shared__ double ws[4];
+__global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
+    __shared__ double ws[16];
     if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && st->done) return;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partial[i];
-    double sum = block_sum_256(acc, ws);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * 1024 < np; i += 4 * 1024) {
+        a0 += partial[i];
+        a1 += partial[i + 1024];
+        a2 += partial[i + 2048];
+        a3 += partial[i + 3072];
+    }
+    for (; i < np; i += 1024) a0 += partial[i];
+    double sum = block_sum_256((a0 + a1) + (a2 + a3), ws);
     if (threadIdx.x != 0) return;
     switch (op) {
         case FIN_SLOT: st->slot[slot] = sum; break;
@@ -141,6 +157,7 @@ __global__ __launch_bounds__(256) void k_fin(CgState* st, const double* partial,
             break;
         case FIN_BETA: {
             double dold = st->delta;
+            st->delta_prev = dold;
             st->delta = sum;
             st->it += 1;
             if (sum <= st->thr) {
@@ -166,31 +183,51 @@ __global__ __launch_bounds__(256) void k_fin(CgState* st, const double* partial,
     }
 }
 void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(256), 0, s, st, partial, np, op, slot);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, s, st, partial, np, op, slot);
     PRX_LAUNCH_CHECK();
 }
 
 // x += alpha d ; r -= alpha q ; partial = r.r   (src/solvers.rs:47, :53, :57)
+// One 16-byte vector per array per thread over a flat grid, streamed with
+// non-temporal loads and stores (every operand is touched once per pass).
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
                                                    const T* __restrict__ q, const CgState* st, int64_t n, double* partial) {
     __shared__ double ws[4];
     if (st->done) return;
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
     const double a = st->alpha;
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double di = (double)d[i], qi = (double)q[i];
-        x[i] = (T)((double)x[i] + a * di);
-        T rn = (T)((double)r[i] - a * qi);
-        r[i] = rn;
-        acc += (double)rn * (double)rn;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = n / VEC;
+    if (i < nv) {
+        const int64_t j = i * VEC;
+        const NV xv = ld_nt(x + j), rv = ld_nt(r + j), dv = ld_nt(d + j), qv = ld_nt(q + j);
+        NV xo, ro;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
+            ro[e] = (T)((double)rv[e] - a * (double)qv[e]);
+            acc += (double)ro[e] * (double)ro[e];
+        }
+        st_nt(x + j, xo);
+        st_nt(r + j, ro);
+    } else if (i - nv < n - nv * VEC) {  // tail (n not a multiple of the vector width)
+        const int64_t j = nv * VEC + (i - nv);
+        x[j] = (T)((double)x[j] + a * (double)d[j]);
+        const T rn = (T)((double)r[j] - a * (double)q[j]);
+        r[j] = rn;
+        acc = (double)rn * (double)rn;
     }
     double s = block_sum_256(acc, ws);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 template <typename T>
 void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s) {
-    int nb = red_blocks(n);
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t threads = n / VEC + (n % VEC);
+    const int nb = (int)((threads + 255) / 256);
     hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, s, x, r, d, q, st, n, partial);
     PRX_LAUNCH_CHECK();
     *nblk = nb;

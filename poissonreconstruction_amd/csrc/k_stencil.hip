@@ -6,18 +6,21 @@
 // screening mass on the diagonal (src/fem_tree.rs:203-204) and S the sum of
 // the per-point 8x8 blocks wh3*phi*phi^T (src/fem_tree.rs:206-226).
 //
-// L is applied in difference form with zero padding outside the grid:
-//   (Lx)_i = ce * sum_edges (x_j - x_i) + cc * sum_corners (x_j - x_i),
-// which equals the reference rows (interior row sum c0 + 12ce + 8cc = 0; a
-// missing neighbour contributes ce*(0 - x_i) or cc*(0 - x_i), i.e. exactly the
-// truncated row).  Every term is a difference, so the FP32 operator keeps the
-// tiny screening terms that a direct sum would round away.
+// L is applied with exact integer weights: ce = -2s, cc = -s, c0 = 32s with
+// s = h/12, so every row of the zero-padded 27-point stencil sums to exactly
+// zero, and (Lx)_i is unchanged when a constant is subtracted from x.  Each
+// brick therefore works on x - r0, r0 = the brick's first node value, with
+// nodes outside the grid stored as 0 - r0 (this reproduces the reference's
+// truncated boundary rows).  Rounding then scales with the variation of x
+// inside the brick instead of |x|, which keeps the tiny screening terms
+// resolvable in FP32.  The weights equal the reference's K(d) to 1 ulp.
 //
 // Tiled kernel: one workgroup per aligned B^3 brick, which is a contiguous
 // Morton range.  The brick is read with 16-byte coalesced loads, the one-node
-// halo is gathered from the 26 neighbour bricks, all of it is staged in LDS,
-// and each thread sweeps one (x,y) column along z with a 3-plane register
-// window (9 LDS reads per output).  Outputs go back through LDS so the store
+// halo is gathered from the 26 neighbour bricks, all of it is staged in LDS
+// (interior LDS offsets are linear in the Morton bits, so no per-element
+// decode), and each thread sweeps one (x,y) column along z with a 3-plane
+// register window (9 LDS reads per output).  Outputs go back through LDS so the store
 // is coalesced too.  Screening: points were pre-reduced to s_p =
 // wh3*sum(phi*x) (k_screen_gather); the brick's touched nodes add
 // sum(phi*s_p) in a fixed order, so the result is deterministic.
@@ -28,7 +31,8 @@ namespace prx {
 template <typename T>
 struct LevelArgs {
     uint32_t D, res;
-    T ce, cc, wg, c0;  // c0 = -(12ce + 8cc), the diagonal of L implied by the difference form
+    T ce, cc, wg, c0;  // c0 = -(12ce + 8cc) = 32s, the diagonal of L
+    T s;               // weight scale: ce = -2s, cc = -s
     int has_screen;
     int brick_log;
     const uint32_t* brick;
@@ -50,7 +54,9 @@ static LevelArgs<T> make_args(const LevelView& L, int tile_log) {
     } else {
         a.ce = L.k32.ce; a.cc = L.k32.cc; a.wg = L.k32.wg; a.wh3 = L.k32.wh3;
     }
-    a.c0 = -(T(12) * a.ce + T(8) * a.cc);
+    a.s = -a.cc;
+    a.ce = T(-2) * a.s;
+    a.c0 = T(32) * a.s;
     a.has_screen = (L.scr && L.scr->T > 0) ? 1 : 0;
     if (a.has_screen) {
         const NodeList& S = *L.scr;
@@ -67,10 +73,6 @@ static LevelArgs<T> make_args(const LevelView& L, int tile_log) {
     return a;
 }
 
-template <typename T> struct V16;
-template <> struct V16<float> { using type = float4; static constexpr int n = 4; };
-template <> struct V16<double> { using type = double2; static constexpr int n = 2; };
-
 __device__ inline double block_sum_red(double v, double* wsum) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -82,7 +84,28 @@ __device__ inline double block_sum_red(double v, double* wsum) {
     return s;
 }
 
-// halo shell index -> (hx,hy,hz) in [-1,B]^3 \ [0,B)^3
+// ------------------------------------------------------------------ tile index tables
+// Morton bit helpers usable at compile time.
+constexpr uint32_t spread3c(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+constexpr uint32_t compact3c(uint32_t m) {
+    uint32_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((m >> (3 * b)) & 1u) << b;
+    return r;
+}
+// LDS offset of local Morton index m in an E^3 tile (before the +1 halo
+// shift).  It is linear in the bits of m, so the offset of m = a | b | c with
+// disjoint bit fields is the sum of the three offsets: each thread computes
+// its own part once and adds compile-time constants per element.
+template <int E>
+constexpr int lds_lin(uint32_t m) {
+    return (int)(compact3c(m) + E * compact3c(m >> 1) + E * E * compact3c(m >> 2));
+}
+
+// halo shell index -> (hx,hy,hz) in [-1,B]^3 \\ [0,B)^3
 template <int B>
 __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
     constexpr int E = B + 2, P2 = E * E, R = 4 * B + 4;
@@ -104,9 +127,11 @@ __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
 template <typename T, int B, int OP>
 __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO<T> io, uint32_t nbricks, int swz) {
     if (io.done && *io.done) return;
-    constexpr int NT = B * B, B3 = B * B * B, E = B + 2, E3 = E * E * E;
-    using VT = typename V16<T>::type;
-    constexpr int VEC = V16<T>::n, PER = B3 / NT, NV = PER / VEC;
+    constexpr int NT = B * B, B3 = B * B * B, E = B + 2, E3 = E * E * E, S = E3 - B3;
+    using NV = typename NV16<T>::type;
+    typedef float nf2 __attribute__((ext_vector_type(2)));
+    typedef double nd2 __attribute__((ext_vector_type(2)));
+    constexpr int VEC = 16 / (int)sizeof(T), PER = B3 / NT, NVEC = PER / VEC, NH = (S + NT - 1) / NT;
     __shared__ T tile[E3];
     __shared__ double wsum[(NT + 63) / 64];
     const int tid = threadIdx.x;
@@ -119,58 +144,98 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     uint32_t ox, oy, oz;
     demorton3(bid, ox, oy, oz);
     ox *= B; oy *= B; oz *= B;
+    double bt = 0.0;
+    if constexpr (OP == OP_DIR_APPLY) bt = *io.beta;
 
-    // ---- brick interior: 16-byte coalesced loads, scattered into LDS
-    T xin[PER];
+    // ---- 1. issue the brick's interior loads (16-byte, coalesced)
+    NV qa[NVEC], qb[NVEC];
+    nf2 ee[OP == OP_RESID_UPD ? NVEC : 1];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const uint32_t m = (uint32_t)(v * NT + tid) * VEC;
+    for (int v = 0; v < NVEC; ++v) {
+        const uint64_t g = base + (uint32_t)(v * NT + tid) * VEC;
         if constexpr (OP == OP_RESID_UPD) {
-            const double2 xx = *(const double2*)(io.x64 + base + m);
-            const float2 ee = *(const float2*)(io.e32 + base + m);
-            xin[v * VEC] = xx.x + (double)ee.x;
-            xin[v * VEC + 1] = xx.y + (double)ee.y;
+            qa[v] = ld_v((const T*)io.x64 + g);
+            ee[v] = *(const nf2*)(io.e32 + g);
+        } else if constexpr (OP == OP_DIR_APPLY) {
+            qa[v] = ld_v(io.r + g);
+            qb[v] = ld_v(io.dold + g);
         } else {
-            const VT q = *(const VT*)(io.x + base + m);
-            const T* qq = (const T*)&q;
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) xin[v * VEC + e] = qq[e];
-        }
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) {
-            uint32_t lx, ly, lz;
-            demorton3(m + e, lx, ly, lz);
-            tile[((lz + 1) * E + ly + 1) * E + lx + 1] = xin[v * VEC + e];
+            qa[v] = ld_v(io.x + g);
         }
     }
-    // ---- one-node halo from the neighbouring bricks (zero outside the grid)
-    for (int i = tid; i < E3 - B3; i += NT) {
-        int hx, hy, hz;
-        shell_pos<B>(i, hx, hy, hz);
-        const int gx = (int)ox + hx, gy = (int)oy + hy, gz = (int)oz + hz;
-        T v = 0;
-        if ((unsigned)gx < L.res && (unsigned)gy < L.res && (unsigned)gz < L.res) {
-            const uint32_t g = morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
-            if constexpr (OP == OP_RESID_UPD) v = io.x64[g] + (double)io.e32[g];
-            else v = io.x[g];
+    // ---- 2. issue the one-node halo loads (x = 0 outside the grid)
+    T hv[NH];
+    int hl[NH];
+    bool hin[NH];
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int i = tid + k * NT;
+        hl[k] = -1;
+        hin[k] = false;
+        hv[k] = T(0);
+        if (i < S) {
+            int hx, hy, hz;
+            shell_pos<B>(i, hx, hy, hz);
+            hl[k] = ((hz + 1) * E + hy + 1) * E + hx + 1;
+            const int gx = (int)ox + hx, gy = (int)oy + hy, gz = (int)oz + hz;
+            if ((unsigned)gx < L.res && (unsigned)gy < L.res && (unsigned)gz < L.res) {
+                const uint32_t g = morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+                hin[k] = true;
+                if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
+                else if constexpr (OP == OP_DIR_APPLY) hv[k] = (T)fma(bt, (double)io.dold[g], (double)io.r[g]);
+                else hv[k] = io.x[g];
+            }
         }
-        tile[((hz + 1) * E + hy + 1) * E + hx + 1] = v;
     }
+    // brick reference value r0 (the brick's first node, formed as the node is)
+    T r0;
+    if constexpr (OP == OP_RESID_UPD) r0 = io.x64[base] + (double)io.e32[base];
+    else if constexpr (OP == OP_DIR_APPLY) r0 = (T)fma(bt, (double)io.dold[base], (double)io.r[base]);
+    else r0 = io.x[base];
+    // ---- 3. interior values into LDS (offset linear in the Morton bits)
+    T xin[PER];
+    const int lbase = (E * E + E + 1) + (int)(compact3(tid * VEC) + E * compact3((tid * VEC) >> 1) + E * E * compact3((tid * VEC) >> 2));
+#pragma unroll
+    for (int v = 0; v < NVEC; ++v) {
+        const uint64_t g = base + (uint32_t)(v * NT + tid) * VEC;
+        if constexpr (OP == OP_RESID_UPD) {
+            xin[v * VEC] = qa[v][0] + (double)ee[v][0];
+            xin[v * VEC + 1] = qa[v][1] + (double)ee[v][1];
+        } else if constexpr (OP == OP_DIR_APPLY) {
+            // d_new = r + beta*d_old, the same fma wherever a node is formed
+            NV nq;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                nq[e] = (T)fma(bt, (double)qb[v][e], (double)qa[v][e]);
+                xin[v * VEC + e] = nq[e];
+            }
+            st_nt(io.dnew + g, nq);
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) xin[v * VEC + e] = qa[v][e];
+        }
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+            tile[lbase + lds_lin<E>((uint32_t)(v * NT * VEC)) + lds_lin<E>((uint32_t)e)] = xin[v * VEC + e] - r0;
+    }
+#pragma unroll
+    for (int k = 0; k < NH; ++k)
+        if (hl[k] >= 0) tile[hl[k]] = hin[k] ? hv[k] - r0 : -r0;
     __syncthreads();
 
-    // ---- z sweep with a 3-plane window
+    // ---- 4. z sweep with a 3-plane register window (values are x - r0)
     const int tx = tid % B, ty = tid / B;
     T out[B];
     {
-        const T ce = L.ce, cc = L.cc, c4 = T(4) * (L.ce + L.cc), wg = L.wg;
+        const T sc = L.s, wg = L.wg;
         T Cm, Fm, Dm, Cc, Fc, Dc;
         auto plane = [&](int p, T& C, T& F, T& Dg) {
-            const T* r0 = tile + (p * E + ty) * E + tx;
-            const T* r1 = r0 + E;
-            const T* r2 = r1 + E;
-            C = r1[1];
-            F = ((r0[1] - C) + (r2[1] - C)) + ((r1[0] - C) + (r1[2] - C));
-            Dg = ((r0[0] - C) + (r0[2] - C)) + ((r2[0] - C) + (r2[2] - C));
+            const T* a0 = tile + (p * E + ty) * E + tx;
+            const T* a1 = a0 + E;
+            const T* a2 = a1 + E;
+            C = a1[1];
+            F = (a0[1] + a2[1]) + (a1[0] + a1[2]);
+            Dg = (a0[0] + a0[2]) + (a2[0] + a2[2]);
         };
         plane(0, Cm, Fm, Dm);
         plane(1, Cc, Fc, Dc);
@@ -178,18 +243,21 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         for (int z = 0; z < B; ++z) {
             T Cp, Fp, Dp;
             plane(z + 2, Cp, Fp, Dp);
-            T lap = ce * (Dc + Fm + Fp) + cc * (Dm + Dp) + c4 * ((Cm - Cc) + (Cp - Cc));
-            out[z] = lap + wg * Cc;
+            const T lap = sc * (T(32) * Cc - T(2) * ((Dc + Fm) + Fp) - (Dm + Dp));
+            out[z] = lap + wg * (Cc + r0);
             Cm = Cc; Fm = Fc; Dm = Dc;
             Cc = Cp; Fc = Fp; Dc = Dp;
         }
     }
     __syncthreads();
+    {
+        const uint32_t mxy = spread3((uint32_t)tx) | (spread3((uint32_t)ty) << 1);
 #pragma unroll
-    for (int z = 0; z < B; ++z) tile[morton3((uint32_t)tx, (uint32_t)ty, (uint32_t)z)] = out[z];
+        for (int z = 0; z < B; ++z) tile[mxy | (spread3c((uint32_t)z) << 2)] = out[z];
+    }
     __syncthreads();
 
-    // ---- screening: touched nodes of this brick, fixed summation order
+    // ---- 5. screening: touched nodes of this brick, fixed summation order
     if (L.has_screen) {
         const uint32_t t0 = L.brick[bid], t1 = L.brick[bid + 1];
         if (t0 < t1) {
@@ -203,47 +271,46 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         }
     }
 
-    // ---- coalesced output + reduction partial
-    double acc = 0.0;
+    // ---- 6. coalesced stores + reduction partial (per-thread terms in T,
+    // folded in FP64 across the block and the grid)
+    T tacc = T(0);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
+    for (int v = 0; v < NVEC; ++v) {
         const uint32_t m = (uint32_t)(v * NT + tid) * VEC;
-        if constexpr (OP == OP_APPLY) {
-            VT q;
-            T* qq = (T*)&q;
+        const uint64_t g = base + m;
+        if constexpr (OP == OP_APPLY || OP == OP_DIR_APPLY) {
+            NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                qq[e] = tile[m + e];
-                acc += (double)xin[v * VEC + e] * (double)qq[e];
+                q[e] = tile[m + e];
+                tacc += xin[v * VEC + e] * q[e];
             }
-            *(VT*)(io.y + base + m) = q;
+            *(NV*)(io.y + g) = q;
         } else if constexpr (OP == OP_RESID) {
-            const VT bb = *(const VT*)(io.b + base + m);
-            const T* bq = (const T*)&bb;
-            VT q;
-            T* qq = (T*)&q;
+            const NV bb = ld_v(io.b + g);
+            NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                qq[e] = bq[e] - tile[m + e];
-                acc += (double)qq[e] * (double)qq[e];
+                q[e] = bb[e] - tile[m + e];
+                tacc += q[e] * q[e];
             }
-            *(VT*)(io.y + base + m) = q;
+            *(NV*)(io.y + g) = q;
         } else {
-            const double2 bb = *(const double2*)((const double*)io.b + base + m);
-            double2 xo;
-            xo.x = xin[v * VEC];
-            xo.y = xin[v * VEC + 1];
-            *(double2*)(io.x_out + base + m) = xo;
-            const double r0 = bb.x - tile[m], r1 = bb.y - tile[m + 1];
-            float2 rr;
-            rr.x = (float)(r0 * io.rscale);
-            rr.y = (float)(r1 * io.rscale);
-            *(float2*)(io.r32 + base + m) = rr;
-            acc += r0 * r0 + r1 * r1;
+            const nd2 bb = *(const nd2*)((const double*)io.b + g);
+            nd2 xo;
+            xo[0] = xin[v * VEC];
+            xo[1] = xin[v * VEC + 1];
+            __builtin_nontemporal_store(xo, (nd2*)(io.x_out + g));
+            const double q0 = bb[0] - tile[m], q1 = bb[1] - tile[m + 1];
+            nf2 rr;
+            rr[0] = (float)(q0 * io.rscale);
+            rr[1] = (float)(q1 * io.rscale);
+            __builtin_nontemporal_store(rr, (nf2*)(io.r32 + g));
+            tacc += q0 * q0 + q1 * q1;
         }
     }
-    const double s = block_sum_red(acc, wsum);
-    if (tid == 0 && io.partial) io.partial[bid] = s;
+    const double sacc = block_sum_red((double)tacc, wsum);
+    if (tid == 0 && io.partial) io.partial[bid] = sacc;
 }
 
 // ------------------------------------------------------------------ per-node form (small grids, GS)
@@ -262,13 +329,14 @@ __device__ inline int64_t touched_index(const LevelArgs<T>& L, uint32_t i) {
 }
 
 // (A x)_i with x read from global memory; also returns the diagonal a_ii.
-template <typename T, int MODE>
+template <typename T, int MODE>  // MODE 0: x ; 1: x64 + e32 ; 2: r + beta*d (x = r, x64 unused, d = dd)
 __device__ inline T apply_node(const LevelArgs<T>& L, const T* x, const double* x64, const float* e32, const T* s, uint32_t ix,
-                               uint32_t iy, uint32_t iz, uint32_t i, T* diag) {
+                               uint32_t iy, uint32_t iz, uint32_t i, T* diag, const T* dd = nullptr, double bt = 0.0) {
     auto X = [&](int jx, int jy, int jz) -> T {
         if ((unsigned)jx >= L.res || (unsigned)jy >= L.res || (unsigned)jz >= L.res) return T(0);
         const uint32_t j = morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
         if constexpr (MODE == 1) return (T)(x64[j] + (double)e32[j]);
+        else if constexpr (MODE == 2) return (T)fma(bt, (double)dd[j], (double)x[j]);
         else return x[j];
     };
     const T xc = X(ix, iy, iz);
@@ -312,6 +380,13 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
             const double r = io.b[i] - a;
             io.r32[i] = (float)(r * io.rscale);
             acc = r * r;
+        } else if constexpr (OP == OP_DIR_APPLY) {
+            const double bt = *io.beta;
+            const T a = apply_node<T, 2>(L, io.r, nullptr, nullptr, io.s, ix, iy, iz, i, nullptr, io.dold, bt);
+            const T dn = (T)fma(bt, (double)io.dold[i], (double)io.r[i]);
+            io.dnew[i] = dn;
+            io.y[i] = a;
+            acc = (double)dn * (double)a;
         } else {
             const T a = apply_node<T, 0>(L, io.x, nullptr, nullptr, io.s, ix, iy, iz, i, nullptr);
             if constexpr (OP == OP_APPLY) {
@@ -339,6 +414,9 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         const int nb = (int)((N + 255) / 256);
         switch (op) {
             case OP_APPLY: hipLaunchKernelGGL((k_stencil_node<T, OP_APPLY>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
+            case OP_DIR_APPLY:
+                hipLaunchKernelGGL((k_stencil_node<T, OP_DIR_APPLY>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N);
+                break;
             case OP_RESID: hipLaunchKernelGGL((k_stencil_node<T, OP_RESID>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
             default:
                 if constexpr (sizeof(T) == 8)
@@ -356,10 +434,12 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
     hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz)
     if (B == 16) {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(16, OP_APPLY);
+        else if (op == OP_DIR_APPLY) PRX_TILE_LAUNCH(16, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_TILE_LAUNCH(16, OP_RESID);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(16, OP_RESID_UPD);
     } else {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(8, OP_APPLY);
+        else if (op == OP_DIR_APPLY) PRX_TILE_LAUNCH(8, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_TILE_LAUNCH(8, OP_RESID);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(8, OP_RESID_UPD);
     }

@@ -8,7 +8,7 @@ namespace prx {
 // Scalars stay on the device so an iteration needs no host round-trip; every
 // kernel of an iteration returns at once when `done` is set.
 struct CgState {
-    double delta, delta0, dq, alpha, beta, thr, eps;
+    double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
     double bnorm2;
     double slot[8];
     int64_t it, iters, max_iters;
@@ -39,7 +39,7 @@ struct LevelView {
     const DevPoints* pts = nullptr;
 };
 
-enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2 };
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3 };
 
 // ---- k_grid.hip
 void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
@@ -77,8 +77,11 @@ void launch_invert_perm(const uint32_t* perm, int64_t P, uint32_t* inv, hipStrea
 void launch_splat(uint32_t D, const NodeList& L, const DevPoints& pts, double* nf, double* w, hipStream_t s);
 void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s);
 void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, double wh3, double* rhs, hipStream_t s);
+// s_p over the point's corners of x (x64 + e32 when x64 is given; r + beta*d
+// when r is given).
 template <typename T>
-void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s);
+void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s,
+                          const T* r = nullptr, const T* d = nullptr, const double* beta = nullptr);
 void launch_csr_count(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, int64_t* rowcnt, hipStream_t s);
 void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, double K0, double Ke, double Kc,
                      double wg, double wh3, const int64_t* off, int64_t* col, double* val, hipStream_t s);
@@ -88,6 +91,8 @@ void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int ha
 // OP_RESID: y = b - A x, partial = |y|^2.  OP_RESID_UPD (T = double):
 // x_out = x64 + e32, r32 = (b - A x_out) * rscale, partial = |b - A x_out|^2.
 // x_out must not alias x64: neighbouring workgroups still read x64 as halo.
+// OP_DIR_APPLY: dnew = r + beta*dold (beta read on the device), y = A dnew,
+// partial = dnew.y; dnew must not alias dold for the same reason.
 template <typename T>
 struct StencilIO {
     const T* x = nullptr;
@@ -98,6 +103,10 @@ struct StencilIO {
     const float* e32 = nullptr;
     float* r32 = nullptr;
     double rscale = 1.0;
+    const T* r = nullptr;
+    const T* dold = nullptr;
+    T* dnew = nullptr;
+    const double* beta = nullptr;
     const T* s = nullptr;  // per-point screening values wh3*sum(phi*x)
     double* partial = nullptr;
     const int32_t* done = nullptr;

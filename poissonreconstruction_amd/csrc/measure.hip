@@ -11,18 +11,11 @@ void host_unit_transform(const prx_point* p, size_t n, double scale, double m[16
 void host_apply_transform(const prx_point* in, size_t n, const double m[16], prx_point* out);
 int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
 
-// Streaming copy: 16-byte loads and stores, four loads in flight per thread.
-__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
-        b[i] = v0;
-        b[i + stride] = v1;
-        b[i + 2 * stride] = v2;
-        b[i + 3 * stride] = v3;
-    }
-    for (; i < n; i += stride) b[i] = a[i];
+// Streaming copy: one 16-byte vector per thread over a flat grid,
+// non-temporal loads and stores (the fastest copy shape measured on gfx950).
+__global__ __launch_bounds__(256) void k_copy16(const float* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) st_nt(b + 4 * i, ld_nt(a + 4 * i));
 }
 }  // namespace prx
 
@@ -49,7 +42,7 @@ int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms) {
         using namespace prx;
         PRX_HIP(hipSetDevice(device));
         const int64_t n = (int64_t)(bytes / 2 / 16);
-        float4 *a = nullptr, *b = nullptr;
+        float *a = nullptr, *b = nullptr;
         PRX_HIP(hipMalloc((void**)&a, n * 16));
         PRX_HIP(hipMalloc((void**)&b, n * 16));
         hipStream_t s;
@@ -58,7 +51,7 @@ int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms) {
         hipEvent_t e0, e1;
         PRX_HIP(hipEventCreate(&e0));
         PRX_HIP(hipEventCreate(&e1));
-        const int nb = 256 * 8 * 4;
+        const int nb = (int)((n + 255) / 256);
         hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, a, b, n);
         PRX_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, s, a, b, n);

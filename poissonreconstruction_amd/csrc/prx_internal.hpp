@@ -79,6 +79,30 @@ struct Coef {
     T c0, ce, cc, wg, wh3;
 };
 
+// 16-byte vector of T (one global load/store instruction per lane).
+template <typename T> struct V16;
+template <> struct V16<float> { using type = float4; static constexpr int n = 4; };
+template <> struct V16<double> { using type = double2; static constexpr int n = 2; };
+
+// Native 16-byte vectors for the non-temporal load/store builtins: streamed
+// data (read once, written once per pass) bypasses the cache hierarchy's
+// normal allocation, which measurably raises streaming bandwidth on gfx950.
+template <typename T> struct NV16;
+template <> struct NV16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct NV16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename T>
+__device__ inline typename NV16<T>::type ld_nt(const T* p) {
+    return __builtin_nontemporal_load((const typename NV16<T>::type*)p);
+}
+template <typename T>
+__device__ inline void st_nt(T* p, typename NV16<T>::type v) {
+    __builtin_nontemporal_store(v, (typename NV16<T>::type*)p);
+}
+template <typename T>
+__device__ inline typename NV16<T>::type ld_v(const T* p) {
+    return *(const typename NV16<T>::type*)p;
+}
+
 // Touched-node lists: one entry per (point, corner) that reaches a node,
 // sorted by node and, within a node, by input point order.
 struct NodeList {

@@ -253,77 +253,166 @@ static double dev_dot(prx_tree* t, const T* a, const T* b, int64_t n) {
     return t->st_host->slot[0];
 }
 
-// solve_cg (src/solvers.rs:20-69) on level L.  `x_zero`: x is known to be 0,
-// so r = b without an operator pass.  `mode`: PRX_RESID_REFERENCE
-// (||r|| <= eps ||r0||) or PRX_RESID_TRUE (||r|| <= eps ||b||, bnorm2 = ||b||^2,
-// confirmed on the true residual, restarting from it if the recursive one
-// drifted).  Refresh r = b - A x on iterations refresh, 2*refresh, ...
+// solve_cg (src/solvers.rs:20-69) on level L, with the update d = r + beta d
+// fused into the next operator pass (OP_DIR_APPLY).  The fused kernel reads
+// the previous direction and writes the new one, so the two live in a
+// ping-pong pair: iteration k reads dd[k&1] and writes dd[(k+1)&1].  Every
+// kernel of an iteration returns at once when the device flag `done` is set,
+// so chunks of iterations are enqueued without a host round-trip and the
+// iteration count is exact.  Refresh r = b - A x on iterations refresh,
+// 2*refresh, ... as the reference does every 10th (refresh 0 = never).
 template <typename T>
-static CgOut run_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d, T* q, uint64_t max_iters, double eps,
-                    int mode, int refresh, double bnorm2, bool x_zero) {
-    CgOut out;
-    const int64_t n = L.N;
-    const hipStream_t s = t->stream;
-    init_state(t, eps, max_iters, bnorm2);
-    int nb;
-    if (x_zero) {
-        PRX_HIP(hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-        launch_dot<T>(r, r, n, t->partial.p, &nb, s);
-    } else {
-        nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
-        out.applies++;
+struct Cg {
+    prx_tree* t;
+    LevelView L;
+    T* x;
+    const T* b;
+    T* r;
+    T* dd[2];
+    T* q;
+    int refresh = 0;
+    uint64_t it = 0;  // completed iterations (the device's count after run())
+    uint64_t applies = 0;
+
+    T* dir() { return dd[it & 1]; }
+
+    // r = b - A x (or r = b when x is known to be 0); delta0 = r.r
+    void start(bool x_zero, int mode, double eps, uint64_t max_iters, double bnorm2) {
+        const hipStream_t s = t->stream;
+        init_state(t, eps, max_iters, bnorm2);
+        int nb;
+        if (x_zero) {
+            PRX_HIP(hipMemcpyAsync(r, b, sizeof(T) * L.N, hipMemcpyDeviceToDevice, s));
+            launch_dot<T>(r, r, L.N, t->partial.p, &nb, s);
+        } else {
+            nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
+            applies++;
+        }
+        launch_fin(t->st, t->partial.p, nb, mode == PRX_RESID_TRUE ? FIN_INIT_TRUE : FIN_INIT_REF, 0, s);
+        launch_fill<T>(dd[0], T(0), L.N, s);
+        it = 0;
     }
-    launch_fin(t->st, t->partial.p, nb, mode == PRX_RESID_TRUE ? FIN_INIT_TRUE : FIN_INIT_REF, 0, s);
-    PRX_HIP(hipMemcpyAsync(d, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    const int32_t* done = &t->st->done;
-    uint64_t it = 0;
-    int chunk = n >= (1 << 21) ? 8 : 32;
-    bool finished = false;
-    while (!finished) {
-        for (int k = 0; k < chunk && it < max_iters; ++k, ++it) {
-            nb = op_pass<T>(t, L, OP_APPLY, d, nullptr, q, done, t->partial.p);
-            launch_fin(t->st, t->partial.p, nb, FIN_ALPHA, 0, s);
-            if (refresh > 0 && (it + 1) % (uint64_t)refresh == 0) {
-                launch_axpy_alpha<T>(x, d, t->st, n, s);
-                nb = op_pass<T>(t, L, OP_RESID, x, b, r, done, t->partial.p);
-                out.applies++;
-            } else {
-                launch_cg_update<T>(x, r, d, q, t->st, n, t->partial.p, &nb, s);
-            }
-            out.applies++;
-            launch_fin(t->st, t->partial.p, nb, FIN_BETA, 0, s);
-            launch_xpby_beta<T>(d, r, t->st, n, s);
+    // start from a residual already in r (x = 0 implied): delta0 = r.r
+    void start_from_residual(double eps, uint64_t max_iters) {
+        const hipStream_t s = t->stream;
+        init_state(t, eps, max_iters, 0.0);
+        int nb;
+        launch_dot<T>(r, r, L.N, t->partial.p, &nb, s);
+        launch_fin(t->st, t->partial.p, nb, FIN_INIT_REF, 0, s);
+        launch_fill<T>(dd[0], T(0), L.N, s);
+        it = 0;
+    }
+    void enqueue(uint64_t k) {
+        const hipStream_t s = t->stream;
+        const int32_t* done = &t->st->done;
+        T* dold = dd[k & 1];
+        T* dnew = dd[(k + 1) & 1];
+        StencilIO<T> io;
+        io.r = r;
+        io.dold = dold;
+        io.dnew = dnew;
+        io.beta = &t->st->beta;
+        io.y = q;
+        io.partial = t->partial.p;
+        io.done = done;
+        if (L.scr) {
+            T* sb = sbuf_for<T>(t);
+            launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, r, dold, &t->st->beta);
+            io.s = sb;
         }
+        int nb = 0;
+        launch_stencil<T>(L, OP_DIR_APPLY, io, &nb, t->tile, s);
+        applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_ALPHA, 0, s);
+        if (refresh > 0 && (k + 1) % (uint64_t)refresh == 0) {
+            launch_axpy_alpha<T>(x, dnew, t->st, L.N, s);
+            nb = op_pass<T>(t, L, OP_RESID, x, b, r, done, t->partial.p);
+            applies++;
+        } else {
+            launch_cg_update<T>(x, r, dnew, q, t->st, L.N, t->partial.p, &nb, s);
+        }
+        launch_fin(t->st, t->partial.p, nb, FIN_BETA, 0, s);
+    }
+    // Iterate until the device flag is set or `cap` iterations are complete.
+    void run(uint64_t cap) {
+        const int chunk = L.N >= (1 << 21) ? 8 : 32;
+        uint64_t k = it;
+        for (;;) {
+            for (int c = 0; c < chunk && k < cap; ++c) enqueue(k++);
+            read_state(t);
+            if (t->st_host->done || k >= cap) break;
+        }
+        it = (uint64_t)t->st_host->it;
+    }
+    // Replace the recursive residual by `delta_true` = |r|^2 of a true
+    // residual already written to r.  While the two residuals still agree
+    // (within 4x) CG continues along its current direction with beta =
+    // delta_true / delta_prev; otherwise it restarts from the true residual.
+    void replace_residual(double delta_true, double thr) {
+        CgState h = *t->st_host;
+        const bool drifted = !(delta_true <= 4.0 * h.delta);
+        h.beta = (!drifted && h.delta_prev > 0.0) ? delta_true / h.delta_prev : 0.0;
+        h.delta = delta_true;
+        h.thr = thr;
+        h.done = 0;
+        h.status = 0;
+        *t->st_host = h;
+        PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, t->stream));
+    }
+};
+
+template <typename T>
+static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d0, T* d1, T* q, int refresh) {
+    Cg<T> c;
+    c.t = t;
+    c.L = L;
+    c.x = x;
+    c.b = b;
+    c.r = r;
+    c.dd[0] = d0;
+    c.dd[1] = d1;
+    c.q = q;
+    c.refresh = refresh;
+    return c;
+}
+
+// Faithful solve_cg: stop when delta <= delta0*eps^2 (REFERENCE) or, in TRUE
+// mode, when the recursive residual meets eps*||b|| and the true residual
+// confirms it (otherwise continue from the true residual).
+template <typename T>
+static CgOut run_cg(Cg<T>& cg, uint64_t max_iters, double eps, int mode, double bnorm2) {
+    CgOut out;
+    prx_tree* t = cg.t;
+    cg.start(false, mode, eps, max_iters, bnorm2);
+    double last_rt = INFINITY;
+    bool stalled = false;
+    for (;;) {
+        cg.run(max_iters);
+        const CgState h = *t->st_host;
+        if (mode != PRX_RESID_TRUE || !h.done || h.status != PRX_OK || h.iters == 0) break;
+        int nb = op_pass<T>(t, cg.L, OP_RESID, cg.x, cg.b, cg.r, nullptr, t->partial.p);
+        cg.applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 1, t->stream);
         read_state(t);
-        const CgState& h = *t->st_host;
-        if (h.done || it >= max_iters) {
-            if (mode == PRX_RESID_TRUE && h.done && h.status == PRX_OK && h.iters > 0) {
-                // confirm on the true residual; restart from it if it misses
-                nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
-                out.applies++;
-                launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 1, s);
-                read_state(t);
-                const double rt = t->st_host->slot[1];
-                if (rt > (bnorm2 * eps) * eps && it < max_iters) {
-                    CgState hs = *t->st_host;
-                    hs.delta = rt;
-                    hs.done = 0;
-                    *t->st_host = hs;
-                    PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
-                    PRX_HIP(hipMemcpyAsync(d, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-                    continue;
-                }
-                t->st_host->delta = rt;
-            }
-            finished = true;
+        const double rt = t->st_host->slot[1];
+        // stop when the true residual meets the target, or stops improving
+        // (the attainable accuracy of the arithmetic has been reached)
+        stalled = rt > 0.25 * last_rt;
+        if (rt <= h.thr || cg.it >= max_iters || stalled) {
+            t->st_host->delta = rt;
+            if (rt > h.thr) t->st_host->status = PRX_ERR_NOT_CONVERGED;
+            break;
         }
+        last_rt = rt;
+        cg.replace_residual(rt, h.thr);
     }
     const CgState& h = *t->st_host;
-    out.iters = h.done ? (uint64_t)h.iters : max_iters;
-    if (!h.done) out.iters = it;
+    out.iters = h.done ? (uint64_t)h.iters : cg.it;
+    if (h.done && h.status == PRX_OK && mode == PRX_RESID_TRUE) out.iters = cg.it;
     out.status = h.done ? h.status : PRX_ERR_NOT_CONVERGED;
     const double den = mode == PRX_RESID_TRUE ? bnorm2 : h.delta0;
     out.rel = den > 0.0 ? std::sqrt(h.delta / den) : 0.0;
+    out.applies = cg.applies;
     return out;
 }
 
@@ -348,11 +437,12 @@ static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, d
     L.res = 1u << cdepth;
     L.N = nc;
     L.k64 = {stiffness_entry(hc, 0, 0, 0), stiffness_entry(hc, 1, 1, 0), stiffness_entry(hc, 1, 1, 1), epw > 0.0 ? wh3c * 1.005 : 0.0, 0.0};
-    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(nc); t->cdd.ensure(nc); t->cq.ensure(nc);
+    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(nc); t->cdd.ensure(2 * nc); t->cq.ensure(nc);
     launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
     launch_fill<double>(t->cx.p, 0.0, nc, s);
     gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
-    CgOut c = run_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cq.p, iters, eps, PRX_RESID_REFERENCE, 10, 0.0, false);
+    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, 10);
+    CgOut c = run_cg<double>(cg, iters, eps, PRX_RESID_REFERENCE, 0.0);
     rep.coarse_iters = c.iters;
     rep.coarse_rel = c.rel;
     rep.applies += c.applies;
@@ -373,7 +463,7 @@ static double true_rel(prx_tree* t) {
 static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
-    t->r64.ensure(n); t->d64.ensure(n); t->q64.ensure(n);
+    t->r64.ensure(n); t->d64.ensure(2 * n); t->q64.ensure(n);
     {
         Timer tm(t->stream);
         gs_sweeps<double>(t, L, t->x.p, t->b.p, c.gs);
@@ -382,8 +472,8 @@ static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_so
     Timer tm(t->stream);
     double bb = 0.0;
     if (c.o.residual_mode == PRX_RESID_TRUE) bb = dev_dot<double>(t, t->b.p, t->b.p, n);
-    CgOut o = run_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->q64.p, cap, c.eps, c.o.residual_mode,
-                             c.o.refresh_period, bb, false);
+    Cg<double> cg = make_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, c.o.refresh_period);
+    CgOut o = run_cg<double>(cg, cap, c.eps, c.o.residual_mode, bb);
     rep.ms_fine_cg = tm.ms();
     rep.fine_iters = o.iters;
     rep.fine_rel = o.rel;
@@ -392,55 +482,77 @@ static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_so
     rep.status = o.status;
 }
 
-// Mixed precision: the fine correction equations A e = r are solved with the
-// FP32 operator (scaled by 1/h), FP32 vectors and FP64 CG scalars; the FP64
-// loop keeps the solution, forms the true residual b - A x and decides
-// convergence.
+// Mixed precision: CG runs on the correction e (FP32 vectors, the FP32
+// operator scaled by 1/h, FP64 scalars and reductions) while the FP64 solution
+// x stays in HBM untouched.  Whenever the FP32 recursive residual has fallen
+// by the factor inner_tol since the last replacement (or claims convergence),
+// x += e is folded into the FP64 solution, the FP64 true residual b - A x is
+// formed in the same pass and replaces the FP32 residual, and CG continues
+// along its current direction (a "reliable update"), so the Krylov space is
+// kept across the FP64 corrections.  Convergence is decided on the FP64 true
+// residual only.
 static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
     const hipStream_t s = t->stream;
-    t->e32.ensure(n); t->r32.ensure(n); t->ri32.ensure(n); t->d32.ensure(n); t->q32.ensure(n);
+    t->e32.ensure(n); t->ri32.ensure(n); t->d32.ensure(2 * n); t->q32.ensure(n);
     const double ih = 1.0 / t->h;
-    double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
+    const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.1;
+    const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
     // smoothing on the correction: e = GS^k(0; A/h, r/h), x += e
     {
         Timer tm(s);
         launch_fill<float>(t->e32.p, 0.0f, n, s);
-        op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
-        if (c.gs > 0) {
-            gs_sweeps<float>(t, L, t->e32.p, t->r32.p, c.gs);
-        }
+        op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
+        if (c.gs > 0) gs_sweeps<float>(t, L, t->e32.p, t->ri32.p, c.gs);
         rep.ms_fine_gs = tm.ms();
     }
     Timer tm(s);
-    int nb = op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
+    int nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
     rep.applies += 2;
     launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
     read_state(t);
     double rr = t->st_host->slot[0];
     const double r0 = rr;
     const double thr = c.o.residual_mode == PRX_RESID_TRUE ? (bb * c.eps) * c.eps : (r0 * c.eps) * c.eps;
-    uint64_t used = 0, outer = 0;
+    const double thr32 = thr * ih * ih;  // the same test on the scaled FP32 residual
+    uint64_t outer = 0;
     int status = PRX_OK;
-    while (rr > thr && rr > 0.0) {
-        if (used >= cap || outer >= c.o.max_outer) { status = PRX_ERR_NOT_CONVERGED; break; }
+    Cg<float> cg = make_cg<float>(t, L, t->e32.p, nullptr, t->ri32.p, t->d32.p, t->d32.p + n, t->q32.p, 0);
+    if (rr > thr && rr > 0.0) {
         launch_fill<float>(t->e32.p, 0.0f, n, s);
-        CgOut o = run_cg<float>(t, L, t->e32.p, t->r32.p, t->ri32.p, t->d32.p, t->q32.p, cap - used, c.o.inner_tol,
-                                PRX_RESID_REFERENCE, 0, 0.0, true);
-        used += o.iters;
-        rep.applies += o.applies;
-        nb = op_resid_update(t, L, t->e32.p, t->b.p, t->r32.p, ih);
-        rep.applies++;
-        launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+        cg.start_from_residual(1.0, cap);
         read_state(t);
-        rr = t->st_host->slot[0];
-        ++outer;
-        if (o.iters == 0) { status = PRX_ERR_BREAKDOWN; break; }
+        cg.replace_residual(t->st_host->delta, std::max(theta * theta * t->st_host->delta, thr32));
+        for (;;) {
+            cg.run(cap);
+            const CgState h = *t->st_host;
+            // fold e into x and replace the residual by the FP64 true one
+            nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
+            rep.applies++;
+            launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+            read_state(t);
+            rr = t->st_host->slot[0];
+            ++outer;
+            if (rr <= thr) break;
+            if (!h.done || h.status == PRX_ERR_NOT_CONVERGED || cg.it >= cap || outer >= c.o.max_outer) {
+                status = PRX_ERR_NOT_CONVERGED;
+                break;
+            }
+            if (h.status == PRX_ERR_BREAKDOWN) {
+                status = PRX_ERR_BREAKDOWN;
+                break;
+            }
+            launch_fill<float>(t->e32.p, 0.0f, n, s);
+            const double dt = dev_dot<float>(t, t->ri32.p, t->ri32.p, n);
+            *t->st_host = h;
+            cg.replace_residual(dt, std::max(theta * theta * dt, thr32));
+        }
     }
     rep.ms_fine_cg = tm.ms();
-    rep.fine_iters = used;
+    rep.fine_iters = cg.it;
     rep.outer_iters = outer;
+    rep.applies += cg.applies;
     const double den = c.o.residual_mode == PRX_RESID_TRUE ? bb : r0;
     rep.fine_rel = den > 0.0 ? std::sqrt(rr / den) : 0.0;
     rep.status = status;
@@ -682,7 +794,7 @@ void prx_params_default(prx_params* p) {
     p->gs_iters = 20;
     p->base_depth = -1;
     p->refresh_period = 10;
-    p->inner_tol = 1e-3;
+    p->inner_tol = 0.1;
     p->max_outer = 64;
 }
 
@@ -692,7 +804,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->residual_mode = PRX_RESID_REFERENCE;
     o->refresh_period = 10;
     o->gs_iters = 20;
-    o->inner_tol = 1e-3;
+    o->inner_tol = 0.1;
     o->max_outer = 64;
 }
 
@@ -954,8 +1066,9 @@ int prx_cg(prx_tree* t, int precision, const double* b, double* x, uint64_t max_
     PRX_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
     PRX_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
     double bb = residual_mode == PRX_RESID_TRUE ? prx::dev_dot<double>(t, db.p, db.p, n) : 0.0;
-    prx::CgOut o = prx::run_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->q64.p, max_iters, eps, residual_mode,
-                                       refresh_period, bb, false);
+    t->d64.ensure(2 * n);
+    prx::Cg<double> cg = prx::make_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, refresh_period);
+    prx::CgOut o = prx::run_cg<double>(cg, max_iters, eps, residual_mode, bb);
     PRX_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     PRX_HIP(hipStreamSynchronize(s));
     if (iters) *iters = o.iters;

@@ -154,7 +154,7 @@ class FEMTree:
     # ------------------------------------------------------------------ solve
     def solve_cascadic(self, points, pw: float, base_depth: int, gs: int, cg: int, eps: float,
                        precision: int = L.PRX_PREC_FP64, residual_mode: int = L.PRX_RESID_REFERENCE,
-                       refresh_period: int = 10, inner_tol: float = 1e-3, max_outer: int = 64) -> dict:
+                       refresh_period: int = 10, inner_tol: float = 0.1, max_outer: int = 64) -> dict:
         """src/fem_tree.rs:458-487; returns the solve report."""
         self._assemble(points, pw)
         o = L.default_solve_opts()

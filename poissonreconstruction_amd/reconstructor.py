@@ -31,7 +31,7 @@ class ReconstructParams:
     gs_iters: int = 20
     base_depth: int = -1
     refresh_period: int = 10
-    inner_tol: float = 1e-3
+    inner_tol: float = 0.1
     max_outer: int = 64
 
     def to_c(self) -> L.Params:

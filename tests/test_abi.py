@@ -1,0 +1,92 @@
+"""The C ABI surface on the CPU: the library loads, exports every function
+include/prx.h declares, and reports the reference's defaults; compute entry
+points fail with PRX_ERR_DEVICE (never fall back) when no GPU is visible."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import poissonreconstruction_amd as pr
+from poissonreconstruction_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "prx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(prx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(L.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(declared_functions()) <= exported
+
+
+def test_params_defaults_match_the_reference():
+    # src/reconstructor.rs:14 defaults (8, 1.5, 1.1, 1e-3, 500, 0.0, false)
+    p = L.default_params()
+    assert (p.depth, p.samples_per_node, p.scale, p.cg_accuracy, p.cg_iters, p.point_weight, p.verbose) == \
+        (8, 1.5, 1.1, 1e-3, 500, 0.0, 0)
+    assert p.base_depth == -1 and p.gs_iters == 20 and p.refresh_period == 10
+    d = pr.ReconstructParams()
+    assert (d.depth, d.samples_per_node, d.scale, d.cg_accuracy, d.cg_iters, d.point_weight, d.verbose) == \
+        (8, 1.5, 1.1, 1e-3, 500, 0.0, False)
+
+
+def test_struct_layouts_match_the_header():
+    # the ctypes mirrors must have the C layout: compare sizes with a C compile of the header
+    src = os.path.join(ROOT, "include")
+    prog = r'''
+#include <stdio.h>
+#include "prx.h"
+int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(prx_point), sizeof(prx_params), sizeof(prx_stats),
+ sizeof(prx_mesh), sizeof(prx_solve_opts), sizeof(prx_solve_report));return 0;}'''
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "s")
+        subprocess.run(["gcc", "-I", src, c, "-o", exe], check=True)
+        sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split()))
+    assert sizes == [L.POINT_DTYPE.itemsize, ctypes.sizeof(L.Params), ctypes.sizeof(L.Stats), ctypes.sizeof(L.Mesh),
+                     ctypes.sizeof(L.SolveOpts), ctypes.sizeof(L.SolveReport)]
+
+
+def test_unit_transform_is_host_code_and_matches_the_oracle():
+    import oracle
+    from poissonreconstruction_amd import synthetic
+    raw = synthetic.torus(2000, seed=3)
+    up, m, mi = L.unit_transform(raw, 1.1)
+    mo, mio = oracle.unit_transform(raw, 1.1)
+    uo = oracle.apply_transform(raw, mo)
+    assert np.array_equal(m, mo) and np.array_equal(mi, mio)
+    for f in ("px", "py", "pz", "nx", "ny", "nz"):
+        assert np.array_equal(up[f], uo[f])
+
+
+@pytest.mark.skipif(L.device_count() > 0, reason="checks the no-GPU behaviour")
+def test_no_gpu_means_loud_failure():
+    with pytest.raises(pr.PrxError) as e:
+        pr.FEMTree(4)
+    assert e.value.code == L.PRX_ERR_DEVICE
+    rc = L.lib().prx_csr_spmv(0, None, None, None, None, None)
+    assert rc in (L.PRX_ERR_INVALID, L.PRX_ERR_DEVICE)
+
+
+def test_empty_input_returns_empty_mesh_without_device():
+    # src/reconstructor.rs:28: empty input -> empty mesh and default stats (no device needed)
+    mesh, st = pr.reconstruct(np.zeros((0, 6)), pr.ReconstructParams())
+    assert len(mesh.vertices) == 0 and st.fem_nodes == 0 and st.time_total == 0.0
