@@ -216,6 +216,22 @@ int prx_unit_transform(const prx_point* points, size_t n, double scale, double* 
 int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
 
+/* ------------------------------------------------------------ adjoint
+ * No counterpart in the reference (it is not differentiable): the vjp of the
+ * field at the splatted points, F_p = evaluate(x, q_p) with x the current
+ * converged solution, through the splat, RHS, screening and solve.  Given
+ * dL/dF [P] (input point order) it writes dL/dq [3P] and dL/dn [3P] (AoS,
+ * unit-cube coordinates, the point transform held fixed).  The adjoint system
+ * A lam = dL/dx is solved to ||r|| <= eps ||dL/dx|| in `precision`. */
+typedef struct prx_vjp_report {
+    uint64_t iters;
+    double rel;
+    double ms;
+    int32_t status;
+} prx_vjp_report;
+int prx_vjp_points(prx_tree* t, const double* dL_dF, double* dL_dpos, double* dL_dnrm, int precision, double eps,
+                   prx_vjp_report* report);
+
 /* ------------------------------------------------------------ generic CSR
  * SparseMatrix::multiply_vector (src/sparse.rs:105-117) and solve_cg
  * (src/solvers.rs:20-69) on a caller CSR, run on the device. */

@@ -103,6 +103,11 @@ def lib():
         L.orc_iso.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64]
         L.orc_solve_cascadic.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_double,
                                          ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _vp, _vp]
+        L.orc_vjp_points.restype = ctypes.c_int64
+        L.orc_vjp_points.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_double, _vp, _vp,
+                                     ctypes.c_double, _vp, _vp, _vp]
+        L.orc_field_at_points.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_double,
+                                          ctypes.c_double, _vp, _vp]
         L.orc_stiffness_entry.restype = ctypes.c_double
         L.orc_stiffness_entry.argtypes = [ctypes.c_double, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
         L.orc_num_threads.restype = ctypes.c_int
@@ -302,6 +307,30 @@ def solve_cascadic(depth, g2i, i2o, nf, points, pw, base=None, gs=20, cg_iters=5
     lib().orc_solve_cascadic(depth, _p(g2i), _p(i2o), _p(np.ascontiguousarray(nf)), _p(pts), len(pts), float(pw),
                              int(base), int(gs), int(cg_iters), float(eps), _p(sol), ctypes.addressof(rep))
     return sol, {k: getattr(rep, k) for k, _ in _SolveReport._fields_}
+
+
+def field_at_points(depth, points, pw, eps=1e-13):
+    """(F at the points, solution) after an FP64 CG solve of the assembled
+    system from zero to ||r|| <= eps ||b||."""
+    pts = points_array(points)
+    g2i, i2o = tree(depth)
+    x = np.empty(1 << (3 * depth))
+    F = np.empty(len(pts))
+    lib().orc_field_at_points(depth, _p(g2i), _p(i2o), _p(pts), len(pts), float(pw), float(eps), _p(x), _p(F))
+    return F, x
+
+
+def vjp_points(depth, points, pw, x, gF, eps=1e-13):
+    """(d L/d positions [P,3], d L/d normals [P,3], lambda) for dL/dF = gF
+    at the points, with x the converged solution (unit-cube coordinates)."""
+    pts = points_array(points)
+    g2i, i2o = tree(depth)
+    gpos = np.empty((len(pts), 3))
+    gnrm = np.empty((len(pts), 3))
+    lam = np.empty(1 << (3 * depth))
+    lib().orc_vjp_points(depth, _p(g2i), _p(i2o), _p(pts), len(pts), float(pw), _p(np.ascontiguousarray(x)),
+                         _p(np.ascontiguousarray(gF, dtype=np.float64)), float(eps), _p(lam), _p(gpos), _p(gnrm))
+    return gpos, gnrm, lam
 
 
 def reconstruct_field(points, depth=8, scale=1.1, cg_accuracy=1e-3, cg_iters=500, point_weight=0.0):

@@ -668,3 +668,155 @@ void orc_set_threads(int t) {
     (void)t;
 #endif
 }
+
+/* ---------------------------------------------------------------- adjoint
+ * Gradient of L(F) with F_p = evaluate(x, q_p) at the splatted points
+ * themselves, x = A^-1 b the converged solution, with respect to the unit-cube
+ * positions q_p and normals n_p (the transform is held fixed).  Not part of
+ * the reference; it differentiates the reference's own forward path:
+ *   splat  nf_i = -N_i/W_i, N_i = sum_p n_p w_ip, W_i = sum_p w_ip  (floor cells)
+ *   RHS    b_i = h^2 sum_d nf_{i+d}.g(d) + wh3/2 sum_p phi_ip          (trunc cells)
+ *   matrix A = L + wg I + sum_p wh3 phi_p phi_p^T
+ *   field  F_p = sum_o x_o N1(s-o) (with the reference's clamping)
+ * dL/dtheta = gF.dF/dtheta|_x + lam^T (db/dtheta - dA/dtheta x),  A lam = dL/dx.
+ * Derivatives of the piecewise-linear weights are taken on the open pieces
+ * (kinks at cell faces have measure zero). */
+static inline double dbspline1(double t) { if (t > 0.0 && t < 1.0) return 1.0; if (t > 1.0 && t < 2.0) return -1.0; return 0.0; }
+
+static void eval_basis(int64_t res, double h, const double q[3], int64_t o[3][2], double b[3][2], double db[3][2]) {
+    for (int k = 0; k < 3; ++k) {
+        double s = q[k] / h;
+        int64_t f = sat_i64(floor(s));
+        o[k][0] = clampi(wadd(f, -1), 0, res - 1);
+        o[k][1] = clampi(f, 0, res - 1);
+        for (int a = 0; a < 2; ++a) {
+            b[k][a] = bspline1(s - (double)o[k][a]);
+            db[k][a] = dbspline1(s - (double)o[k][a]) / h;
+        }
+    }
+}
+
+/* corners of a floor (mode 0) or trunc (mode 1) cell: node, weight, d weight/dq */
+static int cell_basis(int mode, const double q[3], double h, int64_t res, const int64_t* g2i, int64_t idx[8], double w[8],
+                      double dw[8][3]) {
+    int64_t c[3];
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = mode == 0 ? sat_i64(floor(q[a] / h)) : sat_i64(q[a] / h);
+        t[a] = q[a] / h - (double)c[a];
+    }
+    int nb = 0;
+    for (int dz = 0; dz <= 1; ++dz) for (int dy = 0; dy <= 1; ++dy) for (int dx = 0; dx <= 1; ++dx) {
+        const int d[3] = {dx, dy, dz};
+        int ok = 1;
+        uint64_t o[3];
+        for (int a = 0; a < 3; ++a) {
+            if (mode == 0) { o[a] = (uint64_t)wadd(c[a], d[a]); if (o[a] >= (uint64_t)res) ok = 0; }
+            else { uint32_t u = (uint32_t)(uint64_t)wadd(c[a], d[a]); o[a] = u; if (u >= (uint32_t)res) ok = 0; }
+        }
+        if (!ok) continue;
+        double bb[3], db[3];
+        for (int a = 0; a < 3; ++a) { bb[a] = d[a] ? t[a] : 1.0 - t[a]; db[a] = (d[a] ? 1.0 : -1.0) / h; }
+        idx[nb] = idx_at(g2i, res, o[0], o[1], o[2]);
+        w[nb] = bb[0] * bb[1] * bb[2];
+        dw[nb][0] = db[0] * bb[1] * bb[2];
+        dw[nb][1] = bb[0] * db[1] * bb[2];
+        dw[nb][2] = bb[0] * bb[1] * db[2];
+        ++nb;
+    }
+    return nb;
+}
+
+int64_t orc_vjp_points(uint32_t D, const int64_t* g2i, const uint32_t* i2o, const orc_point* pts, int64_t P, double pw,
+                       const double* x, const double* gF, double cg_eps, double* lam_out, double* gpos, double* gnrm) {
+    const int64_t res = (int64_t)1 << D, n = res * res * res;
+    const double h = 1.0 / (double)res;
+    double* nf = (double*)malloc(3 * n * sizeof(double));
+    double* W = (double*)malloc(n * sizeof(double));
+    orc_splat(D, g2i, pts, P, nf, W);
+    orc_csr* A = orc_assemble(D, g2i, i2o, pts, P, pw);
+    double* gx = (double*)calloc(n, sizeof(double));
+    for (int64_t p = 0; p < P; ++p) {
+        const double q[3] = {pts[p].px, pts[p].py, pts[p].pz};
+        int64_t o[3][2]; double b[3][2], db[3][2];
+        eval_basis(res, h, q, o, b, db);
+        for (int a = 0; a < 2; ++a) for (int c = 0; c < 2; ++c) for (int e = 0; e < 2; ++e) {
+            int64_t i = idx_at(g2i, res, o[0][a], o[1][c], o[2][e]);
+            gx[i] += gF[p] * b[0][a] * b[1][c] * b[2][e];
+        }
+    }
+    double* lam = (double*)calloc(n, sizeof(double));
+    int64_t its = orc_cg(A, gx, lam, 200000, cg_eps, NULL);
+    /* d L / d nf_j = h^2 sum_{i = j - d} lam_i g(d) */
+    double* gnf = (double*)calloc(3 * n, sizeof(double));
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t jx = i2o[3 * j], jy = i2o[3 * j + 1], jz = i2o[3 * j + 2];
+        for (int64_t dx = -1; dx <= 1; ++dx) for (int64_t dy = -1; dy <= 1; ++dy) for (int64_t dz = -1; dz <= 1; ++dz) {
+            int64_t ix = jx - dx, iy = jy - dy, iz = jz - dz;
+            if (ix < 0 || ix >= res || iy < 0 || iy >= res || iz < 0 || iz >= res) continue;
+            double l = lam[idx_at(g2i, res, ix, iy, iz)] * (h * h);
+            gnf[3 * j] += l * deriv_mass_1d(dx) * mass_1d(dy) * mass_1d(dz);
+            gnf[3 * j + 1] += l * mass_1d(dx) * deriv_mass_1d(dy) * mass_1d(dz);
+            gnf[3 * j + 2] += l * mass_1d(dx) * mass_1d(dy) * deriv_mass_1d(dz);
+        }
+    }
+    const double wh3 = pw > 0.0 ? pw * (h * h * h) / 1.5 : 0.0;
+    for (int64_t p = 0; p < P; ++p) {
+        const double q[3] = {pts[p].px, pts[p].py, pts[p].pz};
+        const double nv[3] = {pts[p].nx, pts[p].ny, pts[p].nz};
+        double gq[3] = {0, 0, 0}, gn[3] = {0, 0, 0};
+        /* explicit dF/dq */
+        int64_t o[3][2]; double b[3][2], db[3][2];
+        eval_basis(res, h, q, o, b, db);
+        for (int a = 0; a < 2; ++a) for (int c = 0; c < 2; ++c) for (int e = 0; e < 2; ++e) {
+            double xv = x[idx_at(g2i, res, o[0][a], o[1][c], o[2][e])];
+            gq[0] += gF[p] * xv * db[0][a] * b[1][c] * b[2][e];
+            gq[1] += gF[p] * xv * b[0][a] * db[1][c] * b[2][e];
+            gq[2] += gF[p] * xv * b[0][a] * b[1][c] * db[2][e];
+        }
+        /* splat */
+        int64_t idx[8]; double w[8], dw[8][3];
+        int nb = cell_basis(0, q, h, res, g2i, idx, w, dw);
+        for (int k = 0; k < nb; ++k) {
+            int64_t i = idx[k];
+            if (!(W[i] > 0.0)) continue;
+            const double* g = gnf + 3 * i;
+            for (int c = 0; c < 3; ++c) gn[c] += -(w[k] / W[i]) * g[c];
+            double s = 0.0;
+            for (int c = 0; c < 3; ++c) s += g[c] * (-(nv[c] + nf[3 * i + c]) / W[i]);
+            for (int a = 0; a < 3; ++a) gq[a] += s * dw[k][a];
+        }
+        /* screening RHS and matrix */
+        if (pw > 0.0) {
+            nb = cell_basis(1, q, h, res, g2i, idx, w, dw);
+            double px = 0.0, pl = 0.0, dpx[3] = {0, 0, 0}, dpl[3] = {0, 0, 0};
+            for (int k = 0; k < nb; ++k) {
+                px += w[k] * x[idx[k]];
+                pl += w[k] * lam[idx[k]];
+                for (int a = 0; a < 3; ++a) { dpx[a] += dw[k][a] * x[idx[k]]; dpl[a] += dw[k][a] * lam[idx[k]]; }
+            }
+            for (int a = 0; a < 3; ++a) gq[a] += wh3 * 0.5 * dpl[a] - wh3 * (dpl[a] * px + pl * dpx[a]);
+        }
+        for (int a = 0; a < 3; ++a) { gpos[3 * p + a] = gq[a]; gnrm[3 * p + a] = gn[a]; }
+    }
+    if (lam_out) memcpy(lam_out, lam, n * sizeof(double));
+    free(nf); free(W); free(gx); free(lam); free(gnf); orc_csr_free(A);
+    return its;
+}
+
+/* F at the points after a full tight solve: the forward of the vjp above. */
+void orc_field_at_points(uint32_t D, const int64_t* g2i, const uint32_t* i2o, const orc_point* pts, int64_t P, double pw,
+                         double eps, double* x_out, double* F) {
+    const int64_t n = (int64_t)1 << (3 * D);
+    double* nf = (double*)malloc(3 * n * sizeof(double));
+    double* W = (double*)malloc(n * sizeof(double));
+    orc_splat(D, g2i, pts, P, nf, W);
+    orc_csr* A = orc_assemble(D, g2i, i2o, pts, P, pw);
+    double* b = (double*)malloc(n * sizeof(double));
+    orc_rhs(D, g2i, i2o, nf, pts, P, pw, b);
+    double* x = (double*)calloc(n, sizeof(double));
+    orc_cg(A, b, x, 200000, eps, NULL);
+    for (int64_t p = 0; p < P; ++p) F[p] = orc_evaluate1(D, g2i, x, pts[p].px, pts[p].py, pts[p].pz);
+    if (x_out) memcpy(x_out, x, n * sizeof(double));
+    free(nf); free(W); free(b); free(x); orc_csr_free(A);
+}

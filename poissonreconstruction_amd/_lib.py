@@ -76,6 +76,10 @@ class SolveReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class VjpReport(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_uint64), ("rel", ctypes.c_double), ("ms", ctypes.c_double), ("status", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int
 _u32 = ctypes.c_uint32
@@ -116,6 +120,7 @@ SIGNATURES = {
     "prx_unit_transform": (_i32, [_vp, _sz, _f64, _vp, _vp, _vp]),
     "prx_time_operator": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
+    "prx_vjp_points": (_i32, [_vp, _vp, _vp, _vp, _i32, _f64, _vp]),
     "prx_csr_spmv": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "prx_csr_cg": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _u64, _f64, _vp]),
 }

@@ -97,17 +97,60 @@ __global__ void k_scatter_heads(const uint32_t* key, const uint32_t* head, const
 }
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
-// weight of corner k of point p, (bx*by)*bz as in src/fem_tree.rs:155-158 / :215-218
+// evaluate's corner offsets (src/fem_tree.rs:645-649): (clamp(f-1), clamp(f)), f = floor(s)
+__device__ inline int64_t eval_off(double s, int64_t res, int which) {
+    const int64_t f = sat_i64(floor(s));
+    const int64_t o = which ? f : wadd(f, -1);
+    return o < 0 ? 0 : (o > res - 1 ? res - 1 : o);
+}
+__device__ inline double bspline1_d(double t) {
+    if (t < 0.0 || t > 2.0) return 0.0;
+    if (t <= 1.0) return t;
+    return 2.0 - t;
+}
+
+// weight of corner k of point p: (bx*by)*bz as in src/fem_tree.rs:155-158 /
+// :215-218 for the splat (mode 0) and screening (mode 1) cells, and the
+// evaluate weights N1(s-o) (src/fem_tree.rs:498-507) for mode 2
 __device__ inline double corner_weight(uint32_t D, const double* pos, int64_t P, uint32_t p, int k, int mode) {
     const double h = 1.0 / (double)(1u << D);
     double b[3];
     for (int a = 0; a < 3; ++a) {
         double q = pos[a * P + p];
-        int64_t c = mode == 0 ? cell_floor(q, h) : cell_trunc(q, h);
-        double t = q / h - (double)c;
-        b[a] = ((k >> a) & 1) ? t : 1.0 - t;
+        if (mode == 2) {
+            const double s = q / h;
+            b[a] = bspline1_d(s - (double)eval_off(s, (int64_t)1 << D, (k >> a) & 1));
+        } else {
+            int64_t c = mode == 0 ? cell_floor(q, h) : cell_trunc(q, h);
+            double t = q / h - (double)c;
+            b[a] = ((k >> a) & 1) ? t : 1.0 - t;
+        }
     }
     return b[0] * b[1] * b[2];
+}
+
+// node keys of the 8 evaluate corners of each point (always in range)
+__global__ void k_point_keys_eval(uint32_t D, const double* __restrict__ pos, int64_t P, uint32_t* key, uint32_t* val) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int64_t res = (int64_t)1 << D;
+    const double h = 1.0 / (double)res;
+    int64_t o[3][2];
+    for (int a = 0; a < 3; ++a) {
+        const double s = pos[a * P + p] / h;
+        o[a][0] = eval_off(s, res, 0);
+        o[a][1] = eval_off(s, res, 1);
+    }
+    for (int k = 0; k < 8; ++k) {
+        key[8 * p + k] = morton3((uint32_t)o[0][k & 1], (uint32_t)o[1][(k >> 1) & 1], (uint32_t)o[2][k >> 2]);
+        val[8 * p + k] = (uint32_t)(8 * p + k);
+    }
+}
+
+void launch_point_keys_eval(uint32_t D, const double* pos, int64_t P, uint32_t* key, uint32_t* val, hipStream_t s) {
+    if (P <= 0) return;
+    hipLaunchKernelGGL(k_point_keys_eval, dim3(nblk(P)), dim3(256), 0, s, D, pos, P, key, val);
+    PRX_LAUNCH_CHECK();
 }
 
 __global__ void k_list_entries(uint32_t D, int mode, const uint32_t* code, int64_t M, const double* __restrict__ pos, int64_t P,

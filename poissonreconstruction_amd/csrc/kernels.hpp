@@ -67,6 +67,7 @@ void launch_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const do
 // and trunc (screening) cells; invalid corners get key N.
 void launch_point_keys(uint32_t D, const double* pos, int64_t P, uint32_t* key_floor, uint32_t* key_trunc, uint32_t* val,
                        uint32_t* cell_key, uint32_t* cell_val, hipStream_t s);
+void launch_point_keys_eval(uint32_t D, const double* pos, int64_t P, uint32_t* key, uint32_t* val, hipStream_t s);
 void launch_point_sorted_data(uint32_t D, const double* pos, int64_t P, const uint32_t* perm, uint32_t* cell, double* t64,
                               float* t32, hipStream_t s);
 // Run-length encode the sorted keys into a NodeList (node/off), fill code,
@@ -117,5 +118,15 @@ template <typename T>
 void launch_gs_color(const LevelView& L, int color, T* x, const T* b, const T* sbuf, hipStream_t s);
 
 int partial_capacity(int64_t N);
+
+// ---- k_adjoint.hip
+// gx[node] = sum over the node's evaluate entries of gF[p] * phi (input order)
+void launch_adj_load(const NodeList& E, const double* gF, double* gx, hipStream_t s);
+// gnf[3t] = h^2 sum_d lam_{j-d} g(d) at the splat-list nodes j = F.node[t]
+void launch_adj_gnf(uint32_t D, const NodeList& F, const double* lam, double* gnf, hipStream_t s);
+// per point (input order): d L / d position and d L / d normal (AoS, unit cube)
+void launch_adj_points(uint32_t D, const DevPoints& pts, const NodeList& F, const double* gF, const double* x,
+                       const double* lam, const double* nf, const double* w, const double* gnf, double wh3, double* gpos,
+                       double* gnrm, hipStream_t s);
 
 }  // namespace prx

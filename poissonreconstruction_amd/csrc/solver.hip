@@ -104,11 +104,11 @@ struct prx_tree {
     DArr<uint32_t> perm, cell;
     DArr<float> t32;
     DevPoints pts;
-    NodeList floor_list, trunc_list;
+    NodeList floor_list, trunc_list, eval_list;
     bool splatted = false, assembled = false;
 
     // fields
-    DArr<double> nf, w, b, x, x2;
+    DArr<double> nf, w, b, x, x2, adj_g, adj_lam, adj_gnf, adj_out;
     double pw = 0.0;
     Coef<double> k64{};
     Coef<float> k32{};
@@ -124,10 +124,11 @@ struct prx_tree {
     uint32_t cd = 0;
     DArr<double> cb, cx, cr, cdd, cq, cpart;
 
-    prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; }
+    prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
         free_list(floor_list);
         free_list(trunc_list);
+        free_list(eval_list);
         if (st) (void)hipFree(st);
         if (st_host) (void)hipHostFree(st_host);
         if (stream) (void)hipStreamDestroy(stream);
@@ -602,6 +603,7 @@ static void splat(prx_tree* t) {
     const int64_t P = t->P, M8 = 8 * P;
     free_list(t->floor_list);
     free_list(t->trunc_list);
+    free_list(t->eval_list);
     t->perm.ensure(std::max<int64_t>(P, 1));
     t->cell.ensure(3 * std::max<int64_t>(P, 1));
     t->t64.ensure(3 * std::max<int64_t>(P, 1));
@@ -683,6 +685,90 @@ static bool finite_points(const prx_point* p, size_t n) {
             !std::isfinite(p[i].ny) || !std::isfinite(p[i].nz))
             return false;
     return true;
+}
+
+// Per-node list of the evaluate corners of the points (built on first use).
+static void build_eval_list(prx_tree* t) {
+    if (t->eval_list.T > 0 || t->P == 0) return;
+    const hipStream_t s = t->stream;
+    const int64_t P = t->P, M8 = 8 * P;
+    DArr<uint32_t> k, v, k2, v2, inv;
+    k.ensure(M8); v.ensure(M8); k2.ensure(M8); v2.ensure(M8); inv.ensure(P);
+    launch_point_keys_eval(t->D, t->pos.p, P, k.p, v.p, s);
+    const int kbits = 3 * (int)t->D + 1;
+    size_t need = 0, need2 = 0;
+    PRX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k.p, k2.p, v.p, v2.p, (int)M8, 0, kbits, s));
+    PRX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, k.p, k2.p, (int)M8, s));
+    size_t sb = std::max(need, need2);
+    DArr<uint8_t> scratch;
+    scratch.ensure((int64_t)sb);
+    PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, k.p, k2.p, v.p, v2.p, (int)M8, 0, kbits, s));
+    launch_invert_perm(t->perm.p, P, inv.p, s);
+    build_node_list(t->eval_list, t->D, k2.p, v2.p, M8, t->pts, inv.p, scratch.p, sb, s);
+    PRX_HIP(hipStreamSynchronize(s));
+}
+
+// vjp of the field at the points (see include/prx.h): one adjoint solve with
+// the forward operator (A is symmetric), then point-local terms.
+static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, double* gnrm_host, int precision, double eps,
+                       prx_vjp_report* rep) {
+    if (!t->assembled) throw Error(PRX_ERR_STATE, "vjp before assemble/solve");
+    const hipStream_t s = t->stream;
+    const int64_t N = t->N, P = t->P;
+    Timer tm(s);
+    t->ensure_state();
+    build_eval_list(t);
+    DArr<double> gF;
+    gF.ensure(std::max<int64_t>(P, 1));
+    if (P > 0) PRX_HIP(hipMemcpyAsync(gF.p, gF_host, sizeof(double) * P, hipMemcpyHostToDevice, s));
+    t->adj_g.ensure(N);
+    t->adj_lam.ensure(N);
+    launch_fill<double>(t->adj_g.p, 0.0, N, s);
+    launch_adj_load(t->eval_list, gF.p, t->adj_g.p, s);
+    // solve A lam = g with the fine solver: swap the RHS and solution slots
+    t->b.swap(t->adj_g);
+    t->x.swap(t->adj_lam);
+    prx_solve_report r;
+    std::memset(&r, 0, sizeof r);
+    try {
+        launch_fill<double>(t->x.p, 0.0, N, s);
+        SolveCfg c;
+        c.base = t->D;
+        c.gs = 20;
+        c.cg_iters = 4u * 200000u;
+        c.eps = eps;
+        prx_solve_opts_default(&c.o);
+        c.o.precision = precision;
+        c.o.residual_mode = PRX_RESID_TRUE;
+        c.o.refresh_period = 0;
+        c.o.inner_tol = 0.1;
+        c.o.max_outer = 200;
+        if (precision == PRX_PREC_MIXED) fine_solve_mixed(t, c, 200000, r);
+        else fine_solve_fp64(t, c, 200000, r);
+    } catch (...) {
+        t->x.swap(t->adj_lam);
+        t->b.swap(t->adj_g);
+        throw;
+    }
+    t->x.swap(t->adj_lam);
+    t->b.swap(t->adj_g);
+    // nf-adjoint at the splat nodes, then the per-point terms
+    t->adj_gnf.ensure(3 * std::max<int64_t>(t->floor_list.T, 1));
+    launch_adj_gnf(t->D, t->floor_list, t->adj_lam.p, t->adj_gnf.p, s);
+    t->adj_out.ensure(6 * std::max<int64_t>(P, 1));
+    launch_adj_points(t->D, t->pts, t->floor_list, gF.p, t->x.p, t->adj_lam.p, t->nf.p, t->w.p, t->adj_gnf.p,
+                      t->pw > 0.0 ? t->wh3 : 0.0, t->adj_out.p, t->adj_out.p + 3 * P, s);
+    if (P > 0) {
+        PRX_HIP(hipMemcpyAsync(gpos_host, t->adj_out.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipMemcpyAsync(gnrm_host, t->adj_out.p + 3 * P, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, s));
+    }
+    PRX_HIP(hipStreamSynchronize(s));
+    if (rep) {
+        rep->iters = r.fine_iters;
+        rep->rel = r.fine_rel;
+        rep->ms = tm.ms();
+        rep->status = r.status;
+    }
 }
 
 int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil) {
@@ -1160,6 +1246,16 @@ int prx_iso_value(prx_tree* t, double* iso) {
     launch_fill<double>(ones.p, 1.0, t->P, t->stream);
     const double s = prx::dev_dot<double>(t, v.p, ones.p, t->P);
     *iso = s / (double)t->P;
+    PRX_API_END
+}
+
+int prx_vjp_points(prx_tree* t, const double* gF, double* gpos, double* gnrm, int precision, double eps,
+                   prx_vjp_report* report) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && (gF || t->P == 0) && (gpos || t->P == 0) && (gnrm || t->P == 0), PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(eps > 0.0 && eps < 1.0, PRX_ERR_INVALID, "eps must be in (0,1)");
+    set_device(t);
+    prx::vjp_points(t, gF, gpos, gnrm, precision, eps, report);
     PRX_API_END
 }
 

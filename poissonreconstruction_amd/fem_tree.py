@@ -216,6 +216,20 @@ class FEMTree:
         check(L.lib().prx_evaluate_corners(self._h, ptr(out)))
         return out.reshape(r1, r1, r1)
 
+    def vjp_points(self, dL_dF, precision: int = L.PRX_PREC_MIXED, eps: float = 1e-10):
+        """Gradient of a loss on F = the field at the splatted points (the
+        current solution must be converged): returns (dL/dpositions [P,3],
+        dL/dnormals [P,3], report) in unit-cube coordinates."""
+        g = np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(-1)
+        P = 0 if self._points is None else len(self._points)
+        if g.shape[0] != P:
+            raise ValueError("dL_dF must have one entry per splatted point")
+        gpos = np.empty((P, 3))
+        gnrm = np.empty((P, 3))
+        rep = L.VjpReport()
+        check(L.lib().prx_vjp_points(self._h, ptr(g), ptr(gpos), ptr(gnrm), int(precision), float(eps), ctypes.byref(rep)))
+        return gpos, gnrm, {k: getattr(rep, k) for k, _ in rep._fields_}
+
     def iso_value(self) -> float:
         """Mean field over the splatted points (src/reconstructor.rs:56)."""
         v = ctypes.c_double(0.0)

@@ -1,0 +1,91 @@
+"""Adjoint (vjp) of the field at the points: GPU vs the FP64 oracle gradient,
+GPU vs central finite differences of the GPU forward, and the
+torch.autograd.Function over the same C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+import poissonreconstruction_amd as pr
+from poissonreconstruction_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("px", "py", "pz", "nx", "ny", "nz")
+
+
+def unit_points(raw):
+    m, _ = oracle.unit_transform(raw, 1.1)
+    return oracle.apply_transform(raw, m)
+
+
+def gpu_loss_and_tree(D, pts, pw, eps=1e-12):
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=pr.PRX_PREC_FP64, residual_mode=pr.PRX_RESID_TRUE,
+                     refresh_period=0)
+    q = np.stack([pts["px"], pts["py"], pts["pz"]], axis=1)
+    F = t.evaluate_many(q)
+    return ((F - F.mean()) ** 2).sum(), F, t
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_vjp_matches_oracle_gradient(gpu, prec):
+    D, pw = 5, 4.0
+    pts = unit_points(synthetic.ellipsoids(3000, seed=1))
+    L, F, t = gpu_loss_and_tree(D, pts, pw)
+    gF = 2.0 * (F - F.mean())
+    gpos, gnrm, rep = t.vjp_points(gF, precision=prec, eps=1e-11)
+    Fo, xo = oracle.field_at_points(D, pts, pw, eps=1e-13)
+    gpo, gno, _ = oracle.vjp_points(D, pts, pw, xo, 2.0 * (Fo - Fo.mean()), eps=1e-13)
+    scale_p, scale_n = np.abs(gpo).max(), np.abs(gno).max()
+    assert np.abs(gpos - gpo).max() <= 1e-6 * scale_p
+    assert np.abs(gnrm - gno).max() <= 1e-6 * scale_n
+    assert rep["status"] == 0
+
+
+def test_vjp_matches_central_differences(gpu):
+    D, pw = 6, 4.0
+    pts = unit_points(synthetic.ellipsoids(20000, seed=2))
+    L0, F, t = gpu_loss_and_tree(D, pts, pw)
+    gpos, gnrm, _ = t.vjp_points(2.0 * (F - F.mean()), precision=pr.PRX_PREC_MIXED, eps=1e-11)
+    rng = np.random.default_rng(0)
+    h = 1.0 / (1 << D)
+    step = 1e-5
+    checked = 0
+    worst = 0.0
+    for _ in range(40):
+        p = int(rng.integers(len(pts)))
+        f = FIELDS[int(rng.integers(6))]
+        if f[0] == "p":  # skip coordinates whose +-step crosses a cell face (kink of the weights)
+            s = pts[f][p] / h
+            if abs(s - np.round(s)) * h < 2 * step:
+                continue
+        a, b = pts.copy(), pts.copy()
+        a[f][p] += step
+        b[f][p] -= step
+        fd = (gpu_loss_and_tree(D, a, pw)[0] - gpu_loss_and_tree(D, b, pw)[0]) / (2 * step)
+        an = (gpos if f[0] == "p" else gnrm)[p, "xyz".index(f[1])]
+        scale = max(abs(fd), 1e-3 * np.abs(gpos if f[0] == "p" else gnrm).max())
+        worst = max(worst, abs(fd - an) / scale)
+        checked += 1
+        if checked >= 12:
+            break
+    assert checked >= 8
+    assert worst <= 1e-4
+
+
+def test_torch_autograd_function(gpu):
+    torch = pytest.importorskip("torch")
+    from poissonreconstruction_amd.autograd import field_at_points
+    D, pw = 5, 4.0
+    pts = unit_points(synthetic.sphere(2000, seed=5))
+    q = torch.tensor(np.stack([pts["px"], pts["py"], pts["pz"]], axis=1), requires_grad=True)
+    n = torch.tensor(np.stack([pts["nx"], pts["ny"], pts["nz"]], axis=1), requires_grad=True)
+    F = field_at_points(q, n, D, pw, precision=pr.PRX_PREC_FP64)
+    loss = ((F - F.mean()) ** 2).sum()
+    loss.backward()
+    L0, F0, t = gpu_loss_and_tree(D, pts, pw)
+    gpos, gnrm, _ = t.vjp_points(2.0 * (F0 - F0.mean()), precision=pr.PRX_PREC_FP64, eps=1e-11)
+    assert abs(loss.item() - L0) <= 1e-9 * L0
+    assert np.abs(q.grad.numpy() - gpos).max() <= 1e-7 * np.abs(gpos).max()
+    assert np.abs(n.grad.numpy() - gnrm).max() <= 1e-7 * np.abs(gnrm).max()
