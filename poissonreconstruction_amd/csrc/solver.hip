@@ -426,7 +426,12 @@ struct SolveCfg {
     prx_solve_opts o;
 };
 
-static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, double eps, prx_solve_report& rep) {
+// Coarsest level: FP64 CG.  The reference's test is ||r|| <= eps ||r0|| with
+// r0 the residual after the GS sweeps (src/solvers.rs:36-58), which an eps
+// near machine precision cannot reach once GS has already reduced r0; in
+// TRUE residual mode the test is ||b - A x|| <= eps ||b|| with the same
+// stagnation stop as the fine level.
+static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, double eps, int mode, prx_solve_report& rep) {
     const hipStream_t s = t->stream;
     const int64_t nc = (int64_t)1 << (3 * cdepth);
     const double hc = 1.0 / (double)(1u << cdepth);
@@ -442,8 +447,10 @@ static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, d
     launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
     launch_fill<double>(t->cx.p, 0.0, nc, s);
     gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
-    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, 10);
-    CgOut c = run_cg<double>(cg, iters, eps, PRX_RESID_REFERENCE, 0.0);
+    const bool tru = mode == PRX_RESID_TRUE;
+    const double bb = tru ? dev_dot<double>(t, t->cb.p, t->cb.p, nc) : 0.0;
+    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, tru ? 0 : 10);
+    CgOut c = run_cg<double>(cg, iters, eps, tru ? PRX_RESID_TRUE : PRX_RESID_REFERENCE, bb);
     rep.coarse_iters = c.iters;
     rep.coarse_rel = c.rel;
     rep.applies += c.applies;
@@ -568,7 +575,7 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
         launch_fill<double>(t->x.p, 0.0, t->N, s);
     } else {
         Timer tm(s);
-        coarse_solve(t, c.base, c.gs, c.cg_iters, c.eps, rep);
+        coarse_solve(t, c.base, c.gs, c.cg_iters, c.eps, c.o.residual_mode, rep);
         launch_prolongate(t->D, c.base, t->cx.p, t->x.p, s);
         rep.ms_coarse = tm.ms();
     }
