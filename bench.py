@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-ref", action="store_true", help="skip the FP64 1e-12 reference solve (field error)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--op-reps", type=int, default=20)
     ap.add_argument("--op-only", action="store_true",
                     help="only time the operator kernels and the copy kernel (for counter runs)")
@@ -128,6 +129,8 @@ def main():
 
     roofline = operator_roofline(L, tree, N, P, args.op_reps)
 
+    adjoint = None if args.skip_adjoint else adjoint_bench(pr, L, synthetic, args.tol)
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(raw, D, pw, rep_fp64=fp64[-1][1] if fp64 else rep_mixed)
@@ -148,9 +151,39 @@ def main():
         "field_error": field or None,
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
         "roofline": roofline,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def adjoint_bench(pr, L, synthetic, tol):
+    """vjp of L = sum_p (F(p) - mean F)^2 on the differentiation workload
+    (union of 3 ellipsoids, 200,000 points, depth 8, point weight 4): the
+    forward mixed solve and the adjoint solve (FP32 CG with FP64 refinement,
+    same operator) plus the point-local backward kernels, each to `tol`."""
+    D, pw = 8, 4.0
+    raw = synthetic.ellipsoids(200000, seed=7)
+    unit, _, _ = L.unit_transform(raw, 1.1)
+    t = pr.FEMTree(D)
+    t.splat_normal_field(unit)
+    q = np.stack([unit["px"], unit["py"], unit["pz"]], axis=1)
+    out = {"workload": "union of 3 ellipsoids, 200000 points, depth 8, point weight 4, loss sum (F - mean F)^2"}
+    fwd, bwd = [], []
+    for i in range(3):
+        ts = time.perf_counter()
+        rep = t.solve_cascadic(unit, pw, D - 2, 20, 800000, tol, precision=L.PRX_PREC_MIXED,
+                               residual_mode=L.PRX_RESID_TRUE, refresh_period=0)
+        fwd.append((time.perf_counter() - ts) * 1e3)
+        F = t.evaluate_many(q)
+        ts = time.perf_counter()
+        gpos, gnrm, vrep = t.vjp_points(2.0 * (F - F.mean()), precision=L.PRX_PREC_MIXED, eps=tol)
+        bwd.append((time.perf_counter() - ts) * 1e3)
+    out.update({"forward_ms": round(float(np.median(fwd[1:])), 2), "vjp_ms": round(float(np.median(bwd[1:])), 2),
+                "vjp_device_ms": round(vrep["ms"], 2), "vjp_iters": vrep["iters"], "vjp_rel": vrep["rel"],
+                "forward_fine_iters": rep["fine_iters"], "tol": tol})
+    log(f"adjoint {out}")
+    return out
 
 
 def operator_roofline(L, tree, N, P, reps):
