@@ -232,6 +232,36 @@ typedef struct prx_vjp_report {
 int prx_vjp_points(prx_tree* t, const double* dL_dF, double* dL_dpos, double* dL_dnrm, int precision, double eps,
                    prx_vjp_report* report);
 
+/* ------------------------------------------------------------ batches
+ * No counterpart in the reference (it reconstructs one cloud per call,
+ * src/reconstructor.rs:25): many clouds, each a full FEMTree of its own depth
+ * (6..10), solved together.  prx_batch_add transforms nothing (points are in
+ * the unit cube, as for prx_splat_normal_field), splats and assembles one
+ * instance; prx_solve_batched runs FEMTree::solve_cascadic
+ * (src/fem_tree.rs:458-487, base depth D-2) for every instance over one
+ * concatenated node array, with the CG iterations and GS sweeps replayed as
+ * captured HIP graphs.  FP64 only.  With autotune != 0 the first solve times
+ * the operator brick size, the GS block size and the CG chunk length on the
+ * batch and keeps the fastest (reported).  Instance trees (prx_batch_tree)
+ * are borrowed: evaluate / get_solution / vjp work on them; never destroy
+ * them.  per_instance (optional) receives one report per instance. */
+typedef struct prx_batch prx_batch;
+typedef struct prx_batch_report {
+    uint32_t instances, converged;
+    uint64_t total_nodes, total_points;
+    uint64_t max_fine_iters, sum_fine_iters, max_coarse_iters, graph_launches;
+    double ms_total, ms_coarse, ms_fine_gs, ms_fine_cg, ms_tune;
+    double max_true_rel;   /* max over instances of ||b-Ax||/||b|| */
+    int32_t tile, chunk, gs_block, status;
+} prx_batch_report;
+int prx_batch_create(int device, prx_batch** out);
+void prx_batch_destroy(prx_batch* b);
+int prx_batch_add(prx_batch* b, uint32_t depth, const prx_point* unit_pts, size_t n, double point_weight, uint32_t* index);
+int prx_batch_size(prx_batch* b, uint32_t* count);
+int prx_batch_tree(prx_batch* b, uint32_t index, prx_tree** tree);
+int prx_solve_batched(prx_batch* b, uint32_t gs, uint64_t cg_iters, double eps, const prx_solve_opts* opts, int autotune,
+                      prx_batch_report* report, prx_solve_report* per_instance);
+
 /* ------------------------------------------------------------ generic CSR
  * SparseMatrix::multiply_vector (src/sparse.rs:105-117) and solve_cg
  * (src/solvers.rs:20-69) on a caller CSR, run on the device. */

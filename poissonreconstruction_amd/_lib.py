@@ -80,6 +80,18 @@ class VjpReport(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_uint64), ("rel", ctypes.c_double), ("ms", ctypes.c_double), ("status", ctypes.c_int32)]
 
 
+class BatchReport(ctypes.Structure):
+    _fields_ = [("instances", ctypes.c_uint32), ("converged", ctypes.c_uint32), ("total_nodes", ctypes.c_uint64),
+                ("total_points", ctypes.c_uint64), ("max_fine_iters", ctypes.c_uint64), ("sum_fine_iters", ctypes.c_uint64),
+                ("max_coarse_iters", ctypes.c_uint64), ("graph_launches", ctypes.c_uint64), ("ms_total", ctypes.c_double),
+                ("ms_coarse", ctypes.c_double), ("ms_fine_gs", ctypes.c_double), ("ms_fine_cg", ctypes.c_double),
+                ("ms_tune", ctypes.c_double), ("max_true_rel", ctypes.c_double), ("tile", ctypes.c_int32),
+                ("chunk", ctypes.c_int32), ("gs_block", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int
 _u32 = ctypes.c_uint32
@@ -121,6 +133,12 @@ SIGNATURES = {
     "prx_time_operator": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
     "prx_vjp_points": (_i32, [_vp, _vp, _vp, _vp, _i32, _f64, _vp]),
+    "prx_batch_create": (_i32, [_i32, _vp]),
+    "prx_batch_destroy": (None, [_vp]),
+    "prx_batch_add": (_i32, [_vp, _u32, _vp, _sz, _f64, _vp]),
+    "prx_batch_size": (_i32, [_vp, _vp]),
+    "prx_batch_tree": (_i32, [_vp, _u32, _vp]),
+    "prx_solve_batched": (_i32, [_vp, _u32, _u64, _f64, _vp, _i32, _vp, _vp]),
     "prx_csr_spmv": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "prx_csr_cg": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _u64, _f64, _vp]),
 }

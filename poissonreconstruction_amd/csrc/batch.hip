@@ -1,0 +1,739 @@
+// Batched screened-Poisson solves: K point clouds of uneven size solved
+// together.  Each cloud is a complete FEMTree (its own depth, points, splat,
+// RHS and screening lists, built by the single-tree code), and the solve runs
+// over one concatenated node array: instance k owns nodes [off[k], off[k+1])
+// (each a multiple of 4096), so every 16^3 brick lies in one instance and
+// every batched kernel finds its instance's weights, CG scalars and run flag
+// from the brick.  The per-instance algorithm is the cascadic solve of
+// FEMTree::solve_cascadic (src/fem_tree.rs:458-487) at base depth D-2: coarse
+// FP64 CG after multicolour GS, prolongation (src/fem_tree.rs:310-343), fine
+// GS and CG (src/solvers.rs:20-69), with the stopping rules of the single
+// tree (reference ||r|| <= eps ||r0|| or the FP64 true residual).
+//
+// One CG iteration of all instances is five launches (screening gather,
+// fused direction update + operator, alpha, vector update, beta); a chunk of
+// iterations is captured once into a hipGraph and replayed until every
+// instance is done, and the GS sweeps are captured the same way.  An
+// autotuner picks the operator's brick size, the GS block size and the chunk
+// length by timing them on the batch itself.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "tree.hpp"
+
+namespace prx {
+
+__global__ void k_copy_add_u32(uint32_t* dst, const uint32_t* src, int64_t n, uint32_t add) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + add;
+}
+__global__ void k_fill_u32(uint32_t* dst, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+static unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
+static void copy_add_u32(uint32_t* dst, const uint32_t* src, int64_t n, uint32_t add, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_add_u32, dim3(blocks_for(n)), dim3(256), 0, s, dst, src, n, add);
+    PRX_LAUNCH_CHECK();
+}
+static void fill_u32(uint32_t* dst, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, s, dst, n, v);
+    PRX_LAUNCH_CHECK();
+}
+template <typename T>
+static void dmalloc_n(T** p, int64_t n) {
+    PRX_HIP(hipMalloc((void**)p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+}
+
+// One level (fine or coarse) of all instances, concatenated.
+struct BLevel {
+    int K = 0;
+    int64_t N = 0;
+    std::vector<uint64_t> off_h;
+    std::vector<BCoef> coef_h;
+    DArr<uint64_t> off;
+    DArr<uint32_t> brick_inst;
+    DArr<BCoef> coef;
+    DArr<double> x, b, r, dd, q, sbuf;
+    NodeList scr;  // global-id screening list (fine level)
+    bool has_scr = false;
+    int64_t P = 0;
+    DArr<uint32_t> cell, pinst;
+    DArr<double> t64;
+    DArr<float> t32;
+    BLevel() { scr.mode = 1; }
+    ~BLevel() { free_list(scr); }
+    BPoints pts() const {
+        BPoints p;
+        p.P = P;
+        p.cell = cell.p;
+        p.t64 = t64.p;
+        p.t32 = t32.p;
+        p.inst = pinst.p;
+        return p;
+    }
+    const NodeList* scr_ptr() const { return has_scr ? &scr : nullptr; }
+};
+
+struct GraphEntry {
+    const BLevel* L;
+    int kind, chunk, refresh, tile, gsb;
+    hipGraphExec_t exec;
+};
+
+}  // namespace prx
+
+struct prx_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<prx_tree*> inst;
+    prx::BLevel fine, coarse;
+    bool built = false;
+    prx::DArr<prx::CgState> st;
+    prx::CgState* st_host = nullptr;
+    prx::DArr<double> partial;
+    prx::DArr<int32_t> flags, ones;
+    std::vector<int32_t> flags_h;
+    int tile = 16, chunk = 16, gs_block = 256;
+    bool tuned = false;
+    double tune_ms = 0.0;
+    uint64_t graph_launches = 0;
+    std::vector<prx::GraphEntry> graphs;
+
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+    ~prx_batch() {
+        drop_graphs();
+        for (prx_tree* t : inst) delete t;
+        if (st_host) (void)hipHostFree(st_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace prx {
+
+static BatchTab tab_of(prx_batch* B, const BLevel& L, const int32_t* run, int stride, int when) {
+    BatchTab t;
+    t.K = L.K;
+    t.off = L.off.p;
+    t.brick_inst = L.brick_inst.p;
+    t.coef = L.coef.p;
+    t.st = B->st.p;
+    t.run = run;
+    t.run_stride = stride;
+    t.run_when = when;
+    return t;
+}
+// instances whose CG is still running (done == 0)
+static BatchTab tab_running(prx_batch* B, const BLevel& L) {
+    return tab_of(B, L, &B->st.p[0].done, (int)(sizeof(CgState) / sizeof(int32_t)), 0);
+}
+static BatchTab tab_all(prx_batch* B, const BLevel& L) { return tab_of(B, L, B->ones.p, 1, 1); }
+static BatchTab tab_flagged(prx_batch* B, const BLevel& L) { return tab_of(B, L, B->flags.p, 1, 1); }
+
+static void read_states(prx_batch* B) {
+    PRX_HIP(hipMemcpyAsync(B->st_host, B->st.p, sizeof(CgState) * B->inst.size(), hipMemcpyDeviceToHost, B->stream));
+    PRX_HIP(hipStreamSynchronize(B->stream));
+}
+static void write_states(prx_batch* B) {
+    PRX_HIP(hipMemcpyAsync(B->st.p, B->st_host, sizeof(CgState) * B->inst.size(), hipMemcpyHostToDevice, B->stream));
+}
+static void write_flags(prx_batch* B) {
+    PRX_HIP(hipMemcpyAsync(B->flags.p, B->flags_h.data(), sizeof(int32_t) * B->flags_h.size(), hipMemcpyHostToDevice, B->stream));
+}
+
+static void setup_level(prx_batch* B, BLevel& L, const std::vector<uint32_t>& depth, const std::vector<BCoef>& coef) {
+    const hipStream_t s = B->stream;
+    L.K = (int)depth.size();
+    L.off_h.assign(L.K + 1, 0);
+    for (int k = 0; k < L.K; ++k) L.off_h[k + 1] = L.off_h[k] + ((uint64_t)1 << (3 * depth[k]));
+    L.N = (int64_t)L.off_h[L.K];
+    if (L.N >= ((int64_t)1 << 32)) throw Error(PRX_ERR_INVALID, "a batch holds at most 2^32 - 1 nodes per level");
+    L.coef_h = coef;
+    L.off.ensure(L.K + 1);
+    L.coef.ensure(L.K);
+    PRX_HIP(hipMemcpyAsync(L.off.p, L.off_h.data(), sizeof(uint64_t) * (L.K + 1), hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(L.coef.p, L.coef_h.data(), sizeof(BCoef) * L.K, hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> bi((size_t)(L.N >> 12));
+    for (int k = 0; k < L.K; ++k)
+        for (uint64_t g = L.off_h[k] >> 12; g < L.off_h[k + 1] >> 12; ++g) bi[g] = (uint32_t)k;
+    L.brick_inst.ensure((int64_t)bi.size());
+    PRX_HIP(hipMemcpyAsync(L.brick_inst.p, bi.data(), sizeof(uint32_t) * bi.size(), hipMemcpyHostToDevice, s));
+    L.x.ensure(L.N);
+    L.b.ensure(L.N);
+    L.r.ensure(L.N);
+    L.dd.ensure(2 * L.N);
+    L.q.ensure(L.N);
+    PRX_HIP(hipStreamSynchronize(s));  // bi goes out of scope
+}
+
+// Concatenate the per-instance screening lists and point data (global node
+// ids and sorted point positions).
+static void build_screening(prx_batch* B, BLevel& L) {
+    const hipStream_t s = B->stream;
+    free_list(L.scr);
+    int64_t T = 0, M = 0, P = 0;
+    for (prx_tree* t : B->inst)
+        if (t->pw > 0.0 && t->trunc_list.T > 0) { T += t->trunc_list.T; M += t->trunc_list.M; P += t->P; }
+    L.has_scr = T > 0;
+    L.P = P;
+    if (!L.has_scr) return;
+    if (M >= ((int64_t)1 << 32) || P >= ((int64_t)1 << 32)) throw Error(PRX_ERR_INVALID, "batch screening list too large");
+    NodeList& G = L.scr;
+    G.T = T;
+    G.M = M;
+    G.brick_log = 12;
+    dmalloc_n(&G.node, T);
+    dmalloc_n(&G.off, T + 1);
+    dmalloc_n(&G.sp, M);
+    dmalloc_n(&G.phi, M);
+    dmalloc_n(&G.phi32, M);
+    dmalloc_n(&G.diag, T);
+    dmalloc_n(&G.brick, (L.N >> 12) + 1);
+    dmalloc_n(&G.brick8, (L.N >> 9) + 1);
+    L.cell.ensure(3 * P);
+    L.t64.ensure(3 * P);
+    L.t32.ensure(3 * P);
+    L.pinst.ensure(P);
+    L.sbuf.ensure(P);
+    int64_t To = 0, Mo = 0, Po = 0;
+    for (int k = 0; k < L.K; ++k) {
+        prx_tree* t = B->inst[k];
+        const uint64_t o = L.off_h[k];
+        const int64_t nb = (int64_t)(((uint64_t)1 << (3 * t->D)) >> 12), nb8 = nb * 8;
+        if (!(t->pw > 0.0 && t->trunc_list.T > 0)) {
+            fill_u32(G.brick + (o >> 12), nb, (uint32_t)To, s);
+            fill_u32(G.brick8 + (o >> 9), nb8, (uint32_t)To, s);
+            continue;
+        }
+        const NodeList& S = t->trunc_list;
+        if (S.brick_log != 12 || !S.brick8) throw Error(PRX_ERR_STATE, "batch instance without 16^3 brick ranges");
+        copy_add_u32(G.node + To, S.node, S.T, (uint32_t)o, s);
+        copy_add_u32(G.off + To, S.off, S.T, (uint32_t)Mo, s);
+        copy_add_u32(G.sp + Mo, S.sp, S.M, (uint32_t)Po, s);
+        PRX_HIP(hipMemcpyAsync(G.phi + Mo, S.phi, sizeof(double) * S.M, hipMemcpyDeviceToDevice, s));
+        PRX_HIP(hipMemcpyAsync(G.phi32 + Mo, S.phi32, sizeof(float) * S.M, hipMemcpyDeviceToDevice, s));
+        PRX_HIP(hipMemcpyAsync(G.diag + To, S.diag, sizeof(double) * S.T, hipMemcpyDeviceToDevice, s));
+        copy_add_u32(G.brick + (o >> 12), S.brick, nb, (uint32_t)To, s);
+        copy_add_u32(G.brick8 + (o >> 9), S.brick8, nb8, (uint32_t)To, s);
+        for (int a = 0; a < 3; ++a) {
+            PRX_HIP(hipMemcpyAsync(L.cell.p + a * P + Po, t->cell.p + a * t->P, sizeof(uint32_t) * t->P, hipMemcpyDeviceToDevice, s));
+            PRX_HIP(hipMemcpyAsync(L.t64.p + a * P + Po, t->t64.p + a * t->P, sizeof(double) * t->P, hipMemcpyDeviceToDevice, s));
+            PRX_HIP(hipMemcpyAsync(L.t32.p + a * P + Po, t->t32.p + a * t->P, sizeof(float) * t->P, hipMemcpyDeviceToDevice, s));
+        }
+        fill_u32(L.pinst.p + Po, t->P, (uint32_t)k, s);
+        To += S.T;
+        Mo += S.M;
+        Po += t->P;
+    }
+    fill_u32(G.off + T, 1, (uint32_t)M, s);
+    fill_u32(G.brick + (L.N >> 12), 1, (uint32_t)T, s);
+    fill_u32(G.brick8 + (L.N >> 9), 1, (uint32_t)T, s);
+    PRX_HIP(hipStreamSynchronize(s));
+}
+
+static void build(prx_batch* B) {
+    if (B->built) return;
+    B->drop_graphs();
+    const int K = (int)B->inst.size();
+    std::vector<uint32_t> dF(K), dC(K);
+    std::vector<BCoef> cF(K), cC(K);
+    for (int k = 0; k < K; ++k) {
+        prx_tree* t = B->inst[k];
+        dF[k] = t->D;
+        dC[k] = t->D - 2;
+        BCoef f{};
+        f.D = t->D;
+        f.res = t->res;
+        f.s64 = -t->k64.cc;
+        f.wg64 = t->k64.wg;
+        f.wh3_64 = t->k64.wh3;
+        f.s32 = -t->k32.cc;
+        f.wg32 = t->k32.wg;
+        f.wh3_32 = t->k32.wh3;
+        cF[k] = f;
+        // coarse level: src/fem_tree.rs:346-404 (effective pw = pw 4^(D-d), diagonal wh3*1.005)
+        const uint32_t cd = t->D - 2;
+        const double hc = 1.0 / (double)(1u << cd);
+        const double epw = t->pw * (double)((uint64_t)1 << (2 * (t->D - cd)));
+        const double wh3c = epw > 0.0 ? epw * (hc * hc * hc) / 1.5 : 0.0;
+        BCoef c{};
+        c.D = cd;
+        c.res = 1u << cd;
+        c.s64 = -stiffness_entry(hc, 1, 1, 1);
+        c.wg64 = epw > 0.0 ? wh3c * 1.005 : 0.0;
+        c.wh3_64 = 0.0;
+        cC[k] = c;
+    }
+    setup_level(B, B->fine, dF, cF);
+    setup_level(B, B->coarse, dC, cC);
+    build_screening(B, B->fine);
+    B->st.ensure(K);
+    if (B->st_host) (void)hipHostFree(B->st_host);
+    B->st_host = nullptr;
+    PRX_HIP(hipHostMalloc((void**)&B->st_host, sizeof(CgState) * std::max(K, 1), hipHostMallocDefault));
+    B->flags.ensure(K);
+    B->ones.ensure(K);
+    B->flags_h.assign(K, 0);
+    std::vector<int32_t> one(K, 1);
+    PRX_HIP(hipMemcpyAsync(B->ones.p, one.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, B->stream));
+    B->partial.ensure((std::max(B->fine.N, B->coarse.N) >> 9) + 64);
+    PRX_HIP(hipStreamSynchronize(B->stream));
+    B->built = true;
+}
+
+// ------------------------------------------------------------------ batched building blocks
+static int ulog_tile(int tile) { return tile == 8 ? 9 : 12; }
+
+// r = b - A x for the instances selected by `tab`; partial per brick.
+static void resid_pass(prx_batch* B, BLevel& L, const BatchTab& tab) {
+    StencilIO<double> io;
+    io.x = L.x.p;
+    io.b = L.b.p;
+    io.y = L.r.p;
+    io.partial = B->partial.p;
+    if (L.has_scr) {
+        launch_bscreen_gather<double>(tab, L.pts(), L.x.p, nullptr, nullptr, L.sbuf.p, nullptr, nullptr, B->stream);
+        io.s = L.sbuf.p;
+    }
+    launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_RESID, io, B->tile, B->stream);
+}
+
+static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
+    const BatchTab tab = tab_all(B, L);
+    for (int k = 0; k < sweeps; ++k)
+        for (int c = 0; c < 8; ++c) {
+            if (L.has_scr) launch_bscreen_gather<double>(tab, L.pts(), L.x.p, nullptr, nullptr, L.sbuf.p, nullptr, nullptr, B->stream);
+            launch_bgs_color<double>(tab, L.N, L.scr_ptr(), c, L.x.p, L.b.p, L.sbuf.p, B->gs_block, B->stream);
+        }
+}
+
+// One CG iteration (index k) of every running instance.
+static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
+    const hipStream_t s = B->stream;
+    const BatchTab tab = tab_running(B, L);
+    double* dold = L.dd.p + (k & 1) * L.N;
+    double* dnew = L.dd.p + ((k + 1) & 1) * L.N;
+    StencilIO<double> io;
+    io.r = L.r.p;
+    io.dold = dold;
+    io.dnew = dnew;
+    io.y = L.q.p;
+    io.partial = B->partial.p;
+    if (L.has_scr) {
+        launch_bscreen_gather<double>(tab, L.pts(), nullptr, nullptr, nullptr, L.sbuf.p, L.r.p, dold, s);
+        io.s = L.sbuf.p;
+    }
+    launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_DIR_APPLY, io, B->tile, s);
+    launch_bfin(tab, B->partial.p, ulog_tile(B->tile), FIN_ALPHA, 0, s);
+    int ulog;
+    if (refresh > 0 && (k + 1) % (uint64_t)refresh == 0) {
+        launch_baxpy_alpha<double>(tab, L.x.p, dnew, L.N, s);
+        resid_pass(B, L, tab);
+        ulog = ulog_tile(B->tile);
+    } else {
+        ulog = launch_bcg_update<double>(tab, L.x.p, L.r.p, dnew, L.q.p, L.N, B->partial.p, s);
+    }
+    launch_bfin(tab, B->partial.p, ulog, FIN_BETA, 0, s);
+}
+
+// kind 0: `chunk` CG iterations starting at an even index; kind 1: GS sweeps
+static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int refresh) {
+    for (auto& g : B->graphs)
+        if (g.L == &L && g.kind == kind && g.chunk == n && g.refresh == refresh && g.tile == B->tile && g.gsb == B->gs_block)
+            return g.exec;
+    hipGraph_t graph;
+    PRX_HIP(hipStreamSynchronize(B->stream));
+    PRX_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        if (kind == 0)
+            for (int k = 0; k < n; ++k) cg_enqueue(B, L, (uint64_t)k, refresh);
+        else
+            gs_enqueue(B, L, n);
+    } catch (...) {
+        hipGraph_t g2;
+        (void)hipStreamEndCapture(B->stream, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+    }
+    PRX_HIP(hipStreamEndCapture(B->stream, &graph));
+    hipGraphExec_t exec;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PRX_HIP(e);
+    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, exec});
+    return exec;
+}
+
+static void launch_graph(prx_batch* B, hipGraphExec_t g) {
+    PRX_HIP(hipGraphLaunch(g, B->stream));
+    B->graph_launches++;
+}
+
+struct BOut {
+    std::vector<uint64_t> iters;
+    std::vector<double> rel;
+    std::vector<int> status;
+};
+
+// CG over every instance of level L from its current x (solve_cg,
+// src/solvers.rs:20-69, per instance), iterations graph-replayed in chunks.
+static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, int mode, int refresh) {
+    const hipStream_t s = B->stream;
+    const int K = L.K;
+    const BatchTab all = tab_all(B, L);
+    // ||b||^2 per instance (TRUE mode threshold)
+    std::vector<double> bb(K, 0.0);
+    if (mode == PRX_RESID_TRUE) {
+        const int ul = launch_bdot<double>(all, L.b.p, L.b.p, L.N, B->partial.p, s);
+        launch_bfin(all, B->partial.p, ul, FIN_SLOT, 0, s);
+        read_states(B);
+        for (int k = 0; k < K; ++k) bb[k] = B->st_host[k].slot[0];
+    }
+    for (int k = 0; k < K; ++k) {
+        CgState h{};
+        h.eps = eps;
+        h.max_iters = (int64_t)max_iters;
+        h.bnorm2 = bb[k];
+        B->st_host[k] = h;
+    }
+    write_states(B);
+    resid_pass(B, L, all);
+    launch_bfin(all, B->partial.p, ulog_tile(B->tile), mode == PRX_RESID_TRUE ? FIN_INIT_TRUE : FIN_INIT_REF, 0, s);
+    launch_fill<double>(L.dd.p, 0.0, L.N, s);
+    int chunk = B->chunk;
+    if (refresh > 0) chunk = std::max(refresh, (chunk / refresh) * refresh);
+    if (chunk & 1) chunk *= 2;
+    const hipGraphExec_t g = graph_for(B, L, 0, chunk, refresh);
+    std::vector<char> fin(K, 0);
+    std::vector<double> last_rt(K, INFINITY);
+    read_states(B);
+    for (;;) {
+        // iterate until every instance is done
+        for (;;) {
+            bool all_done = true;
+            for (int k = 0; k < K; ++k)
+                if (!B->st_host[k].done) { all_done = false; break; }
+            if (all_done) break;
+            launch_graph(B, g);
+            read_states(B);
+        }
+        if (mode != PRX_RESID_TRUE) break;
+        // confirm on the true residual; continue the unconfirmed instances
+        bool any = false;
+        for (int k = 0; k < K; ++k) {
+            const CgState& h = B->st_host[k];
+            const bool c = !fin[k] && h.done && h.status == PRX_OK && h.it > 0;
+            if (!fin[k] && !c) fin[k] = 1;
+            B->flags_h[k] = c ? 1 : 0;
+            any = any || c;
+        }
+        if (!any) break;
+        write_flags(B);
+        const BatchTab fl = tab_flagged(B, L);
+        resid_pass(B, L, fl);
+        launch_bfin(fl, B->partial.p, ulog_tile(B->tile), FIN_SLOT, 1, s);
+        read_states(B);
+        for (int k = 0; k < K; ++k) {
+            if (!B->flags_h[k]) continue;
+            CgState& h = B->st_host[k];
+            const double rt = h.slot[1];
+            const bool stalled = rt > 0.25 * last_rt[k];
+            if (rt <= h.thr || (uint64_t)h.it >= max_iters || stalled) {
+                h.delta = rt;
+                if (rt > h.thr) h.status = PRX_ERR_NOT_CONVERGED;
+                fin[k] = 1;
+                continue;
+            }
+            last_rt[k] = rt;
+            const bool drifted = !(rt <= 4.0 * h.delta);
+            h.beta = (!drifted && h.delta_prev > 0.0) ? rt / h.delta_prev : 0.0;
+            h.delta = rt;
+            h.done = 0;
+            h.status = 0;
+            // the graph restarts at an even iteration index and reads the
+            // direction from dd[0]; an instance that stopped after an odd
+            // number of iterations left its last direction in dd[1]
+            if (h.it & 1) {
+                const int64_t nk = (int64_t)(L.off_h[k + 1] - L.off_h[k]);
+                PRX_HIP(hipMemcpyAsync(L.dd.p + L.off_h[k], L.dd.p + L.N + L.off_h[k], sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+            }
+        }
+        write_states(B);
+        bool running = false;
+        for (int k = 0; k < K; ++k) running = running || !B->st_host[k].done;
+        if (!running) break;
+    }
+    BOut o;
+    o.iters.resize(K);
+    o.rel.resize(K);
+    o.status.resize(K);
+    for (int k = 0; k < K; ++k) {
+        const CgState& h = B->st_host[k];
+        o.iters[k] = (uint64_t)h.it;
+        o.status[k] = h.done ? h.status : PRX_ERR_NOT_CONVERGED;
+        const double den = mode == PRX_RESID_TRUE ? bb[k] : h.delta0;
+        o.rel[k] = den > 0.0 ? std::sqrt(h.delta / den) : 0.0;
+    }
+    PRX_HIP(hipStreamSynchronize(s));
+    return o;
+}
+
+// ------------------------------------------------------------------ autotuner
+static double time_ms(hipStream_t s, int reps, const std::function<void()>& f) {
+    hipEvent_t a, b;
+    PRX_HIP(hipEventCreate(&a));
+    PRX_HIP(hipEventCreate(&b));
+    f();  // warm-up
+    PRX_HIP(hipEventRecord(a, s));
+    for (int i = 0; i < reps; ++i) f();
+    PRX_HIP(hipEventRecord(b, s));
+    PRX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    PRX_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+}
+
+// Brick size of the operator (16^3 or 8^3 tiles), block size of the GS colour
+// kernel and the CG chunk length, measured on this batch's fine level.  The
+// chunk trades host synchronisations (one per replay) against skipped
+// iterations after the last instance converges: with t_sync the round trip
+// and t_skip one iteration of all-done launches, chunk ~ sqrt(2 I t_sync /
+// (R t_skip)) for I expected iterations and R confirmation rounds.
+static void autotune(prx_batch* B) {
+    const auto t0 = std::chrono::steady_clock::now();
+    BLevel& L = B->fine;
+    const hipStream_t s = B->stream;
+    const BatchTab all = tab_all(B, L);
+    StencilIO<double> io;
+    io.x = L.x.p;
+    io.y = L.q.p;
+    io.partial = B->partial.p;
+    io.s = L.sbuf.p;
+    double best = INFINITY;
+    for (int tile : {16, 8}) {
+        const double ms = time_ms(s, 3, [&] { launch_bstencil<double>(all, L.N, L.scr_ptr(), OP_APPLY, io, tile, s); });
+        if (ms < best) { best = ms; B->tile = tile; }
+    }
+    // GS colour pass on a scratch copy of x
+    PRX_HIP(hipMemcpyAsync(L.q.p, L.x.p, sizeof(double) * L.N, hipMemcpyDeviceToDevice, s));
+    best = INFINITY;
+    for (int bs : {64, 128, 256, 512}) {
+        const double ms = time_ms(s, 3, [&] { launch_bgs_color<double>(all, L.N, L.scr_ptr(), 0, L.q.p, L.b.p, L.sbuf.p, bs, s); });
+        if (ms < best) { best = ms; B->gs_block = bs; }
+    }
+    // host round trip and the cost of an iteration whose instances are all done
+    const double t_sync = time_ms(s, 5, [&] {
+        PRX_HIP(hipMemcpyAsync(B->st_host, B->st.p, sizeof(CgState) * L.K, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+    });
+    for (int k = 0; k < L.K; ++k) {
+        B->st_host[k] = CgState{};
+        B->st_host[k].done = 1;
+    }
+    write_states(B);
+    B->chunk = 8;
+    const hipGraphExec_t g = graph_for(B, L, 0, 8, 0);
+    const double t_skip = time_ms(s, 3, [&] { PRX_HIP(hipGraphLaunch(g, s)); }) / 8.0;
+    const double I = 500.0, R = 4.0;
+    double c = std::sqrt(2.0 * I * t_sync / (R * std::max(t_skip, 1e-4)));
+    int ci = (int)std::lround(c / 2.0) * 2;
+    B->chunk = std::min(64, std::max(4, ci));
+    PRX_HIP(hipStreamSynchronize(s));
+    B->tuned = true;
+    B->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------------ the batched cascadic solve
+static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double eps, const prx_solve_opts& o, int tune,
+                          prx_batch_report* rep, prx_solve_report* per) {
+    if (B->inst.empty()) throw Error(PRX_ERR_STATE, "empty batch");
+    if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched solve runs in FP64");
+    build(B);
+    const hipStream_t s = B->stream;
+    BLevel& F = B->fine;
+    BLevel& C = B->coarse;
+    const int K = F.K;
+    // right-hand sides: fine b from each tree, coarse RHS from its normal field
+    for (int k = 0; k < K; ++k) {
+        prx_tree* t = B->inst[k];
+        if (!t->assembled) throw Error(PRX_ERR_STATE, "batch instance not assembled");
+        PRX_HIP(hipMemcpyAsync(F.b.p + F.off_h[k], t->b.p, sizeof(double) * t->N, hipMemcpyDeviceToDevice, s));
+    }
+    if (tune && !B->tuned) {
+        launch_fill<double>(F.x.p, 0.0, F.N, s);
+        autotune(B);
+    }
+    if (!tune && !B->tuned) {
+        B->tile = 16;
+        B->gs_block = 256;
+        B->chunk = 16;
+    }
+    const int mode = o.residual_mode == PRX_RESID_TRUE ? PRX_RESID_TRUE : PRX_RESID_REFERENCE;
+    const int refresh = mode == PRX_RESID_TRUE ? 0 : (o.refresh_period > 0 ? o.refresh_period : 0);
+    Timer total(s);
+    double ms_coarse = 0.0, ms_gs = 0.0, ms_cg = 0.0;
+    BOut co, fo;
+    {
+        Timer tm(s);
+        for (int k = 0; k < K; ++k) {
+            prx_tree* t = B->inst[k];
+            const uint32_t cd = t->D - 2;
+            const double hc = 1.0 / (double)(1u << cd);
+            const double epw = t->pw * (double)((uint64_t)1 << (2 * (t->D - cd)));
+            const double wh3c = epw > 0.0 ? epw * (hc * hc * hc) / 1.5 : 0.0;
+            launch_coarse_rhs(t->D, cd, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, C.b.p + C.off_h[k], s);
+        }
+        launch_fill<double>(C.x.p, 0.0, C.N, s);
+        if (gs > 0) launch_graph(B, graph_for(B, C, 1, (int)gs, 0));
+        co = batch_cg(B, C, cg_iters, eps, mode, mode == PRX_RESID_TRUE ? 0 : 10);
+        for (int k = 0; k < K; ++k) {
+            prx_tree* t = B->inst[k];
+            launch_prolongate(t->D, t->D - 2, C.x.p + C.off_h[k], F.x.p + F.off_h[k], s);
+        }
+        ms_coarse = tm.ms();
+    }
+    {
+        Timer tm(s);
+        if (gs > 0) launch_graph(B, graph_for(B, F, 1, (int)gs, 0));
+        ms_gs = tm.ms();
+    }
+    {
+        Timer tm(s);
+        fo = batch_cg(B, F, cg_iters / 4, eps, mode, refresh);
+        ms_cg = tm.ms();
+    }
+    const double ms_total = total.ms();
+    // final true residuals and the solutions back into the trees
+    const BatchTab all = tab_all(B, F);
+    int ul = launch_bdot<double>(all, F.b.p, F.b.p, F.N, B->partial.p, s);
+    launch_bfin(all, B->partial.p, ul, FIN_SLOT, 2, s);
+    resid_pass(B, F, all);
+    launch_bfin(all, B->partial.p, ulog_tile(B->tile), FIN_SLOT, 3, s);
+    read_states(B);
+    for (int k = 0; k < K; ++k) {
+        prx_tree* t = B->inst[k];
+        PRX_HIP(hipMemcpyAsync(t->x.p, F.x.p + F.off_h[k], sizeof(double) * t->N, hipMemcpyDeviceToDevice, s));
+    }
+    PRX_HIP(hipStreamSynchronize(s));
+    prx_batch_report R{};
+    R.instances = (uint32_t)K;
+    R.total_nodes = (uint64_t)F.N;
+    for (prx_tree* t : B->inst) R.total_points += (uint64_t)t->P;
+    R.ms_total = ms_total;
+    R.ms_coarse = ms_coarse;
+    R.ms_fine_gs = ms_gs;
+    R.ms_fine_cg = ms_cg;
+    R.ms_tune = B->tune_ms;
+    R.tile = B->tile;
+    R.chunk = B->chunk;
+    R.gs_block = B->gs_block;
+    R.graph_launches = B->graph_launches;
+    R.status = PRX_OK;
+    for (int k = 0; k < K; ++k) {
+        const double bb = B->st_host[k].slot[2], rr = B->st_host[k].slot[3];
+        const double tr = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        R.max_true_rel = std::max(R.max_true_rel, tr);
+        R.max_fine_iters = std::max<uint64_t>(R.max_fine_iters, fo.iters[k]);
+        R.sum_fine_iters += fo.iters[k];
+        R.max_coarse_iters = std::max<uint64_t>(R.max_coarse_iters, co.iters[k]);
+        if (fo.status[k] == PRX_OK) R.converged++;
+        else if (R.status == PRX_OK) R.status = fo.status[k];
+        if (per) {
+            prx_solve_report& p = per[k];
+            std::memset(&p, 0, sizeof p);
+            p.coarse_iters = co.iters[k];
+            p.coarse_rel = co.rel[k];
+            p.fine_iters = fo.iters[k];
+            p.fine_rel = fo.rel[k];
+            p.outer_iters = 1;
+            p.true_rel = tr;
+            p.status = fo.status[k];
+            p.ms_total = ms_total;
+        }
+    }
+    if (rep) *rep = R;
+}
+
+}  // namespace prx
+
+// ====================================================================== C ABI
+extern "C" {
+
+int prx_batch_create(int device, prx_batch** out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(out, PRX_ERR_INVALID, "null pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw prx::Error(PRX_ERR_DEVICE, "no HIP device available");
+    PRX_REQUIRE(device >= 0 && device < ndev, PRX_ERR_INVALID, "device ordinal out of range");
+    std::unique_ptr<prx_batch> b(new prx_batch());
+    b->device = device;
+    PRX_HIP(hipSetDevice(device));
+    PRX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
+    PRX_API_END
+}
+
+void prx_batch_destroy(prx_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    delete b;
+}
+
+int prx_batch_add(prx_batch* b, uint32_t depth, const prx_point* pts, size_t n, double point_weight, uint32_t* index) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(b && (pts || n == 0), PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(depth >= 6 && depth <= PRX_MAX_DEPTH, PRX_ERR_INVALID, "batched instances need depth 6..10 (base depth D-2 >= 4)");
+    PRX_REQUIRE(point_weight >= 0.0 && std::isfinite(point_weight), PRX_ERR_INVALID, "point weight must be finite and >= 0");
+    PRX_REQUIRE(prx::finite_points(pts, n), PRX_ERR_NONFINITE, "non-finite point coordinate");
+    PRX_HIP(hipSetDevice(b->device));
+    std::unique_ptr<prx_tree> t(prx::create_tree(depth, b->device, b->stream));
+    prx::upload_points(t.get(), pts, n);
+    prx::splat(t.get());
+    prx::assemble(t.get(), point_weight);
+    b->inst.push_back(t.release());
+    b->built = false;
+    b->tuned = false;
+    if (index) *index = (uint32_t)(b->inst.size() - 1);
+    PRX_API_END
+}
+
+int prx_batch_size(prx_batch* b, uint32_t* count) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(b && count, PRX_ERR_INVALID, "null pointer");
+    *count = (uint32_t)b->inst.size();
+    PRX_API_END
+}
+
+int prx_batch_tree(prx_batch* b, uint32_t index, prx_tree** tree) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(b && tree, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(index < b->inst.size(), PRX_ERR_INVALID, "instance index out of range");
+    *tree = b->inst[index];
+    PRX_API_END
+}
+
+int prx_solve_batched(prx_batch* b, uint32_t gs, uint64_t cg_iters, double eps, const prx_solve_opts* opts, int autotune,
+                      prx_batch_report* report, prx_solve_report* per_instance) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(b, PRX_ERR_INVALID, "null batch");
+    PRX_REQUIRE(eps > 0.0 && eps < 1.0, PRX_ERR_INVALID, "eps must be in (0,1)");
+    prx_solve_opts o;
+    prx_solve_opts_default(&o);
+    if (opts) o = *opts;
+    PRX_HIP(hipSetDevice(b->device));
+    prx::solve_batched(b, gs, cg_iters, eps, o, autotune, report, per_instance);
+    PRX_API_END
+}
+
+}  // extern "C"

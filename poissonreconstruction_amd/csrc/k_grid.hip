@@ -117,22 +117,8 @@ void launch_dot(const T* a, const T* b, int64_t n, double* partial, int* nblk, h
 template void launch_dot<double>(const double*, const double*, int64_t, double*, int*, hipStream_t);
 template void launch_dot<float>(const float*, const float*, int64_t, double*, int*, hipStream_t);
 
-// One block folds the partials in a fixed order and advances the CG scalars
-// exactly as solve_cg does (src/solvers.rs:36-64).
-__global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
-    __shared__ double ws[16];
-    if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && st->done) return;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * 1024 < np; i += 4 * 1024) {
-        a0 += partial[i];
-        a1 += partial[i + 1024];
-        a2 += partial[i + 2048];
-        a3 += partial[i + 3072];
-    }
-    for (; i < np; i += 1024) a0 += partial[i];
-    double sum = block_sum_256((a0 + a1) + (a2 + a3), ws);
-    if (threadIdx.x != 0) return;
+// The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
+__device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
     switch (op) {
         case FIN_SLOT: st->slot[slot] = sum; break;
         case FIN_INIT_REF:  // delta0 = r.r ; stop when delta <= delta0*eps^2
@@ -182,10 +168,137 @@ __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial
         }
     }
 }
+
+// 1024 threads fold np partials in a fixed order (4 strided accumulators,
+// then the block sum), so a reduction is reproducible run to run.
+__device__ inline double fin_sum(const double* partial, int64_t np, double* ws) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * 1024 < np; i += 4 * 1024) {
+        a0 += partial[i];
+        a1 += partial[i + 1024];
+        a2 += partial[i + 2048];
+        a3 += partial[i + 3072];
+    }
+    for (; i < np; i += 1024) a0 += partial[i];
+    return block_sum_256((a0 + a1) + (a2 + a3), ws);
+}
+
+// One block folds the partials in a fixed order and advances the CG scalars
+// exactly as solve_cg does (src/solvers.rs:36-64).
+__global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
+    __shared__ double ws[16];
+    if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && st->done) return;
+    const double sum = fin_sum(partial, np, ws);
+    if (threadIdx.x != 0) return;
+    fin_step(st, sum, op, slot);
+}
 void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s) {
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, s, st, partial, np, op, slot);
     PRX_LAUNCH_CHECK();
 }
+
+// ------------------------------------------------------------------ batched CG pieces
+__device__ inline bool b_runs(const BatchTab& tab, uint32_t k) { return tab.run[(int64_t)k * tab.run_stride] == tab.run_when; }
+
+__global__ __launch_bounds__(1024) void k_bfin(BatchTab tab, const double* partial, int ulog, int op, int slot) {
+    __shared__ double ws[16];
+    const uint32_t k = blockIdx.x;
+    if (!b_runs(tab, k)) return;
+    const int64_t p0 = (int64_t)(tab.off[k] >> ulog), p1 = (int64_t)(tab.off[k + 1] >> ulog);
+    const double sum = fin_sum(partial + p0, p1 - p0, ws);
+    if (threadIdx.x != 0) return;
+    fin_step(tab.st + k, sum, op, slot);
+}
+void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfin, dim3(tab.K), dim3(1024), 0, s, tab, partial, ulog, op, slot);
+    PRX_LAUNCH_CHECK();
+}
+
+// x += alpha_k d ; r -= alpha_k q ; partial = r.r per block.  A block covers
+// 256 * VEC nodes inside one 4096-node brick, so it belongs to one instance.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bcg_update(BatchTab tab, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
+                                                    const T* __restrict__ q, double* partial) {
+    __shared__ double ws[4];
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
+    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
+    if (!b_runs(tab, k)) return;
+    const double a = tab.st[k].alpha;
+    double acc = 0.0;
+    const NV xv = ld_nt(x + j), rv = ld_nt(r + j), dv = ld_nt(d + j), qv = ld_nt(q + j);
+    NV xo, ro;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
+        ro[e] = (T)((double)rv[e] - a * (double)qv[e]);
+        acc += (double)ro[e] * (double)ro[e];
+    }
+    st_nt(x + j, xo);
+    st_nt(r + j, ro);
+    const double sm = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm;
+}
+template <typename T>
+int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* d, const T* q, int64_t Ntot, double* partial, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t nb = Ntot / (256 * VEC);
+    hipLaunchKernelGGL(k_bcg_update<T>, dim3((unsigned)nb), dim3(256), 0, s, tab, x, r, d, q, partial);
+    PRX_LAUNCH_CHECK();
+    return VEC == 2 ? 9 : 10;
+}
+template int launch_bcg_update<double>(const BatchTab&, double*, double*, const double*, const double*, int64_t, double*, hipStream_t);
+template int launch_bcg_update<float>(const BatchTab&, float*, float*, const float*, const float*, int64_t, double*, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_baxpy_alpha(BatchTab tab, T* __restrict__ x, const T* __restrict__ d) {
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
+    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
+    if (!b_runs(tab, k)) return;
+    const double a = tab.st[k].alpha;
+    const NV xv = ld_v(x + j), dv = ld_v(d + j);
+    NV xo;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
+    *(NV*)(x + j) = xo;
+}
+template <typename T>
+void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* d, int64_t Ntot, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    hipLaunchKernelGGL(k_baxpy_alpha<T>, dim3((unsigned)(Ntot / (256 * VEC))), dim3(256), 0, s, tab, x, d);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_baxpy_alpha<double>(const BatchTab&, double*, const double*, int64_t, hipStream_t);
+template void launch_baxpy_alpha<float>(const BatchTab&, float*, const float*, int64_t, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bdot(BatchTab tab, const T* __restrict__ a, const T* __restrict__ b, double* partial) {
+    __shared__ double ws[4];
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
+    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
+    if (!b_runs(tab, k)) return;
+    const NV av = ld_v(a + j), bv = ld_v(b + j);
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc += (double)av[e] * (double)bv[e];
+    const double sm = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm;
+}
+template <typename T>
+int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    hipLaunchKernelGGL(k_bdot<T>, dim3((unsigned)(Ntot / (256 * VEC))), dim3(256), 0, s, tab, a, b, partial);
+    PRX_LAUNCH_CHECK();
+    return VEC == 2 ? 9 : 10;
+}
+template int launch_bdot<double>(const BatchTab&, const double*, const double*, int64_t, double*, hipStream_t);
+template int launch_bdot<float>(const BatchTab&, const float*, const float*, int64_t, double*, hipStream_t);
 
 // x += alpha d ; r -= alpha q ; partial = r.r   (src/solvers.rs:47, :53, :57)
 // One 16-byte vector per array per thread over a flat grid, streamed with

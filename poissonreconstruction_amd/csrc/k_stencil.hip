@@ -124,9 +124,14 @@ __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
     }
 }
 
-template <typename T, int B, int OP>
-__global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO<T> io, uint32_t nbricks, int swz) {
-    if (io.done && *io.done) return;
+// BATCH: the grid is a concatenation of instances (see BatchTab); the
+// brick's instance supplies res, the weights, beta and the run flag, and
+// halo addresses are offset by the instance's first node.
+template <typename T, int B, int OP, bool BATCH = false>
+__global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO<T> io, uint32_t nbricks, int swz, BatchTab tab) {
+    if constexpr (!BATCH) {
+        if (io.done && *io.done) return;
+    }
     constexpr int NT = B * B, B3 = B * B * B, E = B + 2, E3 = E * E * E, S = E3 - B3;
     using NV = typename NV16<T>::type;
     typedef float nf2 __attribute__((ext_vector_type(2)));
@@ -141,11 +146,25 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         bid = (bid & 7u) * per + (bid >> 3);
     }
     const uint64_t base = (uint64_t)bid * B3;
-    uint32_t ox, oy, oz;
-    demorton3(bid, ox, oy, oz);
-    ox *= B; oy *= B; oz *= B;
+    uint32_t lb = bid, res = L.res;
+    uint64_t ib = 0;  // first node of the brick's instance
     double bt = 0.0;
-    if constexpr (OP == OP_DIR_APPLY) bt = *io.beta;
+    if constexpr (BATCH) {
+        const uint32_t inst = tab.brick_inst[base >> 12];
+        if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
+        const BCoef c = tab.coef[inst];
+        res = c.res;
+        if constexpr (sizeof(T) == 8) { L.s = (T)c.s64; L.wg = (T)c.wg64; }
+        else { L.s = (T)c.s32; L.wg = (T)c.wg32; }
+        ib = tab.off[inst];
+        lb = bid - (uint32_t)(ib / B3);
+        if constexpr (OP == OP_DIR_APPLY) bt = tab.st[inst].beta;
+    } else {
+        if constexpr (OP == OP_DIR_APPLY) bt = *io.beta;
+    }
+    uint32_t ox, oy, oz;
+    demorton3(lb, ox, oy, oz);
+    ox *= B; oy *= B; oz *= B;
 
     // ---- 1. issue the brick's interior loads (16-byte, coalesced)
     NV qa[NVEC], qb[NVEC];
@@ -178,8 +197,8 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             shell_pos<B>(i, hx, hy, hz);
             hl[k] = ((hz + 1) * E + hy + 1) * E + hx + 1;
             const int gx = (int)ox + hx, gy = (int)oy + hy, gz = (int)oz + hz;
-            if ((unsigned)gx < L.res && (unsigned)gy < L.res && (unsigned)gz < L.res) {
-                const uint32_t g = morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+            if ((unsigned)gx < res && (unsigned)gy < res && (unsigned)gz < res) {
+                const uint64_t g = ib + morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
                 hin[k] = true;
                 if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
                 else if constexpr (OP == OP_DIR_APPLY) hv[k] = (T)fma(bt, (double)io.dold[g], (double)io.r[g]);
@@ -329,12 +348,15 @@ __device__ inline int64_t touched_index(const LevelArgs<T>& L, uint32_t i) {
 }
 
 // (A x)_i with x read from global memory; also returns the diagonal a_ii.
+// ib: first node of the instance (batched grids; 0 otherwise); i is the
+// global node id, (ix,iy,iz) the coordinates inside the instance.
 template <typename T, int MODE>  // MODE 0: x ; 1: x64 + e32 ; 2: r + beta*d (x = r, x64 unused, d = dd)
 __device__ inline T apply_node(const LevelArgs<T>& L, const T* x, const double* x64, const float* e32, const T* s, uint32_t ix,
-                               uint32_t iy, uint32_t iz, uint32_t i, T* diag, const T* dd = nullptr, double bt = 0.0) {
+                               uint32_t iy, uint32_t iz, uint32_t i, T* diag, const T* dd = nullptr, double bt = 0.0,
+                               uint64_t ib = 0) {
     auto X = [&](int jx, int jy, int jz) -> T {
         if ((unsigned)jx >= L.res || (unsigned)jy >= L.res || (unsigned)jz >= L.res) return T(0);
-        const uint32_t j = morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
+        const uint64_t j = ib + morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
         if constexpr (MODE == 1) return (T)(x64[j] + (double)e32[j]);
         else if constexpr (MODE == 2) return (T)fma(bt, (double)dd[j], (double)x[j]);
         else return x[j];
@@ -431,7 +453,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
     const uint32_t nbricks = (uint32_t)(N >> tl);
     const int swz = (nbricks >= 64 && (nbricks & 7u) == 0) ? 1 : 0;
 #define PRX_TILE_LAUNCH(BB, OPV) \
-    hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz)
+    hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz, BatchTab{})
     if (B == 16) {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(16, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_TILE_LAUNCH(16, OP_DIR_APPLY);
@@ -478,5 +500,92 @@ void launch_gs_color(const LevelView& Lv, int color, T* x, const T* b, const T* 
 }
 template void launch_gs_color<double>(const LevelView&, int, double*, const double*, const double*, hipStream_t);
 template void launch_gs_color<float>(const LevelView&, int, float*, const float*, const float*, hipStream_t);
+
+// ------------------------------------------------------------------ batched grids
+template <typename T>
+static LevelArgs<T> batch_args(const NodeList* scr, int tile_log) {
+    LevelArgs<T> a{};
+    a.has_screen = (scr && scr->T > 0) ? 1 : 0;
+    if (a.has_screen) {
+        a.brick_log = scr->brick_log;
+        a.brick = scr->brick;
+        if (tile_log == 9 && scr->brick8) { a.brick = scr->brick8; a.brick_log = 9; }
+        a.node = scr->node;
+        a.off = scr->off;
+        a.sp = scr->sp;
+        if constexpr (sizeof(T) == 8) a.phi = (const T*)scr->phi;
+        else a.phi = (const T*)scr->phi32;
+        a.diag = scr->diag;
+    }
+    return a;
+}
+
+template <typename T>
+void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int op, const StencilIO<T>& io, int tile,
+                     hipStream_t s) {
+    const int B = tile == 8 ? 8 : 16, tl = B == 16 ? 12 : 9;
+    const LevelArgs<T> a = batch_args<T>(scr, tl);
+    const uint32_t nbricks = (uint32_t)(Ntot >> tl);
+    const int swz = (nbricks >= 64 && (nbricks & 7u) == 0) ? 1 : 0;
+#define PRX_BTILE_LAUNCH(BB, OPV) \
+    hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV, true>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz, tab)
+    if (B == 16) {
+        if (op == OP_APPLY) PRX_BTILE_LAUNCH(16, OP_APPLY);
+        else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(16, OP_DIR_APPLY);
+        else if (op == OP_RESID) PRX_BTILE_LAUNCH(16, OP_RESID);
+        else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
+    } else {
+        if (op == OP_APPLY) PRX_BTILE_LAUNCH(8, OP_APPLY);
+        else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(8, OP_DIR_APPLY);
+        else if (op == OP_RESID) PRX_BTILE_LAUNCH(8, OP_RESID);
+        else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
+    }
+#undef PRX_BTILE_LAUNCH
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bstencil<double>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<double>&, int, hipStream_t);
+template void launch_bstencil<float>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<float>&, int, hipStream_t);
+
+// Multicolour GS over a batch: thread j handles node j & 511 of colour
+// `color` inside 16^3 brick j >> 9 (each brick holds 512 nodes per colour).
+template <typename T>
+__global__ void k_bgs_color(LevelArgs<T> L, BatchTab tab, int color, T* x, const T* b, const T* s, uint64_t nthreads) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nthreads) return;
+    const uint64_t gb = j >> 9;
+    const uint32_t inst = tab.brick_inst[gb];
+    if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
+    const BCoef c = tab.coef[inst];
+    L.res = c.res;
+    T sc;
+    if constexpr (sizeof(T) == 8) { sc = (T)c.s64; L.wg = (T)c.wg64; L.wh3 = (T)c.wh3_64; }
+    else { sc = (T)c.s32; L.wg = (T)c.wg32; L.wh3 = (T)c.wh3_32; }
+    L.s = sc;
+    L.cc = -sc;
+    L.ce = T(-2) * sc;
+    L.c0 = T(32) * sc;
+    const uint64_t ib = tab.off[inst];
+    uint32_t bx, by, bz, X, Y, Z;
+    demorton3((uint32_t)(gb - (ib >> 12)), bx, by, bz);
+    demorton3((uint32_t)(j & 511u), X, Y, Z);
+    const uint32_t ix = 16 * bx + 2 * X + (color & 1), iy = 16 * by + 2 * Y + ((color >> 1) & 1), iz = 16 * bz + 2 * Z + (color >> 2);
+    const uint64_t i = ib + morton3(ix, iy, iz);
+    T dg;
+    const T a = apply_node<T, 0>(L, x, nullptr, nullptr, s, ix, iy, iz, (uint32_t)i, &dg, nullptr, 0.0, ib);
+    if (dg != T(0)) x[i] = x[i] + (b[i] - a) / dg;
+}
+template <typename T>
+void launch_bgs_color(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int color, T* x, const T* b, const T* sbuf,
+                      int block, hipStream_t s) {
+    const LevelArgs<T> a = batch_args<T>(scr, 12);
+    const uint64_t nt = (uint64_t)Ntot / 8;
+    const int bs = (block == 64 || block == 128 || block == 512) ? block : 256;
+    hipLaunchKernelGGL((k_bgs_color<T>), dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0, s, a, tab, color, x, b, sbuf, nt);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bgs_color<double>(const BatchTab&, int64_t, const NodeList*, int, double*, const double*, const double*, int,
+                                       hipStream_t);
+template void launch_bgs_color<float>(const BatchTab&, int64_t, const NodeList*, int, float*, const float*, const float*, int,
+                                      hipStream_t);
 
 }  // namespace prx

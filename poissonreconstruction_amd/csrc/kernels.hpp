@@ -119,6 +119,60 @@ void launch_gs_color(const LevelView& L, int color, T* x, const T* b, const T* s
 
 int partial_capacity(int64_t N);
 
+// ------------------------------------------------------------------ batches of grids
+// K grids (one per point cloud, depth D_k >= 4) concatenated in one array:
+// instance k owns the nodes [off[k], off[k+1]) in its own Morton order, and
+// off[k] is a multiple of 4096, so every aligned 16^3 (or 8^3) brick lies in
+// one instance (brick_inst, per 16^3 brick).  Node ids in batched node lists
+// are global.  Each batched kernel processes instance k only while
+// run[k * run_stride] == run_when (e.g. the instance's CG `done` flag is 0).
+struct BCoef {
+    uint32_t D, res;
+    double s64, wg64, wh3_64;  // Laplacian weight scale s (ce = -2s, cc = -s, c0 = 32s), wg, wh3
+    float s32, wg32, wh3_32;   // the same scaled by 1/h (FP32 correction system)
+    float pad;
+};
+struct BatchTab {
+    int K = 0;
+    const uint64_t* off = nullptr;         // [K+1]
+    const uint32_t* brick_inst = nullptr;  // [Ntot / 4096]
+    const BCoef* coef = nullptr;           // [K]
+    CgState* st = nullptr;                 // [K]
+    const int32_t* run = nullptr;
+    int run_stride = 0, run_when = 0;
+};
+// Batched point data for the screening gather (sorted order inside each
+// instance, instances concatenated).
+struct BPoints {
+    int64_t P = 0;
+    const uint32_t* cell = nullptr;  // [3P] SoA
+    const double* t64 = nullptr;     // [3P]
+    const float* t32 = nullptr;      // [3P]
+    const uint32_t* inst = nullptr;  // [P]
+};
+// One batched operator pass (tile 16 or 8; beta and the run flag per
+// instance from tab; partial per brick).
+template <typename T>
+void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int op, const StencilIO<T>& io, int tile,
+                     hipStream_t s);
+template <typename T>
+void launch_bgs_color(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int color, T* x, const T* b, const T* sbuf,
+                      int block, hipStream_t s);
+template <typename T>
+void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
+                           const T* r, const T* d, hipStream_t s);
+// Per instance: fold its partials [off[k] >> ulog, off[k+1] >> ulog) in the
+// order k_fin uses and apply the same scalar step to st[k].
+void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, int slot, hipStream_t s);
+// x += alpha_k d ; r -= alpha_k q ; partial per (256 * 16 / sizeof(T)) nodes
+template <typename T>
+int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* d, const T* q, int64_t Ntot, double* partial, hipStream_t s);
+template <typename T>
+void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* d, int64_t Ntot, hipStream_t s);
+// partial = a.b per (256 * 16 / sizeof(T)) nodes; returns log2 of that unit
+template <typename T>
+int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s);
+
 // ---- k_adjoint.hip
 // gx[node] = sum over the node's evaluate entries of gF[p] * phi (input order)
 void launch_adj_load(const NodeList& E, const double* gF, double* gx, hipStream_t s);

@@ -10,164 +10,13 @@
 #include <memory>
 #include <type_traits>
 
-#include "kernels.hpp"
+#include "tree.hpp"
 
 namespace prx {
 
-// ------------------------------------------------------------------ device arrays
-template <typename T>
-struct DArr {
-    T* p = nullptr;
-    int64_t n = 0;
-    DArr() = default;
-    DArr(const DArr&) = delete;
-    DArr& operator=(const DArr&) = delete;
-    ~DArr() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void swap(DArr& o) {
-        std::swap(p, o.p);
-        std::swap(n, o.n);
-    }
-    void ensure(int64_t cnt) {
-        if (cnt <= n && p) return;
-        release();
-        PRX_HIP(hipMalloc((void**)&p, (size_t)(cnt > 0 ? cnt : 1) * sizeof(T)));
-        n = cnt;
-    }
-};
+thread_local std::string g_err;
 
-static void free_list(NodeList& L) {
-    void* ptrs[] = {L.node, L.off, L.code, L.sp, L.phi, L.phi32, L.diag, L.brick, L.brick8};
-    for (void* q : ptrs)
-        if (q) (void)hipFree(q);
-    int mode = L.mode;
-    L = NodeList{};
-    L.mode = mode;
-}
-
-// Reference stencil constants (src/fem_tree.rs:39-48), same expression order.
-static double mass_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. / 3. : (d == 1 ? 1. / 6. : 0.0); }
-static double stiff_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. : (d == 1 ? -1. : 0.0); }
-static double stiffness_entry(double h, int dx, int dy, int dz) {
-    return (stiff_1d(dx) * mass_1d(dy) * mass_1d(dz) + mass_1d(dx) * stiff_1d(dy) * mass_1d(dz) +
-            mass_1d(dx) * mass_1d(dy) * stiff_1d(dz)) * h;
-}
-
-struct Timer {
-    hipEvent_t a{}, b{};
-    hipStream_t s;
-    explicit Timer(hipStream_t st) : s(st) {
-        PRX_HIP(hipEventCreate(&a));
-        PRX_HIP(hipEventCreate(&b));
-        PRX_HIP(hipEventRecord(a, s));
-    }
-    ~Timer() {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    }
-    double ms() {
-        PRX_HIP(hipEventRecord(b, s));
-        PRX_HIP(hipEventSynchronize(b));
-        float t = 0.f;
-        PRX_HIP(hipEventElapsedTime(&t, a, b));
-        return (double)t;
-    }
-};
-
-struct CgOut {
-    uint64_t iters = 0;
-    double rel = 0.0;
-    int status = PRX_OK;
-    uint64_t applies = 0;
-};
-
-}  // namespace prx
-
-using namespace prx;
-
-// ------------------------------------------------------------------ the handle
-struct prx_tree {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t D = 0, res = 1;
-    int64_t N = 1;
-    double h = 1.0;
-    int tile = 16;
-
-    // points
-    int64_t P = 0;
-    DArr<double> pos, nrm, t64;
-    DArr<uint32_t> perm, cell;
-    DArr<float> t32;
-    DevPoints pts;
-    NodeList floor_list, trunc_list, eval_list;
-    bool splatted = false, assembled = false;
-
-    // fields
-    DArr<double> nf, w, b, x, x2, adj_g, adj_lam, adj_gnf, adj_out;
-    double pw = 0.0;
-    Coef<double> k64{};
-    Coef<float> k32{};
-    double K0 = 0, Ke = 0, Kc = 0, wh3 = 0, wg = 0;
-
-    // solver workspace
-    DArr<double> r64, d64, q64, s64, partial, tmp64;
-    DArr<float> e32, r32, ri32, d32, q32, s32;
-    CgState* st = nullptr;
-    CgState* st_host = nullptr;
-
-    // coarse level (Morton order inside)
-    uint32_t cd = 0;
-    DArr<double> cb, cx, cr, cdd, cq, cpart;
-
-    prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
-    ~prx_tree() {
-        free_list(floor_list);
-        free_list(trunc_list);
-        free_list(eval_list);
-        if (st) (void)hipFree(st);
-        if (st_host) (void)hipHostFree(st_host);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    LevelView fine_view() const {
-        LevelView v;
-        v.D = D;
-        v.res = res;
-        v.N = N;
-        v.k64 = k64;
-        v.k32 = k32;
-        v.scr = (pw > 0.0 && trunc_list.T > 0) ? &trunc_list : nullptr;
-        v.pts = &pts;
-        return v;
-    }
-    void ensure_state() {
-        if (!st) PRX_HIP(hipMalloc((void**)&st, sizeof(CgState)));
-        if (!st_host) PRX_HIP(hipHostMalloc((void**)&st_host, sizeof(CgState), hipHostMallocDefault));
-        partial.ensure(partial_capacity(N));
-    }
-    void set_coefficients(double pweight) {
-        pw = pweight;
-        K0 = stiffness_entry(h, 0, 0, 0);
-        Ke = stiffness_entry(h, 1, 1, 0);
-        Kc = stiffness_entry(h, 1, 1, 1);
-        wh3 = pw > 0.0 ? pw * (h * h * h) / 1.5 : 0.0;
-        wg = pw > 0.0 ? wh3 * 0.005 : 0.0;
-        k64 = {K0, Ke, Kc, wg, wh3};
-        const double ih = 1.0 / h;
-        k32 = {(float)(K0 * ih), (float)(Ke * ih), (float)(Kc * ih), (float)(wg * ih), (float)(wh3 * ih)};
-    }
-};
-
-namespace prx {
-
-static thread_local std::string g_err;
-
-static void set_device(prx_tree* t) { PRX_HIP(hipSetDevice(t->device)); }
+void set_device(prx_tree* t) { PRX_HIP(hipSetDevice(t->device)); }
 
 // ------------------------------------------------------------------ operator helpers
 template <typename T>
@@ -230,7 +79,7 @@ static void gs_sweeps(prx_tree* t, const LevelView& L, T* x, const T* b, int swe
         }
 }
 
-static void read_state(prx_tree* t) {
+void read_state(prx_tree* t) {
     PRX_HIP(hipMemcpyAsync(t->st_host, t->st, sizeof(CgState), hipMemcpyDeviceToHost, t->stream));
     PRX_HIP(hipStreamSynchronize(t->stream));
 }
@@ -587,7 +436,7 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
 }
 
 // ------------------------------------------------------------------ splat and assembly
-static void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
+void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
     const int64_t P = (int64_t)n;
     if (P >= ((int64_t)1 << 29)) throw Error(PRX_ERR_INVALID, "at most 2^29 points per tree");
     std::vector<double> soa(6 * (size_t)std::max<int64_t>(P, 1));
@@ -605,7 +454,7 @@ static void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
     PRX_HIP(hipStreamSynchronize(t->stream));
 }
 
-static void splat(prx_tree* t) {
+void splat(prx_tree* t) {
     const hipStream_t s = t->stream;
     const int64_t P = t->P, M8 = 8 * P;
     free_list(t->floor_list);
@@ -655,7 +504,7 @@ static void splat(prx_tree* t) {
     t->assembled = false;
 }
 
-static void assemble(prx_tree* t, double pw) {
+void assemble(prx_tree* t, double pw) {
     if (!t->splatted) throw Error(PRX_ERR_STATE, "assemble before splat_normal_field");
     const hipStream_t s = t->stream;
     t->set_coefficients(pw);
@@ -669,7 +518,7 @@ static void assemble(prx_tree* t, double pw) {
     t->assembled = true;
 }
 
-static prx_tree* create_tree(uint32_t depth, int device) {
+prx_tree* create_tree(uint32_t depth, int device, hipStream_t shared) {
     if (depth > PRX_MAX_DEPTH) throw Error(PRX_ERR_INVALID, "depth must be <= 10");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(PRX_ERR_DEVICE, "no HIP device available");
@@ -677,7 +526,12 @@ static prx_tree* create_tree(uint32_t depth, int device) {
     std::unique_ptr<prx_tree> t(new prx_tree());
     t->device = device;
     PRX_HIP(hipSetDevice(device));
-    PRX_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    if (shared) {
+        t->stream = shared;
+        t->own_stream = false;
+    } else {
+        PRX_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    }
     t->D = depth;
     t->res = 1u << depth;
     t->N = (int64_t)1 << (3 * depth);
@@ -686,7 +540,7 @@ static prx_tree* create_tree(uint32_t depth, int device) {
     return t.release();
 }
 
-static bool finite_points(const prx_point* p, size_t n) {
+bool finite_points(const prx_point* p, size_t n) {
     for (size_t i = 0; i < n; ++i)
         if (!std::isfinite(p[i].px) || !std::isfinite(p[i].py) || !std::isfinite(p[i].pz) || !std::isfinite(p[i].nx) ||
             !std::isfinite(p[i].ny) || !std::isfinite(p[i].nz))
@@ -838,26 +692,6 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
 }  // namespace prx
 
 // ====================================================================== C ABI
-#define PRX_API_BEGIN try {
-#define PRX_API_END                                   \
-    }                                                 \
-    catch (const prx::Error& e) {                     \
-        prx::g_err = e.what();                        \
-        return e.code;                                \
-    }                                                 \
-    catch (const std::bad_alloc&) {                   \
-        prx::g_err = "host allocation failed";        \
-        return PRX_ERR_OOM;                           \
-    }                                                 \
-    catch (const std::exception& e) {                 \
-        prx::g_err = e.what();                        \
-        return PRX_ERR_DEVICE;                        \
-    }                                                 \
-    return PRX_OK;
-
-#define PRX_REQUIRE(cond, code, msg) \
-    if (!(cond)) throw prx::Error(code, msg)
-
 extern "C" {
 
 const char* prx_last_error(void) { return prx::g_err.c_str(); }

@@ -1,0 +1,199 @@
+// Host-side state shared by the translation units of libprx: device arrays,
+// the FEMTree handle (prx_tree) and the C-ABI error plumbing.
+#pragma once
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace prx {
+
+// ------------------------------------------------------------------ device arrays
+template <typename T>
+struct DArr {
+    T* p = nullptr;
+    int64_t n = 0;
+    DArr() = default;
+    DArr(const DArr&) = delete;
+    DArr& operator=(const DArr&) = delete;
+    ~DArr() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void swap(DArr& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+    }
+    void ensure(int64_t cnt) {
+        if (cnt <= n && p) return;
+        release();
+        PRX_HIP(hipMalloc((void**)&p, (size_t)(cnt > 0 ? cnt : 1) * sizeof(T)));
+        n = cnt;
+    }
+};
+
+inline void free_list(NodeList& L) {
+    void* ptrs[] = {L.node, L.off, L.code, L.sp, L.phi, L.phi32, L.diag, L.brick, L.brick8};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    int mode = L.mode;
+    L = NodeList{};
+    L.mode = mode;
+}
+
+// Reference stencil constants (src/fem_tree.rs:39-48), same expression order.
+inline double mass_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. / 3. : (d == 1 ? 1. / 6. : 0.0); }
+inline double stiff_1d(int d) { d = d < 0 ? -d : d; return d == 0 ? 2. : (d == 1 ? -1. : 0.0); }
+inline double stiffness_entry(double h, int dx, int dy, int dz) {
+    return (stiff_1d(dx) * mass_1d(dy) * mass_1d(dz) + mass_1d(dx) * stiff_1d(dy) * mass_1d(dz) +
+            mass_1d(dx) * mass_1d(dy) * stiff_1d(dz)) * h;
+}
+
+struct Timer {
+    hipEvent_t a{}, b{};
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        PRX_HIP(hipEventCreate(&a));
+        PRX_HIP(hipEventCreate(&b));
+        PRX_HIP(hipEventRecord(a, s));
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    double ms() {
+        PRX_HIP(hipEventRecord(b, s));
+        PRX_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        PRX_HIP(hipEventElapsedTime(&t, a, b));
+        return (double)t;
+    }
+};
+
+struct CgOut {
+    uint64_t iters = 0;
+    double rel = 0.0;
+    int status = PRX_OK;
+    uint64_t applies = 0;
+};
+
+}  // namespace prx
+
+using namespace prx;
+
+// ------------------------------------------------------------------ the handle
+struct prx_tree {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    uint32_t D = 0, res = 1;
+    int64_t N = 1;
+    double h = 1.0;
+    int tile = 16;
+
+    // points
+    int64_t P = 0;
+    DArr<double> pos, nrm, t64;
+    DArr<uint32_t> perm, cell;
+    DArr<float> t32;
+    DevPoints pts;
+    NodeList floor_list, trunc_list, eval_list;
+    bool splatted = false, assembled = false;
+
+    // fields
+    DArr<double> nf, w, b, x, x2, adj_g, adj_lam, adj_gnf, adj_out;
+    double pw = 0.0;
+    Coef<double> k64{};
+    Coef<float> k32{};
+    double K0 = 0, Ke = 0, Kc = 0, wh3 = 0, wg = 0;
+
+    // solver workspace
+    DArr<double> r64, d64, q64, s64, partial, tmp64;
+    DArr<float> e32, r32, ri32, d32, q32, s32;
+    CgState* st = nullptr;
+    CgState* st_host = nullptr;
+
+    // coarse level (Morton order inside)
+    uint32_t cd = 0;
+    DArr<double> cb, cx, cr, cdd, cq, cpart;
+
+    prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
+    ~prx_tree() {
+        free_list(floor_list);
+        free_list(trunc_list);
+        free_list(eval_list);
+        if (st) (void)hipFree(st);
+        if (st_host) (void)hipHostFree(st_host);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
+    }
+
+    LevelView fine_view() const {
+        LevelView v;
+        v.D = D;
+        v.res = res;
+        v.N = N;
+        v.k64 = k64;
+        v.k32 = k32;
+        v.scr = (pw > 0.0 && trunc_list.T > 0) ? &trunc_list : nullptr;
+        v.pts = &pts;
+        return v;
+    }
+    void ensure_state() {
+        if (!st) PRX_HIP(hipMalloc((void**)&st, sizeof(CgState)));
+        if (!st_host) PRX_HIP(hipHostMalloc((void**)&st_host, sizeof(CgState), hipHostMallocDefault));
+        partial.ensure(partial_capacity(N));
+    }
+    void set_coefficients(double pweight) {
+        pw = pweight;
+        K0 = stiffness_entry(h, 0, 0, 0);
+        Ke = stiffness_entry(h, 1, 1, 0);
+        Kc = stiffness_entry(h, 1, 1, 1);
+        wh3 = pw > 0.0 ? pw * (h * h * h) / 1.5 : 0.0;
+        wg = pw > 0.0 ? wh3 * 0.005 : 0.0;
+        k64 = {K0, Ke, Kc, wg, wh3};
+        const double ih = 1.0 / h;
+        k32 = {(float)(K0 * ih), (float)(Ke * ih), (float)(Kc * ih), (float)(wg * ih), (float)(wh3 * ih)};
+    }
+};
+
+namespace prx {
+
+extern thread_local std::string g_err;
+
+void set_device(prx_tree* t);
+void read_state(prx_tree* t);
+// tree set-up as prx_tree_create / prx_splat_normal_field / prx_assemble do it;
+// `shared` (optional) is a stream owned by the caller
+prx_tree* create_tree(uint32_t depth, int device, hipStream_t shared = nullptr);
+void upload_points(prx_tree* t, const prx_point* pts, size_t n);
+void splat(prx_tree* t);
+void assemble(prx_tree* t, double pw);
+bool finite_points(const prx_point* p, size_t n);
+
+}  // namespace prx
+
+// ------------------------------------------------------------------ C-ABI error plumbing
+#define PRX_API_BEGIN try {
+#define PRX_API_END                                   \
+    }                                                 \
+    catch (const prx::Error& e) {                     \
+        prx::g_err = e.what();                        \
+        return e.code;                                \
+    }                                                 \
+    catch (const std::bad_alloc&) {                   \
+        prx::g_err = "host allocation failed";        \
+        return PRX_ERR_OOM;                           \
+    }                                                 \
+    catch (const std::exception& e) {                 \
+        prx::g_err = e.what();                        \
+        return PRX_ERR_DEVICE;                        \
+    }                                                 \
+    return PRX_OK;
+
+#define PRX_REQUIRE(cond, code, msg) \
+    if (!(cond)) throw prx::Error(code, msg)

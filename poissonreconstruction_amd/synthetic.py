@@ -7,6 +7,12 @@ torus:  R=1.0, r=0.35 about z, uniform by area, positions jittered N(0, jitter^2
 ellipsoids: union of 3 ellipsoids (semi-axes U[0.3,0.8], uniform random
         rotations, centres U[-0.3,0.3]^3), points uniform by area on the union's
         surface (points inside another ellipsoid dropped), outward normals.
+superquadric: x = (a1 C(eta,e1) C(w,e2), a2 C(eta,e1) S(w,e2), a3 S(eta,e1)) with
+        C(t,e) = sign(cos t)|cos t|^e, S likewise; exponents e1,e2 ~ U[0.3,3],
+        scales a ~ U[0.5,1]^3; points uniform by area (a fine parameter grid's
+        triangles drawn by area, uniform inside, mapped through the surface),
+        position noise N(0, noise^2), exact outward normals (the gradient of the
+        inside-outside function at the noise-free point).
 Each returns an (n, 6) float64 array [px, py, pz, nx, ny, nz].
 """
 from __future__ import annotations
@@ -111,3 +117,53 @@ def ellipsoids(n: int, k: int = 3, seed: int = 0) -> np.ndarray:
         xs.append(np.concatenate(got_x))
         ns.append(np.concatenate(got_n))
     return np.ascontiguousarray(np.concatenate([np.concatenate(xs), np.concatenate(ns)], axis=1))
+
+
+def _spow(v, e):
+    return np.sign(v) * np.abs(v) ** e
+
+
+def superquadric(n: int, seed: int = 0, noise: float = 5e-3, grid: int = 256) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    e1, e2 = rng.uniform(0.3, 3.0, size=2)
+    a = rng.uniform(0.5, 1.0, size=3)
+
+    def surf(eta, w):
+        ce, se, cw, sw = np.cos(eta), np.sin(eta), np.cos(w), np.sin(w)
+        return np.stack([a[0] * _spow(ce, e1) * _spow(cw, e2), a[1] * _spow(ce, e1) * _spow(sw, e2), a[2] * _spow(se, e1)], axis=-1)
+
+    # triangulated parameter grid; triangles drawn by their area
+    ge = np.linspace(-np.pi / 2, np.pi / 2, grid + 1)
+    gw = np.linspace(-np.pi, np.pi, 2 * grid + 1)
+    E, W = np.meshgrid(ge, gw, indexing="ij")
+    X = surf(E, W)
+    p00, p10, p01, p11 = X[:-1, :-1], X[1:, :-1], X[:-1, 1:], X[1:, 1:]
+    a1 = 0.5 * np.linalg.norm(np.cross(p10 - p00, p01 - p00), axis=-1)
+    a2 = 0.5 * np.linalg.norm(np.cross(p10 - p11, p01 - p11), axis=-1)
+    areas = np.stack([a1, a2], axis=-1).reshape(-1)
+    tri = rng.choice(areas.size, size=n, p=areas / areas.sum())
+    cell, half = tri // 2, tri % 2
+    i, j = cell // (2 * grid), cell % (2 * grid)
+    u, v = rng.uniform(size=n), rng.uniform(size=n)
+    flip = u + v > 1.0
+    u, v = np.where(flip, 1.0 - u, u), np.where(flip, 1.0 - v, v)
+    de, dw = ge[1] - ge[0], gw[1] - gw[0]
+    # lower triangle (00,10,01); upper triangle (11,01,10) -> parameters
+    eta = np.where(half == 0, ge[i] + u * de, ge[i + 1] - u * de)
+    w = np.where(half == 0, gw[j] + v * dw, gw[j + 1] - v * dw)
+    p = surf(eta, w)
+    # normal: gradient of ((|x/a1|^(2/e2) + |y/a2|^(2/e2))^(e2/e1) + |z/a3|^(2/e1)), direction
+    # (C(eta,2-e1) C(w,2-e2) / a1, C(eta,2-e1) S(w,2-e2) / a2, S(eta,2-e1) / a3)
+    ce = np.cos(eta)
+    ce = np.where(np.abs(ce) < 1e-12, 1e-12, ce)
+    nr = np.stack([_spow(ce, 2 - e1) * _spow(np.cos(w), 2 - e2) / a[0], _spow(ce, 2 - e1) * _spow(np.sin(w), 2 - e2) / a[1],
+                   _spow(np.sin(eta), 2 - e1) / a[2]], axis=-1)
+    nr /= np.linalg.norm(nr, axis=1, keepdims=True)
+    p = p + rng.normal(scale=noise, size=p.shape)
+    return np.ascontiguousarray(np.concatenate([p, nr], axis=1))
+
+
+def batch_sizes(k: int = 256, lo: float = 2e3, hi: float = 2e5, seed: int = 0) -> np.ndarray:
+    """Cloud sizes round(exp(U[ln lo, ln hi]))."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), size=k))).astype(np.int64)

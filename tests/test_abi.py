@@ -53,7 +53,8 @@ def test_struct_layouts_match_the_header():
 #include <stdio.h>
 #include "prx.h"
 int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(prx_point), sizeof(prx_params), sizeof(prx_stats),
- sizeof(prx_mesh), sizeof(prx_solve_opts), sizeof(prx_solve_report));return 0;}'''
+ sizeof(prx_mesh), sizeof(prx_solve_opts), sizeof(prx_solve_report));
+printf("%zu %zu\n", sizeof(prx_vjp_report), sizeof(prx_batch_report));return 0;}'''
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "s.c")
@@ -62,7 +63,8 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(prx_point), sizeof(prx
         subprocess.run(["gcc", "-I", src, c, "-o", exe], check=True)
         sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split()))
     assert sizes == [L.POINT_DTYPE.itemsize, ctypes.sizeof(L.Params), ctypes.sizeof(L.Stats), ctypes.sizeof(L.Mesh),
-                     ctypes.sizeof(L.SolveOpts), ctypes.sizeof(L.SolveReport)]
+                     ctypes.sizeof(L.SolveOpts), ctypes.sizeof(L.SolveReport), ctypes.sizeof(L.VjpReport),
+                     ctypes.sizeof(L.BatchReport)]
 
 
 def test_unit_transform_is_host_code_and_matches_the_oracle():

@@ -1,0 +1,95 @@
+"""Batched solves (prx_solve_batched) against the same clouds solved one by
+one with the single-tree cascadic solve (the path the other GPU tests pin to
+the oracle).  Each instance's batched CG performs the single tree's
+arithmetic, so solutions agree to round-off and iteration counts match."""
+import numpy as np
+import pytest
+
+import oracle
+import poissonreconstruction_amd as pr
+from poissonreconstruction_amd import synthetic
+from poissonreconstruction_amd.batch import Batch, depth_for
+
+pytestmark = pytest.mark.gpu
+
+CLOUDS = [(3000, 4.0, 11), (8000, 4.0, 12), (24000, 4.0, 13), (5000, 0.0, 14), (12000, 2.0, 15)]
+
+
+def unit(raw):
+    m, _ = oracle.unit_transform(raw, 1.1)
+    return oracle.apply_transform(raw, m)
+
+
+def clouds():
+    return [(depth_for(n), unit(synthetic.superquadric(n, seed=sd)), pw) for n, pw, sd in CLOUDS]
+
+
+def single(D, pts, pw, eps, mode, refresh):
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    rep = t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=pr.PRX_PREC_FP64, residual_mode=mode,
+                           refresh_period=refresh)
+    return t.solution, rep, t
+
+
+@pytest.mark.parametrize("mode,eps,refresh", [(pr.PRX_RESID_TRUE, 1e-10, 0), (pr.PRX_RESID_REFERENCE, 1e-6, 10)])
+def test_batch_matches_single_tree_solves(gpu, mode, eps, refresh):
+    cs = clouds()
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    rep, per = B.solve(gs=20, cg_iters=800000, eps=eps, residual_mode=mode, refresh_period=refresh, autotune=False)
+    assert rep["instances"] == len(cs) and rep["converged"] == len(cs)
+    for k, (D, pts, pw) in enumerate(cs):
+        xs, rs, _ = single(D, pts, pw, eps, mode, refresh)
+        xb = B.tree(k).solution
+        scale = np.abs(xs).max()
+        assert np.abs(xb - xs).max() <= 1e-9 * scale, k
+        assert abs(per[k]["fine_iters"] - rs["fine_iters"]) <= 2, (k, per[k]["fine_iters"], rs["fine_iters"])
+        assert abs(per[k]["coarse_iters"] - rs["coarse_iters"]) <= 2
+        if mode == pr.PRX_RESID_TRUE:
+            assert per[k]["true_rel"] <= eps * 1.0001
+
+
+def test_batch_graph_replays_and_autotune(gpu):
+    cs = clouds()
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    ref = [single(D, pts, pw, 1e-10, pr.PRX_RESID_TRUE, 0)[0] for D, pts, pw in cs]
+    for rnd in range(3):  # the captured graphs are replayed by every solve
+        rep, per = B.solve(eps=1e-10, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=True)
+        assert rep["tile"] in (8, 16) and rep["gs_block"] in (64, 128, 256, 512)
+        assert rep["chunk"] % 2 == 0 and 4 <= rep["chunk"] <= 64
+        assert rep["graph_launches"] > 0
+        for k in range(len(cs)):
+            xb = B.tree(k).solution
+            assert np.abs(xb - ref[k]).max() <= 1e-9 * np.abs(ref[k]).max(), (rnd, k)
+
+
+def test_batch_trees_evaluate_like_single_trees(gpu):
+    cs = clouds()[:3]
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    B.solve(eps=1e-10, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False)
+    for k, (D, pts, pw) in enumerate(cs):
+        q = np.stack([pts["px"], pts["py"], pts["pz"]], axis=1)
+        tb = B.tree(k)
+        _, _, ts = single(D, pts, pw, 1e-10, pr.PRX_RESID_TRUE, 0)
+        ts.solution = tb.solution
+        assert np.array_equal(tb.evaluate_many(q), ts.evaluate_many(q))
+        assert tb.iso_value() == ts.iso_value()
+
+
+def test_batch_rejects_what_it_does_not_do(gpu):
+    B = Batch()
+    with pytest.raises(pr.PrxError):
+        B.add(5, unit(synthetic.sphere(1000)), 4.0)  # base depth D-2 must be >= 4
+    B.add(6, unit(synthetic.sphere(1000)), 4.0)
+    import ctypes
+    from poissonreconstruction_amd import _lib as L
+    o = L.default_solve_opts()
+    o.precision = L.PRX_PREC_MIXED
+    rc = L.lib().prx_solve_batched(B._h, 20, 1000, 1e-6, ctypes.byref(o), 0, None, None)
+    assert rc == L.PRX_ERR_INVALID
