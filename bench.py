@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--skip-ref", action="store_true", help="skip the FP64 1e-12 reference solve (field error)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--batch-clouds", type=int, default=256)
     ap.add_argument("--op-reps", type=int, default=20)
     ap.add_argument("--op-only", action="store_true",
                     help="only time the operator kernels and the copy kernel (for counter runs)")
@@ -130,6 +132,8 @@ def main():
     roofline = operator_roofline(L, tree, N, P, args.op_reps)
 
     adjoint = None if args.skip_adjoint else adjoint_bench(pr, L, synthetic, args.tol)
+    del tree
+    batch = None if args.skip_batch else batch_bench(pr, L, synthetic, args.tol, args.batch_clouds)
 
     cpu = None
     if not args.skip_cpu:
@@ -152,6 +156,7 @@ def main():
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
         "roofline": roofline,
         "adjoint": adjoint,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -183,6 +188,61 @@ def adjoint_bench(pr, L, synthetic, tol):
                 "vjp_device_ms": round(vrep["ms"], 2), "vjp_iters": vrep["iters"], "vjp_rel": vrep["rel"],
                 "forward_fine_iters": rep["fine_iters"], "tol": tol})
     log(f"adjoint {out}")
+    return out
+
+
+def batch_bench(pr, L, synthetic, tol, nclouds):
+    """Batching workload: `nclouds` superquadrics (sizes log-uniform in
+    [2,000, 200,000], exponents U[0.3,3], scales U[0.5,1], position noise
+    N(0,5e-3), exact normals), depth 6 below 20,000 points, 7 below 100,000,
+    else 8, point weight 4, FP64 cascadic solves to `tol` true residual: all
+    instances in one batched, graph-replayed solve (autotuned), against the
+    same instances solved one after another with the single-tree path."""
+    from poissonreconstruction_amd.batch import Batch, depth_for
+    t0 = time.time()
+    sizes = synthetic.batch_sizes(nclouds, seed=0)
+    B = Batch()
+    for k, n in enumerate(sizes):
+        raw = synthetic.superquadric(int(n), seed=1000 + k)
+        u, _, _ = L.unit_transform(raw, 1.1)
+        B.add(depth_for(int(n)), u, 4.0)
+    log(f"batch setup: {nclouds} clouds, {int(sizes.sum())} points in {time.time() - t0:.1f}s")
+    runs = []
+    for i in range(3):
+        ts = time.perf_counter()
+        rep, per = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True)
+        wall = (time.perf_counter() - ts) * 1e3
+        log(f"batch solve {i}: device {rep['ms_total']:.1f} ms wall {wall:.1f} ms converged {rep['converged']}/"
+            f"{rep['instances']} max_true_rel {rep['max_true_rel']:.3e} tile {rep['tile']} chunk {rep['chunk']} "
+            f"gs_block {rep['gs_block']}")
+        runs.append((rep, wall))
+    rep, wall = runs[-1]
+    ts = time.perf_counter()
+    seq_iters = 0
+    for k in range(len(sizes)):
+        t = B.tree(k)
+        r = t.solve_cascadic(None, 4.0, t.max_depth - 2, 20, 800000, tol, precision=L.PRX_PREC_FP64,
+                             residual_mode=L.PRX_RESID_TRUE, refresh_period=0)
+        seq_iters += r["fine_iters"]
+    seq_ms = (time.perf_counter() - ts) * 1e3
+    log(f"sequential single-tree solves: {seq_ms:.1f} ms ({seq_iters} fine iterations)")
+    depths = [depth_for(int(n)) for n in sizes]
+    out = {"workload": f"{nclouds} superquadrics, sizes log-uniform [2e3, 2e5], depth 6/7/8 by size, point weight 4, "
+                       f"FP64 to {tol:g} true residual",
+           "clouds": nclouds, "points": int(sizes.sum()), "nodes": int(rep["total_nodes"]),
+           "depth_counts": {d: depths.count(d) for d in (6, 7, 8)},
+           "ms_batched": round(rep["ms_total"], 2), "ms_batched_wall": round(wall, 2),
+           "ms_sequential": round(seq_ms, 2), "speedup_vs_sequential": round(seq_ms / wall, 3),
+           "nodes_per_s": round(rep["total_nodes"] / (rep["ms_total"] * 1e-3), 1),
+           "converged": rep["converged"], "max_true_rel": rep["max_true_rel"],
+           "max_fine_iters": rep["max_fine_iters"], "sum_fine_iters": rep["sum_fine_iters"],
+           "ms_coarse": round(rep["ms_coarse"], 2), "ms_fine_gs": round(rep["ms_fine_gs"], 2),
+           "ms_fine_cg": round(rep["ms_fine_cg"], 2),
+           "autotuned": {"tile": rep["tile"], "chunk": rep["chunk"], "gs_block": rep["gs_block"],
+                         "ms_tune": round(rep["ms_tune"], 1)},
+           "graph_launches": rep["graph_launches"]}
+    B.close()
+    log(f"batch {out}")
     return out
 
 
