@@ -106,6 +106,7 @@ typedef struct prx_solve_opts {
     int32_t gs_iters;
     double inner_tol;
     uint64_t max_outer;
+    double coarse_eps;      /* PRX_RESID_TRUE only: coarsest-level ||b-Ax|| <= coarse_eps ||b||; 0 = eps */
 } prx_solve_opts;
 
 typedef struct prx_solve_report {

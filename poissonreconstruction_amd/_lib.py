@@ -62,7 +62,8 @@ class Mesh(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
-                ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64)]
+                ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
+                ("coarse_eps", ctypes.c_double)]
 
 
 class SolveReport(ctypes.Structure):

@@ -595,7 +595,8 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         }
         launch_fill<double>(C.x.p, 0.0, C.N, s);
         if (gs > 0) launch_graph(B, graph_for(B, C, 1, (int)gs, 0));
-        co = batch_cg(B, C, cg_iters, eps, mode, mode == PRX_RESID_TRUE ? 0 : 10);
+        const double ceps = mode == PRX_RESID_TRUE && o.coarse_eps > 0.0 ? o.coarse_eps : eps;
+        co = batch_cg(B, C, cg_iters, ceps, mode, mode == PRX_RESID_TRUE ? 0 : 10);
         for (int k = 0; k < K; ++k) {
             prx_tree* t = B->inst[k];
             launch_prolongate(t->D, t->D - 2, C.x.p + C.off_h[k], F.x.p + F.off_h[k], s);

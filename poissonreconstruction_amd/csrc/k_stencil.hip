@@ -105,6 +105,30 @@ constexpr int lds_lin(uint32_t m) {
     return (int)(compact3c(m) + E * compact3c(m >> 1) + E * E * compact3c(m >> 2));
 }
 
+// LDS placement of the output staging (flat local Morton index m): 4-element
+// groups stay contiguous (16-byte reads), and for 16^3 bricks Morton bits 6
+// and 9 are XORed into the group index so that the column-wise writes of a
+// half-wave (x bits 0,3,6,9 and the row pair) spread over all 32 banks.
+template <int B>
+__device__ __forceinline__ uint32_t stage_at(uint32_t m) {
+    if constexpr (B == 16) return m ^ ((((m >> 6) & 1u) | (((m >> 9) & 1u) << 2)) << 2);
+    else return m;
+}
+// thread -> (x,y) column of the z sweep.  For 16^3 bricks the two rows read by
+// one half-wave are 8 apart: with the 18-word row pitch they start 144 = 16
+// (mod 32) banks apart, so the 9 window reads of a z step are conflict-free.
+template <int B>
+__device__ __forceinline__ void column_of(int tid, int& tx, int& ty) {
+    if constexpr (B == 16) {
+        tx = tid & 15;
+        const int q = tid >> 4;
+        ty = ((q & 1) << 3) | (q >> 1);
+    } else {
+        tx = tid % B;
+        ty = tid / B;
+    }
+}
+
 // halo shell index -> (hx,hy,hz) in [-1,B]^3 \\ [0,B)^3
 template <int B>
 __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
@@ -121,6 +145,40 @@ __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
         if (j < E) { hy = -1; hx = j - 1; }
         else if (j < 2 * E) { hy = B; hx = j - E - 1; }
         else { j -= 2 * E; hy = j >> 1; hx = (j & 1) ? B : -1; }
+    }
+}
+
+// n consecutive elements (an aligned run) as one vector load
+template <typename U, int n>
+__device__ __forceinline__ void ldrun(const U* p, U* o) {
+    if constexpr (n == 1) {
+        o[0] = *p;
+    } else {
+        typedef U vt __attribute__((ext_vector_type(n)));
+        const vt v = *(const vt*)p;
+#pragma unroll
+        for (int e = 0; e < n; ++e) o[e] = v[e];
+    }
+}
+// the operand value of the pass at n nodes starting at g, formed as the
+// interior is formed (x ; x64 + e32 ; fma(beta, d_old, r))
+template <typename T, int OP, int n>
+__device__ __forceinline__ void halo_run(const StencilIO<T>& io, uint64_t g, double bt, T* o) {
+    if constexpr (OP == OP_RESID_UPD) {
+        double a[n];
+        float e[n];
+        ldrun<double, n>(io.x64 + g, a);
+        ldrun<float, n>(io.e32 + g, e);
+#pragma unroll
+        for (int k = 0; k < n; ++k) o[k] = a[k] + (double)e[k];
+    } else if constexpr (OP == OP_DIR_APPLY) {
+        T r[n], d[n];
+        ldrun<T, n>(io.r + g, r);
+        ldrun<T, n>(io.dold + g, d);
+#pragma unroll
+        for (int k = 0; k < n; ++k) o[k] = (T)fma(bt, (double)d[k], (double)r[k]);
+    } else {
+        ldrun<T, n>(io.x + g, o);
     }
 }
 
@@ -182,27 +240,97 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             qa[v] = ld_v(io.x + g);
         }
     }
-    // ---- 2. issue the one-node halo loads (x = 0 outside the grid)
-    T hv[NH];
-    int hl[NH];
-    bool hin[NH];
+    // ---- 2. issue the one-node halo loads (x = 0 outside the grid).  16^3
+    // bricks: face by face from the six face-, twelve edge- and eight
+    // corner-neighbour bricks.  In Morton order a z face (local z = 15 or 0)
+    // is 64 aligned runs of 4 nodes (2x2 in x,y) and a y face 128 runs of 2
+    // (x pairs), so they load as vectors; x faces, edges and corners are
+    // single nodes.  A face lies in one neighbour brick, so the grid test is
+    // per brick.
+    T zf[4], yf[2][2], xf[2], ev;
+    int zi = 0, yi[2], xi[2], ei = -1;
+    T hv[B == 16 ? 1 : NH];
+    int hl[B == 16 ? 1 : NH];
+    bool hin[B == 16 ? 1 : NH];
+    if constexpr (B == 16) {
+        const int bx = (int)(ox >> 4), by = (int)(oy >> 4), bz = (int)(oz >> 4), nbr = (int)(res >> 4);
+        auto nb_base = [&](int dx, int dy, int dz, bool& ok) -> uint64_t {
+            const int cx = bx + dx, cy = by + dy, cz = bz + dz;
+            ok = (unsigned)cx < (unsigned)nbr && (unsigned)cy < (unsigned)nbr && (unsigned)cz < (unsigned)nbr;
+            return ok ? ib + (uint64_t)morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) * B3 : 0;
+        };
+        auto tix = [](int hx, int hy, int hz) { return (hx + 1) + E * (hy + 1) + E * E * (hz + 1); };
 #pragma unroll
-    for (int k = 0; k < NH; ++k) {
-        const int i = tid + k * NT;
-        hl[k] = -1;
-        hin[k] = false;
-        hv[k] = T(0);
-        if (i < S) {
+        for (int k = 0; k < 4; ++k) zf[k] = T(0);
+        if (tid < 128) {  // z faces: one run of 4 per thread
+            const int side = tid >> 6, a = tid & 7, b = (tid >> 3) & 7;
+            bool ok;
+            const uint64_t sb = nb_base(0, 0, side ? 1 : -1, ok);
+            const uint32_t m0 = spread3(2 * a) | (spread3(2 * b) << 1) | (side ? 0u : (spread3(15u) << 2));
+            if (ok) halo_run<T, OP, 4>(io, sb + m0, bt, zf);
+            zi = tix(2 * a, 2 * b, side ? B : -1);
+        }
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {  // y faces: threads 128..255, a run of 2 on each side
+            yf[sd][0] = yf[sd][1] = T(0);
+            const int t = tid & 127, a = t & 7, c = t >> 3;
+            yi[sd] = tix(2 * a, sd ? B : -1, c);
+            if (tid >= 128) {
+                bool ok;
+                const uint64_t sb = nb_base(0, sd ? 1 : -1, 0, ok);
+                const uint32_t m0 = spread3(2 * a) | ((sd ? 0u : spread3(15u)) << 1) | (spread3((uint32_t)c) << 2);
+                if (ok) halo_run<T, OP, 2>(io, sb + m0, bt, yf[sd]);
+            }
+        }
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {  // x faces: one node on each side per thread
+            const int yy = tid & 15, zz = tid >> 4;
+            xf[sd] = T(0);
+            bool ok;
+            const uint64_t sb = nb_base(sd ? 1 : -1, 0, 0, ok);
+            const uint32_t m = (sd ? 0u : spread3(15u)) | (spread3((uint32_t)yy) << 1) | (spread3((uint32_t)zz) << 2);
+            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, &xf[sd]);
+            xi[sd] = tix(sd ? B : -1, yy, zz);
+        }
+        ev = T(0);
+        if (tid < 200) {  // 12 edges of 16 nodes (threads 0..191), 8 corners (192..199)
             int hx, hy, hz;
-            shell_pos<B>(i, hx, hy, hz);
-            hl[k] = ((hz + 1) * E + hy + 1) * E + hx + 1;
-            const int gx = (int)ox + hx, gy = (int)oy + hy, gz = (int)oz + hz;
-            if ((unsigned)gx < res && (unsigned)gy < res && (unsigned)gz < res) {
-                const uint64_t g = ib + morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
-                hin[k] = true;
-                if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
-                else if constexpr (OP == OP_DIR_APPLY) hv[k] = (T)fma(bt, (double)io.dold[g], (double)io.r[g]);
-                else hv[k] = io.x[g];
+            if (tid < 192) {
+                const int e = tid >> 4, u = tid & 15, axis = e >> 2, s1 = (e & 1) ? B : -1, s2 = (e & 2) ? B : -1;
+                if (axis == 0) { hx = u; hy = s1; hz = s2; }
+                else if (axis == 1) { hy = u; hx = s1; hz = s2; }
+                else { hz = u; hx = s1; hy = s2; }
+            } else {
+                const int c = tid - 192;
+                hx = (c & 1) ? B : -1;
+                hy = (c & 2) ? B : -1;
+                hz = (c & 4) ? B : -1;
+            }
+            bool ok;
+            const uint64_t sb = nb_base(hx < 0 ? -1 : (hx >= B ? 1 : 0), hy < 0 ? -1 : (hy >= B ? 1 : 0), hz < 0 ? -1 : (hz >= B ? 1 : 0), ok);
+            const uint32_t m = spread3((uint32_t)(hx & 15)) | (spread3((uint32_t)(hy & 15)) << 1) | (spread3((uint32_t)(hz & 15)) << 2);
+            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, &ev);
+            ei = tix(hx, hy, hz);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int i = tid + k * NT;
+            hl[k] = -1;
+            hin[k] = false;
+            hv[k] = T(0);
+            if (i < S) {
+                int hx, hy, hz;
+                shell_pos<B>(i, hx, hy, hz);
+                hl[k] = ((hz + 1) * E + hy + 1) * E + hx + 1;
+                const int gx = (int)ox + hx, gy = (int)oy + hy, gz = (int)oz + hz;
+                if ((unsigned)gx < res && (unsigned)gy < res && (unsigned)gz < res) {
+                    const uint64_t g = ib + morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+                    hin[k] = true;
+                    if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
+                    else if constexpr (OP == OP_DIR_APPLY) hv[k] = (T)fma(bt, (double)io.dold[g], (double)io.r[g]);
+                    else hv[k] = io.x[g];
+                }
             }
         }
     }
@@ -237,13 +365,32 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         for (int e = 0; e < VEC; ++e)
             tile[lbase + lds_lin<E>((uint32_t)(v * NT * VEC)) + lds_lin<E>((uint32_t)e)] = xin[v * VEC + e] - r0;
     }
+    if constexpr (B == 16) {
+        if (tid < 128) {
+            tile[zi] = zf[0] - r0;
+            tile[zi + 1] = zf[1] - r0;
+            tile[zi + E] = zf[2] - r0;
+            tile[zi + E + 1] = zf[3] - r0;
+        } else {
 #pragma unroll
-    for (int k = 0; k < NH; ++k)
-        if (hl[k] >= 0) tile[hl[k]] = hin[k] ? hv[k] - r0 : -r0;
+            for (int sd = 0; sd < 2; ++sd) {
+                tile[yi[sd]] = yf[sd][0] - r0;
+                tile[yi[sd] + 1] = yf[sd][1] - r0;
+            }
+        }
+        tile[xi[0]] = xf[0] - r0;
+        tile[xi[1]] = xf[1] - r0;
+        if (ei >= 0) tile[ei] = ev - r0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NH; ++k)
+            if (hl[k] >= 0) tile[hl[k]] = hin[k] ? hv[k] - r0 : -r0;
+    }
     __syncthreads();
 
     // ---- 4. z sweep with a 3-plane register window (values are x - r0)
-    const int tx = tid % B, ty = tid / B;
+    int tx, ty;
+    column_of<B>(tid, tx, ty);
     T out[B];
     {
         const T sc = L.s, wg = L.wg;
@@ -272,7 +419,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     {
         const uint32_t mxy = spread3((uint32_t)tx) | (spread3((uint32_t)ty) << 1);
 #pragma unroll
-        for (int z = 0; z < B; ++z) tile[mxy | (spread3c((uint32_t)z) << 2)] = out[z];
+        for (int z = 0; z < B; ++z) tile[stage_at<B>(mxy | (spread3c((uint32_t)z) << 2))] = out[z];
     }
     __syncthreads();
 
@@ -284,7 +431,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 const uint32_t loc = (uint32_t)(L.node[t] - base);
                 T acc = 0;
                 for (uint32_t j = L.off[t]; j < L.off[t + 1]; ++j) acc += L.phi[j] * io.s[L.sp[j]];
-                tile[loc] += acc;
+                tile[stage_at<B>(loc)] += acc;
             }
             __syncthreads();
         }
@@ -301,7 +448,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                q[e] = tile[m + e];
+                q[e] = tile[stage_at<B>(m) + e];
                 tacc += xin[v * VEC + e] * q[e];
             }
             *(NV*)(io.y + g) = q;
@@ -310,7 +457,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                q[e] = bb[e] - tile[m + e];
+                q[e] = bb[e] - tile[stage_at<B>(m) + e];
                 tacc += q[e] * q[e];
             }
             *(NV*)(io.y + g) = q;
@@ -320,7 +467,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             xo[0] = xin[v * VEC];
             xo[1] = xin[v * VEC + 1];
             __builtin_nontemporal_store(xo, (nd2*)(io.x_out + g));
-            const double q0 = bb[0] - tile[m], q1 = bb[1] - tile[m + 1];
+            const double q0 = bb[0] - tile[stage_at<B>(m)], q1 = bb[1] - tile[stage_at<B>(m) + 1];
             nf2 rr;
             rr[0] = (float)(q0 * io.rscale);
             rr[1] = (float)(q1 * io.rscale);

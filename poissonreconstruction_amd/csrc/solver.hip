@@ -424,7 +424,8 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
         launch_fill<double>(t->x.p, 0.0, t->N, s);
     } else {
         Timer tm(s);
-        coarse_solve(t, c.base, c.gs, c.cg_iters, c.eps, c.o.residual_mode, rep);
+        const double ceps = c.o.residual_mode == PRX_RESID_TRUE && c.o.coarse_eps > 0.0 ? c.o.coarse_eps : c.eps;
+        coarse_solve(t, c.base, c.gs, c.cg_iters, ceps, c.o.residual_mode, rep);
         launch_prolongate(t->D, c.base, t->cx.p, t->x.p, s);
         rep.ms_coarse = tm.ms();
     }
@@ -733,6 +734,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->gs_iters = 20;
     o->inner_tol = 0.1;
     o->max_outer = 64;
+    o->coarse_eps = 0.0;
 }
 
 int prx_tree_create(uint32_t depth, int device, prx_tree** out) {
