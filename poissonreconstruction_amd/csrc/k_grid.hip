@@ -7,6 +7,8 @@
 // twice, as Rust does.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace prx {
@@ -117,6 +119,16 @@ void launch_dot(const T* a, const T* b, int64_t n, double* partial, int* nblk, h
 template void launch_dot<double>(const double*, const double*, int64_t, double*, int*, hipStream_t);
 template void launch_dot<float>(const float*, const float*, int64_t, double*, int*, hipStream_t);
 
+// One CG vector update at a node, evaluated in FP64 and rounded once to the
+// vector type (for the FP32 correction system this keeps r consistent with
+// x to one FP32 rounding per step; FP32 arithmetic here costs iterations).
+template <typename T>
+__device__ __forceinline__ void cg_upd(T& x, T& r, T d, T q, double a, double& acc) {
+    x = (T)((double)x + a * (double)d);
+    r = (T)((double)r - a * (double)q);
+    acc += (double)r * (double)r;
+}
+
 // The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
 __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
     switch (op) {
@@ -169,19 +181,24 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
     }
 }
 
-// 1024 threads fold np partials in a fixed order (4 strided accumulators,
-// then the block sum), so a reduction is reproducible run to run.
+// 1024 threads fold np partials in a fixed order (16 strided accumulators,
+// a fixed pairwise tree, then the block sum), so a reduction is reproducible
+// run to run; 16 independent loads per thread are in flight per round.
 __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double a[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) a[u] = 0.0;
     int64_t i = threadIdx.x;
-    for (; i + 3 * 1024 < np; i += 4 * 1024) {
-        a0 += partial[i];
-        a1 += partial[i + 1024];
-        a2 += partial[i + 2048];
-        a3 += partial[i + 3072];
+    for (; i + 15 * 1024 < np; i += 16 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] += partial[i + u * 1024];
     }
-    for (; i < np; i += 1024) a0 += partial[i];
-    return block_sum_256((a0 + a1) + (a2 + a3), ws);
+    for (; i < np; i += 1024) a[0] += partial[i];
+#pragma unroll
+    for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+        for (int u = 0; u < w; ++u) a[u] += a[u + w];
+    return block_sum_256(a[0], ws);
 }
 
 // One block folds the partials in a fixed order and advances the CG scalars
@@ -216,38 +233,43 @@ void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, i
 }
 
 // x += alpha_k d ; r -= alpha_k q ; partial = r.r per block.  A block covers
-// 256 * VEC nodes inside one 4096-node brick, so it belongs to one instance.
+// 1024 vectors (4096 float or 2048 double nodes) inside one 4096-node brick,
+// so it belongs to one instance.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bcg_update(BatchTab tab, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
                                                     const T* __restrict__ q, double* partial) {
     __shared__ double ws[4];
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
-    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
+    const uint32_t k = tab.brick_inst[((uint64_t)blockIdx.x * 1024 * VEC) >> 12];
     if (!b_runs(tab, k)) return;
     const double a = tab.st[k].alpha;
     double acc = 0.0;
-    const NV xv = ld_nt(x + j), rv = ld_nt(r + j), dv = ld_nt(d + j), qv = ld_nt(q + j);
-    NV xo, ro;
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-        xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
-        ro[e] = (T)((double)rv[e] - a * (double)qv[e]);
-        acc += (double)ro[e] * (double)ro[e];
+    for (int u = 0; u < 4; ++u) {
+        const int64_t j = ((int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x) * VEC;
+        NV xv = ld_nt(x + j), rv = ld_nt(r + j);
+        const NV dv = ld_nt(d + j), qv = ld_nt(q + j);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            T xe = xv[e], re = rv[e];
+            cg_upd<T>(xe, re, dv[e], qv[e], a, acc);
+            xv[e] = xe;
+            rv[e] = re;
+        }
+        st_nt(x + j, xv);
+        st_nt(r + j, rv);
     }
-    st_nt(x + j, xo);
-    st_nt(r + j, ro);
     const double sm = block_sum_256(acc, ws);
     if (threadIdx.x == 0) partial[blockIdx.x] = sm;
 }
 template <typename T>
 int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* d, const T* q, int64_t Ntot, double* partial, hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t nb = Ntot / (256 * VEC);
+    const int64_t nb = Ntot / (1024 * VEC);
     hipLaunchKernelGGL(k_bcg_update<T>, dim3((unsigned)nb), dim3(256), 0, s, tab, x, r, d, q, partial);
     PRX_LAUNCH_CHECK();
-    return VEC == 2 ? 9 : 10;
+    return VEC == 2 ? 11 : 12;
 }
 template int launch_bcg_update<double>(const BatchTab&, double*, double*, const double*, const double*, int64_t, double*, hipStream_t);
 template int launch_bcg_update<float>(const BatchTab&, float*, float*, const float*, const float*, int64_t, double*, hipStream_t);
@@ -301,8 +323,9 @@ template int launch_bdot<double>(const BatchTab&, const double*, const double*, 
 template int launch_bdot<float>(const BatchTab&, const float*, const float*, int64_t, double*, hipStream_t);
 
 // x += alpha d ; r -= alpha q ; partial = r.r   (src/solvers.rs:47, :53, :57)
-// One 16-byte vector per array per thread over a flat grid, streamed with
-// non-temporal loads and stores (every operand is touched once per pass).
+// Four 16-byte vectors per array per thread (1024 vectors per block, each
+// quarter coalesced), streamed with non-temporal loads and stores; one
+// partial per block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
                                                    const T* __restrict__ q, const CgState* st, int64_t n, double* partial) {
@@ -312,26 +335,33 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restr
     constexpr int VEC = 16 / (int)sizeof(T);
     const double a = st->alpha;
     double acc = 0.0;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nv = n / VEC;
-    if (i < nv) {
-        const int64_t j = i * VEC;
-        const NV xv = ld_nt(x + j), rv = ld_nt(r + j), dv = ld_nt(d + j), qv = ld_nt(q + j);
-        NV xo, ro;
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) {
-            xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
-            ro[e] = (T)((double)rv[e] - a * (double)qv[e]);
-            acc += (double)ro[e] * (double)ro[e];
+    for (int u = 0; u < 4; ++u) {
+        const int64_t i = (int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x;
+        if (i < nv) {
+            const int64_t j = i * VEC;
+            NV xv = ld_nt(x + j), rv = ld_nt(r + j);
+            const NV dv = ld_nt(d + j), qv = ld_nt(q + j);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                T xe = xv[e], re = rv[e];
+                cg_upd<T>(xe, re, dv[e], qv[e], a, acc);
+                xv[e] = xe;
+                rv[e] = re;
+            }
+            st_nt(x + j, xv);
+            st_nt(r + j, rv);
         }
-        st_nt(x + j, xo);
-        st_nt(r + j, ro);
-    } else if (i - nv < n - nv * VEC) {  // tail (n not a multiple of the vector width)
-        const int64_t j = nv * VEC + (i - nv);
-        x[j] = (T)((double)x[j] + a * (double)d[j]);
-        const T rn = (T)((double)r[j] - a * (double)q[j]);
-        r[j] = rn;
-        acc = (double)rn * (double)rn;
+    }
+    if (blockIdx.x == gridDim.x - 1) {  // tail (n not a multiple of the vector width)
+        const int64_t j = nv * VEC + threadIdx.x;
+        if (j < n) {
+            T xs = x[j], rs = r[j];
+            cg_upd<T>(xs, rs, d[j], q[j], a, acc);
+            x[j] = xs;
+            r[j] = rs;
+        }
     }
     double s = block_sum_256(acc, ws);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -339,8 +369,8 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restr
 template <typename T>
 void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t threads = n / VEC + (n % VEC);
-    const int nb = (int)((threads + 255) / 256);
+    const int64_t nv = n / VEC;
+    const int nb = (int)std::max<int64_t>(1, (nv + 1023) / 1024);
     hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, s, x, r, d, q, st, n, partial);
     PRX_LAUNCH_CHECK();
     *nblk = nb;

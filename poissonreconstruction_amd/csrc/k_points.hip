@@ -389,7 +389,7 @@ __global__ void k_screen_gather(uint32_t D, int64_t P, const uint32_t* __restric
         T v;
         if (MODE == 0) v = x[m];
         else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else v = (T)fma(bt, (double)dd[m], (double)r[m]);
+        else v = dir_form(bt, dd[m], r[m]);
         acc += bx * by * bz * v;
     }
     s[i] = wh3 * acc;
@@ -447,7 +447,7 @@ __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __rest
         T v;
         if (MODE == 0) v = x[m];
         else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else v = (T)fma(bt, (double)dd[m], (double)r[m]);
+        else v = dir_form(bt, dd[m], r[m]);
         acc += bx * by * bz * v;
     }
     const T wh3 = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;

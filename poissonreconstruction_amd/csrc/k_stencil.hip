@@ -176,7 +176,7 @@ __device__ __forceinline__ void halo_run(const StencilIO<T>& io, uint64_t g, dou
         ldrun<T, n>(io.r + g, r);
         ldrun<T, n>(io.dold + g, d);
 #pragma unroll
-        for (int k = 0; k < n; ++k) o[k] = (T)fma(bt, (double)d[k], (double)r[k]);
+        for (int k = 0; k < n; ++k) o[k] = dir_form(bt, d[k], r[k]);
     } else {
         ldrun<T, n>(io.x + g, o);
     }
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                     const uint64_t g = ib + morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
                     hin[k] = true;
                     if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
-                    else if constexpr (OP == OP_DIR_APPLY) hv[k] = (T)fma(bt, (double)io.dold[g], (double)io.r[g]);
+                    else if constexpr (OP == OP_DIR_APPLY) hv[k] = dir_form(bt, io.dold[g], io.r[g]);
                     else hv[k] = io.x[g];
                 }
             }
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     // brick reference value r0 (the brick's first node, formed as the node is)
     T r0;
     if constexpr (OP == OP_RESID_UPD) r0 = io.x64[base] + (double)io.e32[base];
-    else if constexpr (OP == OP_DIR_APPLY) r0 = (T)fma(bt, (double)io.dold[base], (double)io.r[base]);
+    else if constexpr (OP == OP_DIR_APPLY) r0 = dir_form(bt, io.dold[base], io.r[base]);
     else r0 = io.x[base];
     // ---- 3. interior values into LDS (offset linear in the Morton bits)
     T xin[PER];
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             NV nq;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                nq[e] = (T)fma(bt, (double)qb[v][e], (double)qa[v][e]);
+                nq[e] = dir_form(bt, (T)qb[v][e], (T)qa[v][e]);
                 xin[v * VEC + e] = nq[e];
             }
             st_nt(io.dnew + g, nq);
@@ -505,7 +505,7 @@ __device__ inline T apply_node(const LevelArgs<T>& L, const T* x, const double* 
         if ((unsigned)jx >= L.res || (unsigned)jy >= L.res || (unsigned)jz >= L.res) return T(0);
         const uint64_t j = ib + morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
         if constexpr (MODE == 1) return (T)(x64[j] + (double)e32[j]);
-        else if constexpr (MODE == 2) return (T)fma(bt, (double)dd[j], (double)x[j]);
+        else if constexpr (MODE == 2) return dir_form(bt, dd[j], x[j]);
         else return x[j];
     };
     const T xc = X(ix, iy, iz);
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
         } else if constexpr (OP == OP_DIR_APPLY) {
             const double bt = *io.beta;
             const T a = apply_node<T, 2>(L, io.r, nullptr, nullptr, io.s, ix, iy, iz, i, nullptr, io.dold, bt);
-            const T dn = (T)fma(bt, (double)io.dold[i], (double)io.r[i]);
+            const T dn = dir_form(bt, io.dold[i], io.r[i]);
             io.dnew[i] = dn;
             io.y[i] = a;
             acc = (double)dn * (double)a;

@@ -105,6 +105,14 @@ __device__ inline typename NV16<T>::type ld_v(const T* p) {
 
 // Touched-node lists: one entry per (point, corner) that reaches a node,
 // sorted by node and, within a node, by input point order.
+// CG direction d_new = r + beta*d_old as every kernel forms it (the same
+// expression wherever a node value is formed, so a node read as halo by a
+// neighbouring workgroup is bit-identical to its own interior value): one
+// FP64 fma, rounded once to the vector type (an FP32 fma with beta rounded
+// to FP32 measurably costs CG iterations of the FP32 correction system).
+__device__ __forceinline__ double dir_form(double bt, double d, double r) { return fma(bt, d, r); }
+__device__ __forceinline__ float dir_form(double bt, float d, float r) { return (float)fma(bt, (double)d, (double)r); }
+
 struct NodeList {
     int mode = 0;                 // 0 = floor (splat), 1 = trunc (screening)
     int64_t T = 0;                // touched nodes
