@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--batch-clouds", type=int, default=256)
     ap.add_argument("--op-reps", type=int, default=20)
+    ap.add_argument("--large", action="store_true",
+                    help="run only the large workload (depth 10, 5M points on a displaced sphere, mixed) and print its line")
+    ap.add_argument("--large-fp64", action="store_true", help="with --large: also time the FP64 solve")
     ap.add_argument("--op-only", action="store_true",
                     help="only time the operator kernels and the copy kernel (for counter runs)")
     args = ap.parse_args()
@@ -66,6 +69,9 @@ def main():
 
     if pr.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
+    if args.large:
+        print(json.dumps(large_bench(pr, L, synthetic, args.tol, args.large_fp64)), flush=True)
+        return
     D, P, pw = args.depth, args.points, 4.0
     N = 8 ** D
     t0 = time.time()
@@ -160,6 +166,67 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def large_bench(pr, L, synthetic, tol, with_fp64):
+    """Large workload: 5,000,000 points on a unit sphere displaced radially by
+    8 random real spherical harmonics (l <= 6, amplitude 0.05), depth 10
+    (1,073,741,824 nodes), point weight 4, mixed FP32/FP64 refinement to `tol`
+    true residual, then the field at all 1025^3 marching-cubes corners on the
+    device."""
+    D, P, pw = 10, 5_000_000, 4.0
+    N = 8 ** D
+    t0 = time.time()
+    raw = synthetic.displaced_sphere(P, seed=0)
+    log(f"large: generated {P} points in {time.time() - t0:.1f}s")
+    unit, _, _ = L.unit_transform(raw, 1.1)
+    del raw
+    t0 = time.time()
+    tree = pr.FEMTree(D)
+    tree.splat_normal_field(unit)
+    L.check(L.lib().prx_assemble(tree._h, pw))
+    log(f"large: tree, splat and assembly in {time.time() - t0:.1f}s")
+
+    def solve(prec):
+        o = L.default_solve_opts()
+        o.precision = prec
+        o.residual_mode = L.PRX_RESID_TRUE
+        o.refresh_period = 0
+        o.inner_tol = 0.1
+        o.max_outer = 64
+        rep = L.SolveReport()
+        ts = time.perf_counter()
+        L.check(L.lib().prx_solve_cascadic(tree._h, D - 2, 20, 800000, tol, ctypes.byref(o), ctypes.byref(rep)))
+        ms = (time.perf_counter() - ts) * 1e3
+        r = rep.as_dict()
+        log(f"large: {'mixed' if prec == L.PRX_PREC_MIXED else 'fp64'} solve {ms:.1f} ms fine_iters={r['fine_iters']} "
+            f"outer={r['outer_iters']} coarse_iters={r['coarse_iters']} true_rel={r['true_rel']:.3e} status={r['status']}")
+        return ms, r
+
+    fp64 = solve(L.PRX_PREC_FP64) if with_fp64 else None
+    ms_mixed, rep = solve(L.PRX_PREC_MIXED)
+    cms, css = ctypes.c_double(0), ctypes.c_double(0)
+    L.check(L.lib().prx_time_evaluate_corners(tree._h, 3, 0, ctypes.byref(cms), ctypes.byref(css)))
+    gms, gss = ctypes.c_double(0), ctypes.c_double(0)
+    L.check(L.lib().prx_time_evaluate_corners(tree._h, 1, 1, ctypes.byref(gms), ctypes.byref(gss)))
+    r1 = (1 << D) + 1
+    corner_bytes = 8 * N + 8 * r1 ** 3
+    out = {"metric": "ms per solve to 1e-6 rel. residual, depth 10, 5M pts (mixed FP32/FP64)",
+           "value": round(ms_mixed, 1), "unit": "ms", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": "unit sphere displaced by 8 real spherical harmonics (l <= 6, amplitude 0.05), "
+                                  "5,000,000 points, depth 10, point weight 4, mixed FP32/FP64 to true residual "
+                                  f"{tol:g}", "depth": D, "points": P, "nodes": N},
+           "mixed": {k: rep[k] for k in ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse",
+                                         "ms_fine_gs", "ms_fine_cg", "ms_total", "status")},
+           "fp64_ms": round(fp64[0], 1) if fp64 else None,
+           "mixed_speedup_vs_fp64": round(fp64[0] / ms_mixed, 3) if fp64 else None,
+           "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
+           "corners": {"count": r1 ** 3, "ms": round(cms.value, 3), "GBs": round(corner_bytes / (cms.value * 1e-3) / 1e9, 1),
+                       "algorithmic_bytes": corner_bytes, "general_form_ms": round(gms.value, 3),
+                       "sumsq": css.value, "sumsq_general_form": gss.value,
+                       "identical": css.value == gss.value}}
+    log(f"large {out}")
+    return out
 
 
 def adjoint_bench(pr, L, synthetic, tol):

@@ -216,6 +216,12 @@ int prx_unit_transform(const prx_point* points, size_t n, double scale, double* 
  * average ms per copy. */
 int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
+/* Measurement: evaluate the current solution at every marching-cubes corner
+ * (as prx_evaluate_corners, src/fem_tree.rs:596-605) into a device buffer
+ * `reps` times; *ms = average ms per evaluation, *sumsq = sum of the squared
+ * corner values (a checksum).  general != 0 times the general per-point
+ * evaluate() instead of the corner form. */
+int prx_time_evaluate_corners(prx_tree* t, int reps, int general, double* ms, double* sumsq);
 
 /* ------------------------------------------------------------ adjoint
  * No counterpart in the reference (it is not differentiable): the vjp of the

@@ -596,13 +596,39 @@ void launch_evaluate_pointsSoA(uint32_t D, const double* sol, const double* pos,
 }
 
 // Corner positions are (off + c) * w with w = 1/res (src/fem_tree.rs:596-602).
+// evaluate() at the corner k*h (k in [0,res]^3).  There s = k exactly, so
+// every basis weight is N1(1) = 1 or N1(0) = 0: the node offset is k-1 on
+// each axis, k = 0 gives weight 0 on both offsets (F = 0), and k = res
+// clamps both offsets to res-1 with weight 1 (the node counted twice,
+// src/fem_tree.rs:645-649).  F is then x at one node added m = prod(1 or 2)
+// times in the order evaluate() adds its terms, which reproduces eval_field
+// bit for bit with one read per corner.
 __global__ void k_evaluate_corners(uint32_t D, const double* __restrict__ sol, double* out) {
+    const uint64_t res = 1ull << D, r1 = res + 1;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r1 * r1 * r1) return;
+    const uint64_t kx = i % r1, ky = (i / r1) % r1, kz = i / (r1 * r1);
+    double v = 0.0;
+    if (kx > 0 && ky > 0 && kz > 0) {
+        const double x = sol[morton3((uint32_t)(kx - 1), (uint32_t)(ky - 1), (uint32_t)(kz - 1))];
+        const int m = (kx == res ? 2 : 1) * (ky == res ? 2 : 1) * (kz == res ? 2 : 1);
+        for (int t = 0; t < m; ++t) v += x;
+    }
+    out[i] = v;
+}
+// the general evaluate() at the same corners (parity check of the form above)
+__global__ void k_evaluate_corners_general(uint32_t D, const double* __restrict__ sol, double* out) {
     const uint64_t r1 = (1ull << D) + 1;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= r1 * r1 * r1) return;
     const uint64_t kx = i % r1, ky = (i / r1) % r1, kz = i / (r1 * r1);
     const double w = 1.0 / (double)(1ull << D);
     out[i] = eval_field(D, sol, (double)kx * w, (double)ky * w, (double)kz * w);
+}
+void launch_evaluate_corners_general(uint32_t D, const double* sol, double* out, hipStream_t s) {
+    int64_t r1 = ((int64_t)1 << D) + 1;
+    hipLaunchKernelGGL(k_evaluate_corners_general, dim3(nblocks_for(r1 * r1 * r1)), dim3(256), 0, s, D, sol, out);
+    PRX_LAUNCH_CHECK();
 }
 void launch_evaluate_corners(uint32_t D, const double* sol, double* out, hipStream_t s) {
     int64_t r1 = ((int64_t)1 << D) + 1;

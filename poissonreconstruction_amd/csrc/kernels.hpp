@@ -60,6 +60,7 @@ void launch_rowmajor_to_morton(uint32_t d, const double* src, double* dst, hipSt
 void launch_evaluate(uint32_t D, const double* sol, const double* q, int64_t m, double* out, hipStream_t s);
 void launch_evaluate_pointsSoA(uint32_t D, const double* sol, const double* pos, int64_t P, double* out, hipStream_t s);
 void launch_evaluate_corners(uint32_t D, const double* sol, double* out, hipStream_t s);
+void launch_evaluate_corners_general(uint32_t D, const double* sol, double* out, hipStream_t s);
 void launch_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* x, double* y, hipStream_t s);
 
 // ---- k_points.hip

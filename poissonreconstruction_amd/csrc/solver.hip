@@ -1074,6 +1074,37 @@ int prx_evaluate_corners(prx_tree* t, double* out) {
     PRX_API_END
 }
 
+int prx_time_evaluate_corners(prx_tree* t, int reps, int general, double* ms, double* sumsq) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && reps > 0, PRX_ERR_INVALID, "bad argument");
+    PRX_REQUIRE(t->x.p, PRX_ERR_STATE, "no solution");
+    set_device(t);
+    t->ensure_state();
+    const int64_t r1 = (int64_t)t->res + 1, n = r1 * r1 * r1;
+    DArr<double> d;
+    d.ensure(n);
+    auto run = [&] {
+        if (general) launch_evaluate_corners_general(t->D, t->x.p, d.p, t->stream);
+        else launch_evaluate_corners(t->D, t->x.p, d.p, t->stream);
+    };
+    run();
+    hipEvent_t e0, e1;
+    PRX_HIP(hipEventCreate(&e0));
+    PRX_HIP(hipEventCreate(&e1));
+    PRX_HIP(hipEventRecord(e0, t->stream));
+    for (int i = 0; i < reps; ++i) run();
+    PRX_HIP(hipEventRecord(e1, t->stream));
+    PRX_HIP(hipEventSynchronize(e1));
+    float f = 0.f;
+    PRX_HIP(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = (double)f / reps;
+    const double ss = dev_dot<double>(t, d.p, d.p, n);
+    if (sumsq) *sumsq = ss;
+    PRX_API_END
+}
+
 int prx_iso_value(prx_tree* t, double* iso) {
     PRX_API_BEGIN
     PRX_REQUIRE(t && iso, PRX_ERR_INVALID, "null pointer");

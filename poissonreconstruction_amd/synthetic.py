@@ -13,6 +13,11 @@ superquadric: x = (a1 C(eta,e1) C(w,e2), a2 C(eta,e1) S(w,e2), a3 S(eta,e1)) wit
         triangles drawn by area, uniform inside, mapped through the surface),
         position noise N(0, noise^2), exact outward normals (the gradient of the
         inside-outside function at the noise-free point).
+displaced_sphere: unit sphere displaced radially, r(theta,phi) = 1 + sum_k c_k Y_k
+        over 8 random real spherical harmonics with l <= 6, the coefficients
+        scaled so that max |sum c_k Y_k| = amp (0.05) on a dense grid; points
+        uniform by area (direction rejection on the area element
+        r sqrt(r^2 + |grad_S r|^2)), normals the gradient of |x| - r(theta,phi).
 Each returns an (n, 6) float64 array [px, py, pz, nx, ny, nz].
 """
 from __future__ import annotations
@@ -167,3 +172,87 @@ def batch_sizes(k: int = 256, lo: float = 2e3, hi: float = 2e5, seed: int = 0) -
     """Cloud sizes round(exp(U[ln lo, ln hi]))."""
     rng = np.random.default_rng(seed)
     return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), size=k))).astype(np.int64)
+
+
+class SHDisplacement:
+    """r(theta, phi) = 1 + sum_k c_k Y_{l_k m_k}(theta, phi), real spherical
+    harmonics (theta polar, phi azimuth), with its angular derivatives."""
+
+    def __init__(self, k: int = 8, lmax: int = 6, amp: float = 0.05, seed: int = 0):
+        rng = np.random.default_rng(seed)
+        self.terms = []
+        for _ in range(k):
+            l = int(rng.integers(1, lmax + 1))
+            m = int(rng.integers(-l, l + 1))
+            self.terms.append((l, m, float(rng.normal())))
+        th, ph = np.meshgrid(np.linspace(0, np.pi, 721), np.linspace(-np.pi, np.pi, 1441), indexing="ij")
+        peak = np.abs(self.delta(th, ph)).max()
+        self.terms = [(l, m, c * amp / peak) for l, m, c in self.terms]
+
+    @staticmethod
+    def _real_sh(l, m, th, ph, grad=False):
+        """Real Y_lm and, with grad, (dY/dtheta, dY/dphi)."""
+        from scipy.special import sph_harm_y
+        if grad:
+            y, g = sph_harm_y(l, abs(m), th, ph, diff_n=1)
+            parts = (y, g[..., 0], g[..., 1])
+        else:
+            parts = (sph_harm_y(l, abs(m), th, ph),)
+        if m == 0:
+            out = tuple(q.real for q in parts)
+        else:
+            f = np.sqrt(2.0) * (-1) ** m
+            out = tuple(f * (q.real if m > 0 else q.imag) for q in parts)
+        return out if grad else out[0]
+
+    def delta(self, th, ph):
+        return sum(c * self._real_sh(l, m, th, ph) for l, m, c in self.terms)
+
+    def r(self, th, ph):
+        return 1.0 + self.delta(th, ph)
+
+    def r_and_grad(self, th, ph):
+        """(r, dr/dtheta, dr/dphi)."""
+        r = np.ones_like(th)
+        dth = np.zeros_like(th)
+        dph = np.zeros_like(th)
+        for l, m, c in self.terms:
+            y, gt, gp = self._real_sh(l, m, th, ph, grad=True)
+            r += c * y
+            dth += c * gt
+            dph += c * gp
+        return r, dth, dph
+
+
+def displaced_sphere(n: int, seed: int = 0, k: int = 8, lmax: int = 6, amp: float = 0.05) -> np.ndarray:
+    S = SHDisplacement(k, lmax, amp, seed)
+    rng = np.random.default_rng(seed + 1)
+
+    def density(r, dth, dph, th):
+        return r * np.sqrt(r * r + dth * dth + (dph / np.maximum(np.sin(th), 1e-12)) ** 2)
+
+    # bound of the area element from a grid (x1.25 margin against the grid's gaps)
+    gt, gp = np.meshgrid(np.linspace(1e-3, np.pi - 1e-3, 361), np.linspace(-np.pi, np.pi, 721), indexing="ij")
+    dmax = 1.25 * density(*S.r_and_grad(gt, gp), gt).max()
+    keep_th, keep_ph, keep_r, keep_dt, keep_dp = [], [], [], [], []
+    need = n
+    while need > 0:
+        u = rng.normal(size=(int(need * 1.5) + 1024, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        th = np.arccos(np.clip(u[:, 2], -1.0, 1.0))
+        ph = np.arctan2(u[:, 1], u[:, 0])
+        r, dth, dph = S.r_and_grad(th, ph)
+        keep = np.flatnonzero(rng.uniform(size=len(u)) * dmax < density(r, dth, dph, th))[:need]
+        for lst, a in zip((keep_th, keep_ph, keep_r, keep_dt, keep_dp), (th, ph, r, dth, dph)):
+            lst.append(a[keep])
+        need -= len(keep)
+    th, ph, r, dth, dph = (np.concatenate(a) for a in (keep_th, keep_ph, keep_r, keep_dt, keep_dp))
+    st, ct, sp_, cp = np.sin(th), np.cos(th), np.sin(ph), np.cos(ph)
+    radial = np.stack([st * cp, st * sp_, ct], axis=1)
+    p = r[:, None] * radial
+    # grad(|x| - r(theta, phi)) = x/|x| - (dr/dtheta e_theta + dr/dphi / sin(theta) e_phi) / |x|
+    e_th = np.stack([ct * cp, ct * sp_, -st], axis=1)
+    e_ph = np.stack([-sp_, cp, np.zeros_like(th)], axis=1)
+    g = radial - (dth[:, None] * e_th + (dph / np.maximum(st, 1e-12))[:, None] * e_ph) / r[:, None]
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    return np.ascontiguousarray(np.concatenate([p, g], axis=1))

@@ -37,3 +37,25 @@ def test_ellipsoid_union_points_lie_on_exactly_one_surface():
     pts = synthetic.ellipsoids(3000, seed=1)
     assert pts.shape == (3000, 6)
     assert np.allclose(np.linalg.norm(pts[:, 3:], axis=1), 1.0)
+
+
+def test_displaced_sphere_points_and_normals():
+    pts = synthetic.displaced_sphere(4000, seed=2)
+    S = synthetic.SHDisplacement(seed=2)
+    p, n = pts[:, :3], pts[:, 3:]
+    rad = np.linalg.norm(p, axis=1)
+    th = np.arccos(p[:, 2] / rad)
+    ph = np.arctan2(p[:, 1], p[:, 0])
+    assert np.abs(rad - S.r(th, ph)).max() < 1e-12  # on the surface
+    assert np.abs(S.delta(th, ph)).max() <= 0.05 + 1e-9  # amplitude bound
+    # the normal is orthogonal to the surface: a step along a tangent of the
+    # surface (from the parametrisation) is orthogonal to it to O(step^2)
+    eps = 1e-5
+    for dth, dph in ((eps, 0.0), (0.0, eps)):
+        t2, p2 = th + dth, ph + dph
+        q = S.r(t2, p2)[:, None] * np.stack([np.sin(t2) * np.cos(p2), np.sin(t2) * np.sin(p2), np.cos(t2)], axis=1)
+        tang = (q - p) / eps
+        keep = np.sin(th) > 0.05
+        cos = np.abs(np.sum(tang * n, axis=1)) / np.linalg.norm(tang, axis=1)
+        assert np.median(cos[keep]) < 1e-4
+    assert np.mean(np.sum(p * n, axis=1) > 0) == 1.0  # outward
