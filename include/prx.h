@@ -199,6 +199,12 @@ int prx_evaluate(prx_tree* t, const double* q, size_t m, double* out);
 int prx_evaluate_corners(prx_tree* t, double* out);
 /* iso value: mean of evaluate over the uploaded points (src/reconstructor.rs:56). */
 int prx_iso_value(prx_tree* t, double* iso);
+/* FEMTree::extract_surface (src/fem_tree.rs:576-636) of the current solution
+ * at `iso`, in unit-cube coordinates: the same vertices (to the reference's
+ * 1e-9 position quantisation), numbered in the reference's insertion order,
+ * and the same triangles in the same order.  The arrays are library-owned
+ * until prx_mesh_free. */
+int prx_extract_surface(prx_tree* t, double iso, prx_mesh* out);
 
 /* BBox + unit_cube_transform + transform_point/vector + normalize
  * (src/geometry.rs:349-402, :256-269, :49-56) as applied at

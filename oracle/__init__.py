@@ -99,6 +99,9 @@ def lib():
         L.orc_evaluate1.restype = ctypes.c_double
         L.orc_evaluate1.argtypes = [ctypes.c_uint32, _vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_evaluate.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64, _vp]
+        L.orc_extract_surface.restype = ctypes.c_int64
+        L.orc_extract_surface.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp]
+        L.orc_free.argtypes = [_vp]
         L.orc_iso.restype = ctypes.c_double
         L.orc_iso.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64]
         L.orc_solve_cascadic.argtypes = [ctypes.c_uint32, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_double,
@@ -331,6 +334,22 @@ def vjp_points(depth, points, pw, x, gF, eps=1e-13):
     lib().orc_vjp_points(depth, _p(g2i), _p(i2o), _p(pts), len(pts), float(pw), _p(np.ascontiguousarray(x)),
                          _p(np.ascontiguousarray(gF, dtype=np.float64)), float(eps), _p(lam), _p(gpos), _p(gnrm))
     return gpos, gnrm, lam
+
+
+def extract_surface(depth, g2i, i2o, sol, iso):
+    """FEMTree::extract_surface (src/fem_tree.rs:576-636) in unit-cube
+    coordinates: (vertices [nv,3], triangles [nt,3] int64)."""
+    V = ctypes.POINTER(ctypes.c_double)()
+    T = ctypes.POINTER(ctypes.c_int64)()
+    nt = ctypes.c_int64(0)
+    L = lib()
+    nv = L.orc_extract_surface(depth, _p(g2i), _p(np.ascontiguousarray(i2o, dtype=np.uint32)), _p(np.ascontiguousarray(sol)),
+                               float(iso), ctypes.byref(V), ctypes.byref(T), ctypes.byref(nt))
+    verts = np.ctypeslib.as_array(V, shape=(max(nv, 1) * 3,))[:3 * nv].reshape(-1, 3).copy()
+    tris = np.ctypeslib.as_array(T, shape=(max(nt.value, 1) * 3,))[:3 * nt.value].reshape(-1, 3).copy()
+    L.orc_free(ctypes.cast(V, ctypes.c_void_p))
+    L.orc_free(ctypes.cast(T, ctypes.c_void_p))
+    return verts, tris
 
 
 def reconstruct_field(points, depth=8, scale=1.1, cg_accuracy=1e-3, cg_iters=500, point_weight=0.0):

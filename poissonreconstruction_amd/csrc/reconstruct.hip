@@ -6,7 +6,7 @@
 #include <cstring>
 #include <vector>
 
-#include "kernels.hpp"
+#include "tree.hpp"
 
 namespace prx {
 
@@ -146,6 +146,43 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     rc = prx_iso_value(t, &iso);
     if (rc != PRX_OK) { prx_tree_destroy(t); return rc; }
     st.iso_value = iso;
+    // extract_surface, then vertices back to model space (src/reconstructor.rs:57-64)
+    std::vector<double> uv;
+    std::vector<uint64_t> tri;
+    try {
+        PRX_HIP(hipSetDevice(t->device));
+        prx::extract_surface(t, iso, uv, tri);
+    } catch (const prx::Error& e) {
+        prx::g_err = e.what();
+        prx_tree_destroy(t);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        prx::g_err = "host allocation failed";
+        prx_tree_destroy(t);
+        return PRX_ERR_OOM;
+    }
+    const size_t nv = uv.size() / 3, nt = tri.size() / 3;
+    if (out_mesh) {
+        out_mesh->vertices = (double*)std::malloc(sizeof(double) * std::max<size_t>(3 * nv, 1));
+        out_mesh->triangles = (uint64_t*)std::malloc(sizeof(uint64_t) * std::max<size_t>(3 * nt, 1));
+        if (!out_mesh->vertices || !out_mesh->triangles) {
+            prx_mesh_free(out_mesh);
+            prx_tree_destroy(t);
+            prx::g_err = "host allocation failed";
+            return PRX_ERR_OOM;
+        }
+        for (size_t i = 0; i < nv; ++i) {
+            const double x = uv[3 * i], y = uv[3 * i + 1], z = uv[3 * i + 2];
+            out_mesh->vertices[3 * i] = mi[0] * x + mi[4] * y + mi[8] * z + mi[12];
+            out_mesh->vertices[3 * i + 1] = mi[1] * x + mi[5] * y + mi[9] * z + mi[13];
+            out_mesh->vertices[3 * i + 2] = mi[2] * x + mi[6] * y + mi[10] * z + mi[14];
+        }
+        std::memcpy(out_mesh->triangles, tri.data(), sizeof(uint64_t) * 3 * nt);
+        out_mesh->n_vertices = nv;
+        out_mesh->n_triangles = nt;
+    }
+    st.mesh_vertices = nv;
+    st.mesh_triangles = nt;
     st.time_extract = prx::now_s() - t0;
     prx_tree_destroy(t);
 

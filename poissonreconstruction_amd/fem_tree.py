@@ -235,3 +235,17 @@ class FEMTree:
         v = ctypes.c_double(0.0)
         check(L.lib().prx_iso_value(self._h, ctypes.byref(v)))
         return v.value
+
+    def extract_surface(self, iso: float):
+        """FEMTree::extract_surface (src/fem_tree.rs:576-636) of the current
+        solution: (vertices (n,3) unit cube, triangles (m,3) uint64)."""
+        mesh = L.Mesh()
+        check(L.lib().prx_extract_surface(self._h, float(iso), ctypes.byref(mesh)))
+        try:
+            nv, nt = int(mesh.n_vertices), int(mesh.n_triangles)
+            verts = np.ctypeslib.as_array(mesh.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else np.zeros((0, 3))
+            tris = (np.ctypeslib.as_array(mesh.triangles, shape=(nt * 3,)).reshape(nt, 3).copy() if nt
+                    else np.zeros((0, 3), dtype=np.uint64))
+        finally:
+            L.lib().prx_mesh_free(ctypes.byref(mesh))
+        return verts, tris

@@ -1,0 +1,103 @@
+"""Marching cubes: the case table and the oracle's extract_surface
+(src/fem_tree.rs:576-636), pinned by what the geometry fixes."""
+import itertools
+import os
+
+import numpy as np
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def case_table():
+    src = open(os.path.join(HERE, "..", "poissonreconstruction_amd", "csrc", "mc_cases.h")).read()
+    body = src[src.index("kMcCase[256] = {") + len("kMcCase[256] = {"):src.index("};", src.index("kMcCase[256]"))]
+    cases = [s.strip().strip('"') for s in body.replace("\n", " ").split(",") if s.strip()]
+    assert len(cases) == 256
+    return [[int(c, 16) for c in s] for s in cases]
+
+
+CORNER = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+EDGE = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+
+
+def test_edges_join_adjacent_corners():
+    for a, b in EDGE:
+        assert sum(abs(p - q) for p, q in zip(CORNER[a], CORNER[b])) == 1
+    assert len({frozenset(e) for e in EDGE}) == 12
+
+
+def test_every_case_uses_exactly_its_cut_edges():
+    T = case_table()
+    for c in range(256):
+        cut = {e for e, (a, b) in enumerate(EDGE) if ((c >> a) & 1) != ((c >> b) & 1)}
+        used = set(T[c])
+        assert used == cut, c
+        assert len(T[c]) % 3 == 0 and len(T[c]) // 3 <= 5
+        for i in range(0, len(T[c]), 3):
+            assert len(set(T[c][i:i + 3])) == 3
+    # complementary cases cut the same edges
+    for c in range(256):
+        assert set(T[c]) == set(T[255 - c])
+
+
+def _closed_mesh_checks(V, F):
+    assert len(F) > 0 and len(V) > 0
+    assert F.min() >= 0 and F.max() < len(V)
+    assert np.all((F[:, 0] != F[:, 1]) & (F[:, 1] != F[:, 2]) & (F[:, 0] != F[:, 2]))
+    directed = {}
+    for t in F:
+        for i in range(3):
+            e = (int(t[i]), int(t[(i + 1) % 3]))
+            directed[e] = directed.get(e, 0) + 1
+    undirected = {}
+    for (a, b), k in directed.items():
+        assert k == 1, "a directed edge appears twice: inconsistent orientation"
+        undirected[frozenset((a, b))] = undirected.get(frozenset((a, b)), 0) + 1
+    assert all(k == 2 for k in undirected.values()), "open or non-manifold edge"
+    return len(V) - len(undirected) + len(F)
+
+
+def _field(depth, f):
+    g2i, i2o = oracle.tree(depth)
+    res = 1 << depth
+    # evaluate() reads node o at s = o + 1 (Q15): node o stands for the point (o + 1) h
+    p = (i2o.astype(np.float64) + 1.0) / res
+    return g2i, i2o, f(p)
+
+
+def test_sphere_mesh_is_closed_oriented_genus_zero():
+    c = np.array([0.47, 0.52, 0.5])
+    g2i, i2o, sol = _field(5, lambda p: np.linalg.norm(p - c, axis=1) - 0.3)
+    V, F = oracle.extract_surface(5, g2i, i2o, sol, 0.0)
+    assert _closed_mesh_checks(V, F) == 2
+    # inside (< iso) corners set the case bits: every triangle's normal (right hand) points the same way
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    outward = np.sum(n * (V[F].mean(axis=1) - c), axis=1)
+    assert np.all(outward > 0) or np.all(outward < 0)
+    assert np.abs(np.linalg.norm(V - c, axis=1) - 0.3).max() < 1.0 / 32
+
+
+def test_torus_mesh_is_closed_genus_one():
+    def f(p):
+        q = p - 0.5
+        return np.hypot(np.hypot(q[:, 0], q[:, 1]) - 0.25, q[:, 2]) - 0.1
+    g2i, i2o, sol = _field(5, f)
+    V, F = oracle.extract_surface(5, g2i, i2o, sol, 0.0)
+    assert _closed_mesh_checks(V, F) == 0
+
+
+def test_vertices_lie_on_cut_edges_at_the_linear_zero():
+    # a plane field: at interior corners evaluate() returns the node values
+    # exactly (F(k h) = f(k h) for 1 <= k < res; the clamped boundary corners
+    # are not linear, Q15), so vertices between interior corners lie on the plane
+    a = np.array([0.3, -0.5, 0.8])
+    g2i, i2o, sol = _field(4, lambda p: p @ a - 0.2)
+    V, F = oracle.extract_surface(4, g2i, i2o, sol, 0.0)
+    assert len(V) > 0
+    on_grid = np.isclose(V * 16, np.round(V * 16), atol=1e-9, rtol=0)
+    assert np.all(on_grid.sum(axis=1) >= 2)  # each vertex lies on a grid edge
+    inner = np.all((V >= 1 / 16 - 1e-12) & (V <= 15 / 16 + 1e-12), axis=1)
+    assert inner.sum() > 50
+    assert np.abs(V[inner] @ a - 0.2).max() < 1e-12
