@@ -76,3 +76,28 @@ def test_surface_of_a_larger_tree_is_closed(gpu):
     inner = np.all((V[u[:, 0]] > 2 * h) & (V[u[:, 0]] < 1 - 2 * h) & (V[u[:, 1]] > 2 * h) & (V[u[:, 1]] < 1 - 2 * h), axis=1)
     assert inner.sum() > 10000
     assert np.all(cnt[inner] == 2)
+
+
+def test_command_line_writes_the_reconstruct_mesh(gpu, tmp_path):
+    import subprocess
+    import sys
+    from poissonreconstruction_amd import ply
+    raw = synthetic.sphere(3000, seed=2)
+    src = tmp_path / "in.ply"
+    with open(src, "w") as fh:
+        fh.write(f"ply\nformat ascii 1.0\nelement vertex {len(raw)}\nproperty float x\nproperty float y\nproperty float z\n"
+                 "property float nx\nproperty float ny\nproperty float nz\nend_header\n")
+        for r in raw:
+            fh.write(" ".join(repr(float(v)) for v in r) + "\n")
+    out = tmp_path / "out.ply"
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-m", "poissonreconstruction_amd", "-i", str(src), "-o", str(out), "-d", "5",
+                        "--point-weight", "4", "-v"], capture_output=True, text=True, cwd=root)
+    assert r.returncode == 0, r.stderr
+    assert "Mesh:" in r.stderr and "Wrote" in r.stderr
+    mesh, st = pr.reconstruct(ply.read_oriented_points(str(src)), pr.ReconstructParams(depth=5, point_weight=4.0))
+    lines = open(out).read().split("\n")
+    nv, nt = int(lines[2].split()[-1]), int(lines[6].split()[-1])
+    assert (nv, nt) == (len(mesh.vertices), len(mesh.triangles))
+    V = np.array([[float(t) for t in l.split()] for l in lines[9:9 + nv]])
+    assert np.array_equal(V, mesh.vertices)
