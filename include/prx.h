@@ -212,6 +212,11 @@ int prx_extract_surface(prx_tree* t, double iso, prx_mesh* out);
  * the model->unit map and its inverse (Mat4::inverse, src/geometry.rs:274-316);
  * out receives the n transformed points.  Any output may be NULL. */
 int prx_unit_transform(const prx_point* points, size_t n, double scale, double* m, double* minv, prx_point* out);
+/* The same on the device (bounding-box reduction and point map in kernels,
+ * unfused arithmetic as the reference); bit-identical to prx_unit_transform.
+ * prx_reconstruct uses this path. */
+int prx_unit_transform_device(const prx_point* points, size_t n, double scale, int device, double* m, double* minv,
+                              prx_point* out);
 
 /* ------------------------------------------------------------ measurement
  * No counterpart in the reference (it has no benchmarks): device-side timing

@@ -135,6 +135,7 @@ SIGNATURES = {
     "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
     "prx_time_evaluate_corners": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "prx_extract_surface": (_i32, [_vp, _f64, _vp]),
+    "prx_unit_transform_device": (_i32, [_vp, _sz, _f64, _i32, _vp, _vp, _vp]),
     "prx_vjp_points": (_i32, [_vp, _vp, _vp, _vp, _i32, _f64, _vp]),
     "prx_batch_create": (_i32, [_i32, _vp]),
     "prx_batch_destroy": (None, [_vp]),

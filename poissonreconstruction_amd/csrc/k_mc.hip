@@ -216,7 +216,19 @@ static unsigned nb(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) 
 
 // Extract the iso surface of the tree's current solution: vertices [3 nv]
 // (unit cube) and triangles [3 nt] into host vectors.
-void extract_surface(prx_tree* t, double iso, std::vector<double>& verts, std::vector<uint64_t>& tris) {
+// transform_point (src/geometry.rs:256-261) of the vertices, unfused like the reference
+struct M16 { double v[16]; };
+__global__ void k_mc_to_model(double* pos, int64_t A, M16 M) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A) return;
+    const double* m = M.v;
+    const double x = pos[3 * r], y = pos[3 * r + 1], z = pos[3 * r + 2];
+    pos[3 * r] = m[0] * x + m[4] * y + m[8] * z + m[12];
+    pos[3 * r + 1] = m[1] * x + m[5] * y + m[9] * z + m[13];
+    pos[3 * r + 2] = m[2] * x + m[6] * y + m[10] * z + m[14];
+}
+
+void extract_surface(prx_tree* t, double iso, std::vector<double>& verts, std::vector<uint64_t>& tris, const double* to_model) {
     const hipStream_t s = t->stream;
     upload_tables(s);
     McGrid G;
@@ -306,6 +318,12 @@ void extract_surface(prx_tree* t, double iso, std::vector<double>& verts, std::v
         PRX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, b3, key.p, skey.p, val.p, sval.p, (int)A, 0, kbits, s));
         hipLaunchKernelGGL(k_mc_vpos, dim3(nb(A)), dim3(256), 0, s, G, skey.p, sval.p, A, vid.p, pos.p);
         PRX_LAUNCH_CHECK();
+        if (to_model) {
+            M16 M;
+            std::memcpy(M.v, to_model, sizeof M.v);
+            hipLaunchKernelGGL(k_mc_to_model, dim3(nb(A)), dim3(256), 0, s, pos.p, A, M);
+            PRX_LAUNCH_CHECK();
+        }
     }
     // triangles: per active cell in cell order
     DArr<uint32_t> ntri, toff;

@@ -8,6 +8,10 @@
 
 #include "tree.hpp"
 
+// The reference's arithmetic is unfused (Rust does not contract a*b+c); the
+// device transform kernels below must round the same way.
+#pragma clang fp contract(off)
+
 namespace prx {
 
 template <typename T>
@@ -20,17 +24,22 @@ struct DBuf {
 };
 
 // f64::min / f64::max ignore a NaN operand (src/geometry.rs:357-362).
-static inline double rs_min(double a, double b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
-static inline double rs_max(double a, double b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
+__host__ __device__ inline double rs_min(double a, double b) { return a != a ? b : (b != b ? a : (a < b ? a : b)); }
+__host__ __device__ inline double rs_max(double a, double b) { return a != a ? b : (b != b ? a : (a > b ? a : b)); }
 
 // BBox + unit_cube_transform + inverse (src/geometry.rs:349-402, :274-316);
 // m is column-major m[col*4+row] like Mat4.
+static void transform_from_bbox(const double mn[3], const double mx[3], double scale, double m[16], double mi[16]);
 void host_unit_transform(const prx_point* p, size_t n, double scale, double m[16], double mi[16]) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (size_t i = 0; i < n; ++i) {
         const double q[3] = {p[i].px, p[i].py, p[i].pz};
         for (int k = 0; k < 3; ++k) { mn[k] = rs_min(mn[k], q[k]); mx[k] = rs_max(mx[k], q[k]); }
     }
+    transform_from_bbox(mn, mx, scale, m, mi);
+}
+// unit_cube_transform and Mat4::inverse (src/geometry.rs:384-402, :274-316)
+static void transform_from_bbox(const double mn[3], const double mx[3], double scale, double m[16], double mi[16]) {
     double c[3], sz[3];
     for (int k = 0; k < 3; ++k) { c[k] = (mn[k] + mx[k]) * 0.5; sz[k] = mx[k] - mn[k]; }
     const double md = rs_max(rs_max(sz[0], sz[1]), sz[2]);
@@ -53,6 +62,75 @@ void host_unit_transform(const prx_point* p, size_t n, double scale, double m[16
     mi[13] = -(di * m[12] + ei * m[13] + fi * m[14]);
     mi[14] = -(gi * m[12] + hi * m[13] + iii * m[14]);
     mi[15] = 1.0;
+}
+
+// ------------------------------------------------------------------ the same on the device
+// BBox::extend over all points (src/geometry.rs:356-363) as a block reduction;
+// min/max ignoring NaN are associative, so the order does not matter.
+__global__ __launch_bounds__(256) void k_bbox(const prx_point* __restrict__ p, int64_t n, double* part) {
+    __shared__ double sm[6][256];
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double q[3] = {p[i].px, p[i].py, p[i].pz};
+        for (int k = 0; k < 3; ++k) { mn[k] = rs_min(mn[k], q[k]); mx[k] = rs_max(mx[k], q[k]); }
+    }
+    for (int k = 0; k < 3; ++k) { sm[k][threadIdx.x] = mn[k]; sm[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) {
+                sm[k][threadIdx.x] = rs_min(sm[k][threadIdx.x], sm[k][threadIdx.x + w]);
+                sm[3 + k][threadIdx.x] = rs_max(sm[3 + k][threadIdx.x], sm[3 + k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[6 * blockIdx.x + k] = sm[k][0];
+}
+struct Mat16 { double v[16]; };
+// transform_point / transform_vector + normalize into SoA position/normal arrays
+__global__ void k_transform_points(const prx_point* __restrict__ in, int64_t n, Mat16 M, double* pos, double* nrm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* m = M.v;
+    const prx_point p = in[i];
+    pos[i] = m[0] * p.px + m[4] * p.py + m[8] * p.pz + m[12];
+    pos[n + i] = m[1] * p.px + m[5] * p.py + m[9] * p.pz + m[13];
+    pos[2 * n + i] = m[2] * p.px + m[6] * p.py + m[10] * p.pz + m[14];
+    double vx = m[0] * p.nx + m[4] * p.ny + m[8] * p.nz;
+    double vy = m[1] * p.nx + m[5] * p.ny + m[9] * p.nz;
+    double vz = m[2] * p.nx + m[6] * p.ny + m[10] * p.nz;
+    const double nn = sqrt(vx * vx + vy * vy + vz * vz);
+    if (nn > 0.0) { vx = vx / nn; vy = vy / nn; vz = vz / nn; }
+    else { vx = vy = vz = 0.0; }
+    nrm[i] = vx;
+    nrm[n + i] = vy;
+    nrm[2 * n + i] = vz;
+}
+// the model -> unit map of the points on the device: uploads the AoS points
+// once, reduces the bounding box, builds m and its inverse on the host from
+// the six numbers, transforms into pos/nrm (device SoA [3n] each)
+void device_unit_transform(const prx_point* host_pts, size_t n, double scale, hipStream_t s, double m[16], double mi[16],
+                           double* pos, double* nrm) {
+    DBuf<prx_point> d((int64_t)n);
+    PRX_HIP(hipMemcpyAsync(d.p, host_pts, sizeof(prx_point) * n, hipMemcpyHostToDevice, s));
+    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)n + 255) / 256));
+    DBuf<double> part(6 * nb);
+    hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(256), 0, s, d.p, (int64_t)n, part.p);
+    PRX_LAUNCH_CHECK();
+    std::vector<double> ph(6 * (size_t)nb);
+    PRX_HIP(hipMemcpyAsync(ph.data(), part.p, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipStreamSynchronize(s));
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 3; ++k) { mn[k] = rs_min(mn[k], ph[6 * b + k]); mx[k] = rs_max(mx[k], ph[6 * b + 3 + k]); }
+    transform_from_bbox(mn, mx, scale, m, mi);
+    Mat16 M;
+    std::memcpy(M.v, m, sizeof M.v);
+    hipLaunchKernelGGL(k_transform_points, dim3((unsigned)std::max<int64_t>(1, ((int64_t)n + 255) / 256)), dim3(256), 0, s, d.p,
+                       (int64_t)n, M, pos, nrm);
+    PRX_LAUNCH_CHECK();
+    PRX_HIP(hipStreamSynchronize(s));
 }
 
 // transform_point / transform_vector + normalize (src/geometry.rs:256-269,
@@ -101,20 +179,36 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
             !std::isfinite(q.nz))
             return PRX_ERR_NONFINITE;
     }
+    if (n >= ((size_t)1 << 29)) { prx::g_err = "at most 2^29 points"; return PRX_ERR_INVALID; }
     const double t_total = prx::now_s();
     double t0 = prx::now_s();
     double m[16], mi[16];
-    prx::host_unit_transform(points, n, p.scale, m, mi);
-    st.time_transform = prx::now_s() - t0;
-    std::vector<prx_point> unit(n);
-    prx::host_apply_transform(points, n, m, unit.data());
-
-    t0 = prx::now_s();
     prx_tree* t = nullptr;
     int rc = prx_tree_create(p.depth, p.device, &t);
     if (rc != PRX_OK) return rc;
-    rc = prx_splat_normal_field(t, unit.data(), n);
-    if (rc != PRX_OK) { prx_tree_destroy(t); return rc; }
+    // bounding box, unit-cube map and the transformed points on the device
+    // (src/reconstructor.rs:31-40), written straight into the tree's arrays
+    try {
+        PRX_HIP(hipSetDevice(t->device));
+        t->P = (int64_t)n;
+        t->pos.ensure(3 * (int64_t)n);
+        t->nrm.ensure(3 * (int64_t)n);
+        prx::device_unit_transform(points, n, p.scale, t->stream, m, mi, t->pos.p, t->nrm.p);
+    } catch (const prx::Error& e) {
+        prx::g_err = e.what();
+        prx_tree_destroy(t);
+        return e.code;
+    }
+    st.time_transform = prx::now_s() - t0;
+
+    t0 = prx::now_s();
+    try {
+        prx::splat(t);
+    } catch (const prx::Error& e) {
+        prx::g_err = e.what();
+        prx_tree_destroy(t);
+        return e.code;
+    }
     st.fem_nodes = (uint64_t)prx_tree_node_count(t);
     st.time_tree = prx::now_s() - t0;
 
@@ -146,12 +240,12 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     rc = prx_iso_value(t, &iso);
     if (rc != PRX_OK) { prx_tree_destroy(t); return rc; }
     st.iso_value = iso;
-    // extract_surface, then vertices back to model space (src/reconstructor.rs:57-64)
+    // extract_surface, vertices mapped back to model space on the device (src/reconstructor.rs:57-64)
     std::vector<double> uv;
     std::vector<uint64_t> tri;
     try {
         PRX_HIP(hipSetDevice(t->device));
-        prx::extract_surface(t, iso, uv, tri);
+        prx::extract_surface(t, iso, uv, tri, mi);
     } catch (const prx::Error& e) {
         prx::g_err = e.what();
         prx_tree_destroy(t);
@@ -171,12 +265,7 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
             prx::g_err = "host allocation failed";
             return PRX_ERR_OOM;
         }
-        for (size_t i = 0; i < nv; ++i) {
-            const double x = uv[3 * i], y = uv[3 * i + 1], z = uv[3 * i + 2];
-            out_mesh->vertices[3 * i] = mi[0] * x + mi[4] * y + mi[8] * z + mi[12];
-            out_mesh->vertices[3 * i + 1] = mi[1] * x + mi[5] * y + mi[9] * z + mi[13];
-            out_mesh->vertices[3 * i + 2] = mi[2] * x + mi[6] * y + mi[10] * z + mi[14];
-        }
+        std::memcpy(out_mesh->vertices, uv.data(), sizeof(double) * 3 * nv);
         std::memcpy(out_mesh->triangles, tri.data(), sizeof(uint64_t) * 3 * nt);
         out_mesh->n_vertices = nv;
         out_mesh->n_triangles = nt;
@@ -195,6 +284,39 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     st.time_total = prx::now_s() - t_total;
     if (out_stats) *out_stats = st;
     return PRX_OK;
+}
+
+int prx_unit_transform_device(const prx_point* points, size_t n, double scale, int device, double* m, double* minv,
+                              prx_point* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(points || n == 0, PRX_ERR_INVALID, "null pointer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw prx::Error(PRX_ERR_DEVICE, "no HIP device available");
+    PRX_REQUIRE(device >= 0 && device < ndev, PRX_ERR_INVALID, "device ordinal out of range");
+    PRX_HIP(hipSetDevice(device));
+    hipStream_t s;
+    PRX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    double mm[16], mi[16];
+    try {
+        prx::DBuf<double> pos(3 * (int64_t)n), nrm(3 * (int64_t)n);
+        if (n > 0) prx::device_unit_transform(points, n, scale, s, mm, mi, pos.p, nrm.p);
+        else prx::host_unit_transform(points, 0, scale, mm, mi);
+        if (out && n > 0) {
+            std::vector<double> hp(3 * n), hn(3 * n);
+            PRX_HIP(hipMemcpyAsync(hp.data(), pos.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+            PRX_HIP(hipMemcpyAsync(hn.data(), nrm.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+            PRX_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i)
+                out[i] = prx_point{hp[i], hp[n + i], hp[2 * n + i], hn[i], hn[n + i], hn[2 * n + i]};
+        }
+    } catch (...) {
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    PRX_HIP(hipStreamDestroy(s));
+    if (m) std::memcpy(m, mm, sizeof mm);
+    if (minv) std::memcpy(minv, mi, sizeof mi);
+    PRX_API_END
 }
 
 void prx_mesh_free(prx_mesh* mesh) {

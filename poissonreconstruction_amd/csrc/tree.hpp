@@ -174,8 +174,10 @@ void upload_points(prx_tree* t, const prx_point* pts, size_t n);
 void splat(prx_tree* t);
 void assemble(prx_tree* t, double pw);
 bool finite_points(const prx_point* p, size_t n);
-// marching cubes of the current solution: vertices [3 nv] (unit cube), triangles [3 nt]
-void extract_surface(prx_tree* t, double iso, std::vector<double>& verts, std::vector<uint64_t>& tris);
+// marching cubes of the current solution: vertices [3 nv] (unit cube, or mapped
+// by the column-major 4x4 `to_model` when given), triangles [3 nt]
+void extract_surface(prx_tree* t, double iso, std::vector<double>& verts, std::vector<uint64_t>& tris,
+                     const double* to_model = nullptr);
 
 }  // namespace prx
 
