@@ -333,15 +333,16 @@ static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
     }
     launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_DIR_APPLY, io, B->tile, s);
     launch_bfin(tab, B->partial.p, ulog_tile(B->tile), FIN_ALPHA, 0, s);
-    int ulog;
+    int ulog, fop = FIN_BETA;
     if (refresh > 0 && (k + 1) % (uint64_t)refresh == 0) {
-        launch_baxpy_alpha<double>(tab, L.x.p, dnew, L.N, s);
+        launch_baxpy_alpha<double>(tab, L.x.p, dold, dnew, L.N, s);
         resid_pass(B, L, tab);
         ulog = ulog_tile(B->tile);
+        fop = FIN_BETA_FULL;
     } else {
-        ulog = launch_bcg_update<double>(tab, L.x.p, L.r.p, dnew, L.q.p, L.N, B->partial.p, s);
+        ulog = launch_bcg_update<double>(tab, L.x.p, L.r.p, dold, dnew, L.q.p, L.N, B->partial.p, s);
     }
-    launch_bfin(tab, B->partial.p, ulog, FIN_BETA, 0, s);
+    launch_bfin(tab, B->partial.p, ulog, fop, (int)((k + 1) & 1), s);
 }
 
 // kind 0: `chunk` CG iterations starting at an even index; kind 1: GS sweeps
@@ -402,6 +403,7 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
         h.eps = eps;
         h.max_iters = (int64_t)max_iters;
         h.bnorm2 = bb[k];
+        h.lazy = 1;
         B->st_host[k] = h;
     }
     write_states(B);
@@ -425,6 +427,10 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
             launch_graph(B, g);
             read_states(B);
         }
+        // complete x: add the pending lazy terms
+        launch_bflush_x<double>(all, L.x.p, L.dd.p, L.dd.p + L.N, L.N, s);
+        for (int k = 0; k < K; ++k) B->st_host[k].xpend = 0;
+        write_states(B);
         if (mode != PRX_RESID_TRUE) break;
         // confirm on the true residual; continue the unconfirmed instances
         bool any = false;
@@ -459,9 +465,8 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
             h.done = 0;
             h.status = 0;
             // the graph restarts at an even iteration index and reads the
-            // direction from dd[0]; an instance that stopped after an odd
-            // number of iterations left its last direction in dd[1]
-            if (h.it & 1) {
+            // direction from dd[0]; move it there if it is in dd[1]
+            if (h.dir_buf == 1) {
                 const int64_t nk = (int64_t)(L.off_h[k + 1] - L.off_h[k]);
                 PRX_HIP(hipMemcpyAsync(L.dd.p + L.off_h[k], L.dd.p + L.N + L.off_h[k], sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
             }

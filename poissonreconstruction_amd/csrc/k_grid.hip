@@ -128,6 +128,16 @@ __device__ __forceinline__ void cg_upd(T& x, T& r, T d, T q, double a, double& a
     r = (T)((double)r - a * (double)q);
     acc += (double)r * (double)r;
 }
+template <typename T>
+__device__ __forceinline__ void cg_upd_r(T& r, T q, double a, double& acc) {
+    r = (T)((double)r - a * (double)q);
+    acc += (double)r * (double)r;
+}
+// the two pending terms of a lazy pair: x + alpha_pend * dprev + alpha * d
+template <typename T>
+__device__ __forceinline__ T x_pair(T x, T dp, T d, double ap, double a) {
+    return (T)(((double)x + ap * (double)dp) + a * (double)d);
+}
 
 // The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
 __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
@@ -141,6 +151,7 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
             st->iters = 0;
             st->status = 0;
             st->refresh_due = 0;
+            st->xpend = 0;
             st->thr = ((op == FIN_INIT_REF ? sum : st->bnorm2) * st->eps) * st->eps;
             st->done = (sum == 0.0 || sum <= st->thr) ? 1 : 0;
             break;
@@ -153,7 +164,16 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
                 st->alpha = st->delta / sum;
             }
             break;
-        case FIN_BETA: {
+        case FIN_BETA:
+        case FIN_BETA_FULL: {
+            // the vector update of this iteration has run: account for the lazy x term
+            st->dir_buf = slot;
+            if (op == FIN_BETA_FULL || !st->lazy || st->xpend) {
+                st->xpend = 0;
+            } else {
+                st->xpend = 1;
+                st->alpha_pend = st->alpha;
+            }
             double dold = st->delta;
             st->delta_prev = dold;
             st->delta = sum;
@@ -232,70 +252,121 @@ void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, i
     PRX_LAUNCH_CHECK();
 }
 
-// x += alpha_k d ; r -= alpha_k q ; partial = r.r per block.  A block covers
-// 1024 vectors (4096 float or 2048 double nodes) inside one 4096-node brick,
-// so it belongs to one instance.
+// r -= alpha_k q ; partial = r.r per block ; x as in k_cg_update (eager, or
+// lazy in pairs).  A block covers 1024 vectors (4096 float or 2048 double
+// nodes) inside one 4096-node brick, so it belongs to one instance.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bcg_update(BatchTab tab, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
-                                                    const T* __restrict__ q, double* partial) {
+__global__ __launch_bounds__(256) void k_bcg_update(BatchTab tab, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dp,
+                                                    const T* __restrict__ d, const T* __restrict__ q, double* partial) {
     __shared__ double ws[4];
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
     const uint32_t k = tab.brick_inst[((uint64_t)blockIdx.x * 1024 * VEC) >> 12];
     if (!b_runs(tab, k)) return;
-    const double a = tab.st[k].alpha;
+    const CgState* st = tab.st + k;
+    const double a = st->alpha, ap = st->alpha_pend;
+    const int mode = !st->lazy ? 0 : (st->xpend ? 2 : 1);
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int64_t j = ((int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x) * VEC;
-        NV xv = ld_nt(x + j), rv = ld_nt(r + j);
-        const NV dv = ld_nt(d + j), qv = ld_nt(q + j);
+        NV rv = ld_nt(r + j);
+        const NV qv = ld_nt(q + j);
 #pragma unroll
         for (int e = 0; e < VEC; ++e) {
-            T xe = xv[e], re = rv[e];
-            cg_upd<T>(xe, re, dv[e], qv[e], a, acc);
-            xv[e] = xe;
+            T re = rv[e];
+            cg_upd_r<T>(re, qv[e], a, acc);
             rv[e] = re;
         }
-        st_nt(x + j, xv);
         st_nt(r + j, rv);
+        if (mode != 1) {
+            NV xv = ld_nt(x + j);
+            const NV dv = ld_nt(d + j);
+            if (mode == 0) {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) xv[e] = (T)((double)xv[e] + a * (double)dv[e]);
+            } else {
+                const NV pv = ld_nt(dp + j);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) xv[e] = x_pair<T>(xv[e], pv[e], dv[e], ap, a);
+            }
+            st_nt(x + j, xv);
+        }
     }
     const double sm = block_sum_256(acc, ws);
     if (threadIdx.x == 0) partial[blockIdx.x] = sm;
 }
 template <typename T>
-int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* d, const T* q, int64_t Ntot, double* partial, hipStream_t s) {
+int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* dp, const T* d, const T* q, int64_t Ntot, double* partial,
+                      hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
     const int64_t nb = Ntot / (1024 * VEC);
-    hipLaunchKernelGGL(k_bcg_update<T>, dim3((unsigned)nb), dim3(256), 0, s, tab, x, r, d, q, partial);
+    hipLaunchKernelGGL(k_bcg_update<T>, dim3((unsigned)nb), dim3(256), 0, s, tab, x, r, dp, d, q, partial);
     PRX_LAUNCH_CHECK();
     return VEC == 2 ? 11 : 12;
 }
-template int launch_bcg_update<double>(const BatchTab&, double*, double*, const double*, const double*, int64_t, double*, hipStream_t);
-template int launch_bcg_update<float>(const BatchTab&, float*, float*, const float*, const float*, int64_t, double*, hipStream_t);
+template int launch_bcg_update<double>(const BatchTab&, double*, double*, const double*, const double*, const double*, int64_t, double*,
+                                       hipStream_t);
+template int launch_bcg_update<float>(const BatchTab&, float*, float*, const float*, const float*, const float*, int64_t, double*,
+                                      hipStream_t);
 
+// refresh iteration: x brought fully up to date (the pending lazy term too)
 template <typename T>
-__global__ __launch_bounds__(256) void k_baxpy_alpha(BatchTab tab, T* __restrict__ x, const T* __restrict__ d) {
+__global__ __launch_bounds__(256) void k_baxpy_alpha(BatchTab tab, T* __restrict__ x, const T* __restrict__ dp, const T* __restrict__ d) {
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
     const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
     const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
     if (!b_runs(tab, k)) return;
-    const double a = tab.st[k].alpha;
+    const CgState* st = tab.st + k;
+    const double a = st->alpha, ap = st->alpha_pend;
+    const bool pair = st->lazy && st->xpend;
     const NV xv = ld_v(x + j), dv = ld_v(d + j);
     NV xo;
+    if (pair) {
+        const NV pv = ld_v(dp + j);
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
+        for (int e = 0; e < VEC; ++e) xo[e] = x_pair<T>(xv[e], pv[e], dv[e], ap, a);
+    } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) xo[e] = (T)((double)xv[e] + a * (double)dv[e]);
+    }
     *(NV*)(x + j) = xo;
 }
 template <typename T>
-void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* d, int64_t Ntot, hipStream_t s) {
+void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* dp, const T* d, int64_t Ntot, hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
-    hipLaunchKernelGGL(k_baxpy_alpha<T>, dim3((unsigned)(Ntot / (256 * VEC))), dim3(256), 0, s, tab, x, d);
+    hipLaunchKernelGGL(k_baxpy_alpha<T>, dim3((unsigned)(Ntot / (256 * VEC))), dim3(256), 0, s, tab, x, dp, d);
     PRX_LAUNCH_CHECK();
 }
-template void launch_baxpy_alpha<double>(const BatchTab&, double*, const double*, int64_t, hipStream_t);
-template void launch_baxpy_alpha<float>(const BatchTab&, float*, const float*, int64_t, hipStream_t);
+template void launch_baxpy_alpha<double>(const BatchTab&, double*, const double*, const double*, int64_t, hipStream_t);
+template void launch_baxpy_alpha<float>(const BatchTab&, float*, const float*, const float*, int64_t, hipStream_t);
+
+// per instance: x += alpha_pend * d{dir_buf} where a lazy term is pending
+template <typename T>
+__global__ __launch_bounds__(256) void k_bflush_x(BatchTab tab, T* __restrict__ x, const T* __restrict__ d0, const T* __restrict__ d1) {
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
+    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
+    if (!b_runs(tab, k)) return;
+    const CgState* st = tab.st + k;
+    if (!(st->lazy && st->xpend)) return;
+    const double ap = st->alpha_pend;
+    const NV xv = ld_v(x + j), dv = ld_v((st->dir_buf ? d1 : d0) + j);
+    NV xo;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xo[e] = (T)((double)xv[e] + ap * (double)dv[e]);
+    *(NV*)(x + j) = xo;
+}
+template <typename T>
+void launch_bflush_x(const BatchTab& tab, T* x, const T* d0, const T* d1, int64_t Ntot, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    hipLaunchKernelGGL(k_bflush_x<T>, dim3((unsigned)(Ntot / (256 * VEC))), dim3(256), 0, s, tab, x, d0, d1);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bflush_x<double>(const BatchTab&, double*, const double*, const double*, int64_t, hipStream_t);
+template void launch_bflush_x<float>(const BatchTab&, float*, const float*, const float*, int64_t, hipStream_t);
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bdot(BatchTab tab, const T* __restrict__ a, const T* __restrict__ b, double* partial) {
@@ -327,13 +398,16 @@ template int launch_bdot<float>(const BatchTab&, const float*, const float*, int
 // quarter coalesced), streamed with non-temporal loads and stores; one
 // partial per block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ d,
-                                                   const T* __restrict__ q, const CgState* st, int64_t n, double* partial) {
+__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dp,
+                                                   const T* __restrict__ d, const T* __restrict__ q, const CgState* st, int64_t n,
+                                                   double* partial) {
     __shared__ double ws[4];
     if (st->done) return;
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
-    const double a = st->alpha;
+    const double a = st->alpha, ap = st->alpha_pend;
+    // 0: x += a d every iteration; 1: lazy, x untouched; 2: lazy, both pending terms
+    const int mode = !st->lazy ? 0 : (st->xpend ? 2 : 1);
     double acc = 0.0;
     const int64_t nv = n / VEC;
 #pragma unroll
@@ -341,57 +415,97 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, T* __restr
         const int64_t i = (int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x;
         if (i < nv) {
             const int64_t j = i * VEC;
-            NV xv = ld_nt(x + j), rv = ld_nt(r + j);
-            const NV dv = ld_nt(d + j), qv = ld_nt(q + j);
+            NV rv = ld_nt(r + j);
+            const NV qv = ld_nt(q + j);
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                T xe = xv[e], re = rv[e];
-                cg_upd<T>(xe, re, dv[e], qv[e], a, acc);
-                xv[e] = xe;
+                T re = rv[e];
+                cg_upd_r<T>(re, qv[e], a, acc);
                 rv[e] = re;
             }
-            st_nt(x + j, xv);
             st_nt(r + j, rv);
+            if (mode != 1) {
+                NV xv = ld_nt(x + j);
+                const NV dv = ld_nt(d + j);
+                if (mode == 0) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) xv[e] = (T)((double)xv[e] + a * (double)dv[e]);
+                } else {
+                    const NV pv = ld_nt(dp + j);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) xv[e] = x_pair<T>(xv[e], pv[e], dv[e], ap, a);
+                }
+                st_nt(x + j, xv);
+            }
         }
     }
     if (blockIdx.x == gridDim.x - 1) {  // tail (n not a multiple of the vector width)
         const int64_t j = nv * VEC + threadIdx.x;
         if (j < n) {
-            T xs = x[j], rs = r[j];
-            cg_upd<T>(xs, rs, d[j], q[j], a, acc);
-            x[j] = xs;
+            T rs = r[j];
+            cg_upd_r<T>(rs, q[j], a, acc);
             r[j] = rs;
+            if (mode == 0) x[j] = (T)((double)x[j] + a * (double)d[j]);
+            else if (mode == 2) x[j] = x_pair<T>(x[j], dp[j], d[j], ap, a);
         }
     }
-    double s = block_sum_256(acc, ws);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    double sm = block_sum_256(acc, ws);
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm;
 }
 template <typename T>
-void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s) {
+void launch_cg_update(T* x, T* r, const T* dp, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk,
+                      hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
     const int64_t nv = n / VEC;
     const int nb = (int)std::max<int64_t>(1, (nv + 1023) / 1024);
-    hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, s, x, r, d, q, st, n, partial);
+    hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(256), 0, s, x, r, dp, d, q, st, n, partial);
     PRX_LAUNCH_CHECK();
     *nblk = nb;
 }
-template void launch_cg_update<double>(double*, double*, const double*, const double*, const CgState*, int64_t, double*, int*, hipStream_t);
-template void launch_cg_update<float>(float*, float*, const float*, const float*, const CgState*, int64_t, double*, int*, hipStream_t);
+template void launch_cg_update<double>(double*, double*, const double*, const double*, const double*, const CgState*, int64_t, double*,
+                                       int*, hipStream_t);
+template void launch_cg_update<float>(float*, float*, const float*, const float*, const float*, const CgState*, int64_t, double*, int*,
+                                      hipStream_t);
 
+// refresh iteration: x brought fully up to date (the pending lazy term too)
 template <typename T>
-__global__ void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ d, const CgState* st, int64_t n) {
+__global__ void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ dp, const T* __restrict__ d, const CgState* st, int64_t n) {
     if (st->done) return;
-    const double a = st->alpha;
+    const double a = st->alpha, ap = st->alpha_pend;
+    const bool pair = st->lazy && st->xpend;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = (T)((double)x[i] + a * (double)d[i]);
+        x[i] = pair ? x_pair<T>(x[i], dp[i], d[i], ap, a) : (T)((double)x[i] + a * (double)d[i]);
 }
 template <typename T>
-void launch_axpy_alpha(T* x, const T* d, const CgState* st, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, x, d, st, n);
+void launch_axpy_alpha(T* x, const T* dp, const T* d, const CgState* st, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy_alpha<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, x, dp, d, st, n);
     PRX_LAUNCH_CHECK();
 }
-template void launch_axpy_alpha<double>(double*, const double*, const CgState*, int64_t, hipStream_t);
-template void launch_axpy_alpha<float>(float*, const float*, const CgState*, int64_t, hipStream_t);
+template void launch_axpy_alpha<double>(double*, const double*, const double*, const CgState*, int64_t, hipStream_t);
+template void launch_axpy_alpha<float>(float*, const float*, const float*, const CgState*, int64_t, hipStream_t);
+
+template <typename T>
+__global__ void k_flush_x(T* __restrict__ x, const T* __restrict__ d0, const T* __restrict__ d1, const CgState* st, int64_t n) {
+    if (!(st->lazy && st->xpend)) return;
+    const double ap = st->alpha_pend;
+    const T* d = st->dir_buf ? d1 : d0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] + ap * (double)d[i]);
+}
+template <typename T>
+void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_flush_x<T>, dim3(nblocks_for(n, 4, 256, 65536)), dim3(256), 0, s, x, d0, d1, st, n);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_flush_x<double>(double*, const double*, const double*, const CgState*, int64_t, hipStream_t);
+template void launch_flush_x<float>(float*, const float*, const float*, const CgState*, int64_t, hipStream_t);
+__global__ void k_clear_xpend(CgState* st) {
+    if (threadIdx.x == 0) st->xpend = 0;
+}
+void launch_clear_xpend(CgState* st, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_xpend, dim3(1), dim3(64), 0, s, st);
+    PRX_LAUNCH_CHECK();
+}
 
 // d = r + beta d   (src/solvers.rs:64)
 template <typename T>

@@ -423,15 +423,55 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     }
     __syncthreads();
 
-    // ---- 5. screening: touched nodes of this brick, fixed summation order
+    // ---- 5. screening: touched nodes of this brick.  The entries (one per
+    // point and corner, grouped by node) are read in chunks of 4*NT with
+    // independent loads per thread, their products phi*s_p staged in the LDS
+    // beyond the output staging, and each node then adds its own entries in
+    // ascending order: the sum is deterministic and no thread waits on a
+    // chain of dependent global loads.
     if (L.has_screen) {
         const uint32_t t0 = L.brick[bid], t1 = L.brick[bid + 1];
         if (t0 < t1) {
-            for (uint32_t t = t0 + tid; t < t1; t += NT) {
-                const uint32_t loc = (uint32_t)(L.node[t] - base);
-                T acc = 0;
-                for (uint32_t j = L.off[t]; j < L.off[t + 1]; ++j) acc += L.phi[j] * io.s[L.sp[j]];
-                tile[stage_at<B>(loc)] += acc;
+            constexpr int CH = 4 * NT;
+            static_assert(E3 - B3 >= CH, "the spare LDS holds one chunk");
+            T* prod = tile + B3;
+            for (uint32_t g0 = t0; g0 < t1; g0 += CH) {
+                const uint32_t g1 = min(t1, g0 + (uint32_t)CH);
+                uint32_t ob[4], oe[4];
+                T acc[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t t = g0 + k * NT + tid;
+                    ob[k] = t < g1 ? L.off[t] : 0u;
+                    oe[k] = t < g1 ? L.off[t + 1] : 0u;
+                    acc[k] = T(0);
+                }
+                const uint32_t e0 = L.off[g0], e1 = L.off[g1];
+                for (uint32_t c0 = e0; c0 < e1; c0 += CH) {
+                    uint32_t spv[4];
+                    T phv[4], pv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t j = c0 + u * NT + tid;
+                        spv[u] = j < e1 ? L.sp[j] : 0u;
+                        phv[u] = j < e1 ? L.phi[j] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) pv[u] = c0 + u * NT + tid < e1 ? phv[u] * io.s[spv[u]] : T(0);
+                    __syncthreads();  // the previous chunk has been read
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) prod[u * NT + tid] = pv[u];
+                    __syncthreads();
+                    const uint32_t c1 = min(e1, c0 + (uint32_t)CH);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        for (uint32_t j = max(ob[k], c0); j < min(oe[k], c1); ++j) acc[k] += prod[j - c0];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t t = g0 + k * NT + tid;
+                    if (t < g1) tile[stage_at<B>((uint32_t)(L.node[t] - base))] += acc[k];
+                }
             }
             __syncthreads();
         }

@@ -7,15 +7,24 @@ namespace prx {
 // ------------------------------------------------------------------ device state of one CG solve
 // Scalars stay on the device so an iteration needs no host round-trip; every
 // kernel of an iteration returns at once when `done` is set.
+// With `lazy` set, the solution update x += alpha d is applied every second
+// iteration for two directions at once (both are still in the direction
+// ping-pong pair): the vector update then reads and writes x on half the
+// iterations.  `xpend` marks a term alpha_pend * d[dir_buf] not yet in x;
+// solvers flush it before they read x.  `dir_buf` is the ping-pong index of
+// the latest direction (the FIN_BETA caller passes it as the slot).
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
     double bnorm2;
     double slot[8];
     int64_t it, iters, max_iters;
-    int32_t done, status, refresh_due, pad;
+    int32_t done, status, refresh_due, lazy;
+    double alpha_pend;
+    int32_t xpend, dir_buf;
 };
 
-enum FinOp { FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4 };
+// FIN_BETA_FULL: as FIN_BETA after an iteration that left x fully updated
+enum FinOp { FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4, FIN_BETA_FULL = 5 };
 
 // Points on the device.  Input order: positions/normals (SoA, FP64).  Sorted
 // order (by the Morton code of the screening cell): cell corner + weights.
@@ -47,8 +56,12 @@ void launch_export_g2i(uint32_t D, int64_t* g2i, hipStream_t s);
 template <typename T> void launch_fill(T* p, T v, int64_t n, hipStream_t s);
 template <typename T> void launch_dot(const T* a, const T* b, int64_t n, double* partial, int* nblk, hipStream_t s);
 void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s);
-template <typename T> void launch_cg_update(T* x, T* r, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s);
-template <typename T> void launch_axpy_alpha(T* x, const T* d, const CgState* st, int64_t n, hipStream_t s);
+// dprev: the previous direction (lazy x); may be null when the state is not lazy
+template <typename T> void launch_cg_update(T* x, T* r, const T* dprev, const T* d, const T* q, const CgState* st, int64_t n, double* partial, int* nblk, hipStream_t s);
+template <typename T> void launch_axpy_alpha(T* x, const T* dprev, const T* d, const CgState* st, int64_t n, hipStream_t s);
+// x += alpha_pend * d{dir_buf} when a lazy term is pending (the caller then clears xpend)
+template <typename T> void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s);
+void launch_clear_xpend(CgState* st, hipStream_t s);
 template <typename T> void launch_xpby_beta(T* d, const T* r, const CgState* st, int64_t n, hipStream_t s);
 template <typename T> void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s);
 void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s);
@@ -167,9 +180,13 @@ void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, 
 void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, int slot, hipStream_t s);
 // x += alpha_k d ; r -= alpha_k q ; partial per (256 * 16 / sizeof(T)) nodes
 template <typename T>
-int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* d, const T* q, int64_t Ntot, double* partial, hipStream_t s);
+int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* dprev, const T* d, const T* q, int64_t Ntot, double* partial,
+                      hipStream_t s);
 template <typename T>
-void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* d, int64_t Ntot, hipStream_t s);
+void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* dprev, const T* d, int64_t Ntot, hipStream_t s);
+// per instance: x += alpha_pend * d{dir_buf} where a lazy term is pending
+template <typename T>
+void launch_bflush_x(const BatchTab& tab, T* x, const T* d0, const T* d1, int64_t Ntot, hipStream_t s);
 // partial = a.b per (256 * 16 / sizeof(T)) nodes; returns log2 of that unit
 template <typename T>
 int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s);

@@ -359,11 +359,11 @@ int prx_csr_cg(int64_t n, const int64_t* off, const int64_t* col, const double* 
             launch_dot<double>(dd.p, dq.p, n, part.p, &nb, s);
             launch_fin(st.p, part.p, nb, FIN_ALPHA, 0, s);
             if ((it + 1) % 10 == 0) {
-                launch_axpy_alpha<double>(dx.p, dd.p, st.p, n, s);
+                launch_axpy_alpha<double>(dx.p, nullptr, dd.p, st.p, n, s);
                 launch_csr_spmv(n, doff.p, dcol.p, dval.p, dx.p, dq.p, s);
                 launch_sub<double>(dr.p, db.p, dq.p, n, part.p, &nb, st.p, s);
             } else {
-                launch_cg_update<double>(dx.p, dr.p, dd.p, dq.p, st.p, n, part.p, &nb, s);
+                launch_cg_update<double>(dx.p, dr.p, nullptr, dd.p, dq.p, st.p, n, part.p, &nb, s);
             }
             launch_fin(st.p, part.p, nb, FIN_BETA, 0, s);
             launch_xpby_beta<double>(dd.p, dr.p, st.p, n, s);

@@ -89,6 +89,7 @@ static void init_state(prx_tree* t, double eps, uint64_t max_iters, double bnorm
     h.eps = eps;
     h.max_iters = (int64_t)max_iters;
     h.bnorm2 = bnorm2;
+    h.lazy = 1;
     *t->st_host = h;
     PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, t->stream));
 }
@@ -174,16 +175,19 @@ struct Cg {
         launch_stencil<T>(L, OP_DIR_APPLY, io, &nb, t->tile, s);
         applies++;
         launch_fin(t->st, t->partial.p, nb, FIN_ALPHA, 0, s);
+        int fop = FIN_BETA;
         if (refresh > 0 && (k + 1) % (uint64_t)refresh == 0) {
-            launch_axpy_alpha<T>(x, dnew, t->st, L.N, s);
+            launch_axpy_alpha<T>(x, dold, dnew, t->st, L.N, s);
             nb = op_pass<T>(t, L, OP_RESID, x, b, r, done, t->partial.p);
             applies++;
+            fop = FIN_BETA_FULL;
         } else {
-            launch_cg_update<T>(x, r, dnew, q, t->st, L.N, t->partial.p, &nb, s);
+            launch_cg_update<T>(x, r, dold, dnew, q, t->st, L.N, t->partial.p, &nb, s);
         }
-        launch_fin(t->st, t->partial.p, nb, FIN_BETA, 0, s);
+        launch_fin(t->st, t->partial.p, nb, fop, (int)((k + 1) & 1), s);
     }
-    // Iterate until the device flag is set or `cap` iterations are complete.
+    // Iterate until the device flag is set or `cap` iterations are complete;
+    // x is complete on return (a pending lazy term is flushed).
     void run(uint64_t cap) {
         const int chunk = L.N >= (1 << 21) ? 8 : 32;
         uint64_t k = it;
@@ -193,6 +197,11 @@ struct Cg {
             if (t->st_host->done || k >= cap) break;
         }
         it = (uint64_t)t->st_host->it;
+        if (t->st_host->xpend) {
+            launch_flush_x<T>(x, dd[0], dd[1], t->st, L.N, t->stream);
+            launch_clear_xpend(t->st, t->stream);
+            t->st_host->xpend = 0;
+        }
     }
     // Replace the recursive residual by `delta_true` = |r|^2 of a true
     // residual already written to r.  While the two residuals still agree
