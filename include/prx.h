@@ -50,6 +50,9 @@ enum { PRX_PREC_FP64 = 0, PRX_PREC_MIXED = 1 };
  * the residual after smoothing (src/solvers.rs:36-58).  TRUE: the FP64 true
  * residual ||b-Ax|| <= eps*||b||. */
 enum { PRX_RESID_REFERENCE = 0, PRX_RESID_TRUE = 1 };
+/* Fine solver.  NONE: the reference's cascadic solve (coarse CG, GS, CG).
+ * MG: CG preconditioned with a multigrid V-cycle (an extension). */
+enum { PRX_PRECOND_NONE = 0, PRX_PRECOND_MG = 1 };
 
 /* ReconstructParams (src/reconstructor.rs:8-15): the reference fields in the
  * reference order, then extension fields.  prx_params_default() gives the
@@ -71,6 +74,7 @@ typedef struct prx_params {
     int32_t refresh_period; /* CG true-residual refresh period (reference: 10; 0 = never) */
     double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.1) */
     uint64_t max_outer;     /* mixed: cap on FP64 residual replacements (default 64) */
+    int32_t precond;        /* PRX_PRECOND_NONE (default: the reference's cascadic solve) or PRX_PRECOND_MG */
 } prx_params;
 
 /* ReconstructStats (src/reconstructor.rs:17-23) plus extension fields.
@@ -107,6 +111,14 @@ typedef struct prx_solve_opts {
     double inner_tol;
     uint64_t max_outer;
     double coarse_eps;      /* PRX_RESID_TRUE only: coarsest-level ||b-Ax|| <= coarse_eps ||b||; 0 = eps */
+    /* PRX_PRECOND_MG replaces the cascadic GS + CG by CG preconditioned with
+     * a geometric-multigrid V-cycle, from x = 0, stopping at
+     * ||b-Ax|| <= eps ||b|| (no counterpart in the reference; same system,
+     * same stopping test as PRX_RESID_TRUE) */
+    int32_t precond;        /* PRX_PRECOND_NONE (default) or PRX_PRECOND_MG */
+    int32_t mg_smooth;      /* damped-Jacobi sweeps before and after each coarse correction (default 2) */
+    double mg_omega;        /* Jacobi damping (default 0.8) */
+    double mg_inner_tol;    /* mixed MG: FP32 residual reduction per FP64 refinement step (default 1e-4) */
 } prx_solve_opts;
 
 typedef struct prx_solve_report {
@@ -168,6 +180,12 @@ int prx_export_csr(prx_tree* t, int64_t* off, int64_t* col, double* val);
  * src/sparse.rs:105-117, on the assembled system).  precision
  * PRX_PREC_MIXED applies the FP32 operator. */
 int prx_apply(prx_tree* t, int precision, const double* x, double* y);
+
+/* z = M in, one multigrid V-cycle of the PRX_PRECOND_MG solver (M ~ A^-1
+ * without the per-point screening; smooth <= 0 and omega <= 0 select the
+ * defaults).  MIXED applies the FP32 V-cycle of the scaled system.  No
+ * counterpart in the reference: a diagnostic of the preconditioner. */
+int prx_mg_vcycle(prx_tree* t, int precision, int smooth, double omega, const double* in, double* out);
 
 /* build_system_at_depth (src/fem_tree.rs:346-404) at depth cdepth: RHS in
  * row-major coarse order [8^cdepth] and the operator's constants

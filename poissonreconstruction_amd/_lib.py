@@ -25,6 +25,8 @@ PRX_PREC_FP64 = 0
 PRX_PREC_MIXED = 1
 PRX_RESID_REFERENCE = 0
 PRX_RESID_TRUE = 1
+PRX_PRECOND_NONE = 0
+PRX_PRECOND_MG = 1
 PRX_MAX_DEPTH = 10
 
 STATUS_NAMES = {0: "ok", 1: "invalid", 2: "nonfinite", 3: "not_converged", 4: "breakdown", 5: "device", 6: "oom", 7: "state"}
@@ -43,7 +45,8 @@ class Params(ctypes.Structure):
                 ("cg_accuracy", ctypes.c_double), ("cg_iters", ctypes.c_uint64), ("point_weight", ctypes.c_double),
                 ("verbose", ctypes.c_int32), ("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("gs_iters", ctypes.c_int32), ("base_depth", ctypes.c_int32),
-                ("refresh_period", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64)]
+                ("refresh_period", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
+                ("precond", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -63,7 +66,8 @@ class Mesh(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
                 ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
-                ("coarse_eps", ctypes.c_double)]
+                ("coarse_eps", ctypes.c_double), ("precond", ctypes.c_int32), ("mg_smooth", ctypes.c_int32),
+                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double)]
 
 
 class SolveReport(ctypes.Structure):
@@ -120,6 +124,7 @@ SIGNATURES = {
     "prx_csr_nnz": (_i32, [_vp, _vp]),
     "prx_export_csr": (_i32, [_vp, _vp, _vp, _vp]),
     "prx_apply": (_i32, [_vp, _i32, _vp, _vp]),
+    "prx_mg_vcycle": (_i32, [_vp, _i32, _i32, _f64, _vp, _vp]),
     "prx_coarse_system": (_i32, [_vp, _u32, _vp, _vp]),
     "prx_prolongate": (_i32, [_vp, _u32, _vp, _vp]),
     "prx_gs": (_i32, [_vp, _i32, _u32, _vp, _vp]),

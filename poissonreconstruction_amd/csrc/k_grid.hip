@@ -140,6 +140,18 @@ __device__ __forceinline__ T x_pair(T x, T dp, T d, double ap, double a) {
 }
 
 // The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
+// the vector update of an iteration has run: account for the lazy x term
+// (dir: ping-pong index of the iteration's direction)
+__device__ inline void lazy_step(CgState* st, bool full, int dir) {
+    st->dir_buf = dir;
+    if (full || !st->lazy || st->xpend) {
+        st->xpend = 0;
+    } else {
+        st->xpend = 1;
+        st->alpha_pend = st->alpha;
+    }
+}
+
 __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
     switch (op) {
         case FIN_SLOT: st->slot[slot] = sum; break;
@@ -166,14 +178,7 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
             break;
         case FIN_BETA:
         case FIN_BETA_FULL: {
-            // the vector update of this iteration has run: account for the lazy x term
-            st->dir_buf = slot;
-            if (op == FIN_BETA_FULL || !st->lazy || st->xpend) {
-                st->xpend = 0;
-            } else {
-                st->xpend = 1;
-                st->alpha_pend = st->alpha;
-            }
+            lazy_step(st, op == FIN_BETA_FULL, slot);
             double dold = st->delta;
             st->delta_prev = dold;
             st->delta = sum;
@@ -196,6 +201,50 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
                 st->iters = st->max_iters;
                 st->status = PRX_ERR_NOT_CONVERGED;
             }
+            break;
+        }
+        case FIN_PINIT:  // sum = r0.z0 ; slot[0] holds |r0|^2 ; stop when |r|^2 <= ||b||^2 eps^2
+            st->delta = sum;
+            st->delta0 = sum;
+            st->delta_prev = sum;
+            st->beta = 0.0;
+            st->it = 0;
+            st->iters = 0;
+            st->status = 0;
+            st->refresh_due = 0;
+            st->xpend = 0;
+            st->thr = (st->bnorm2 * st->eps) * st->eps;
+            st->slot[1] = st->slot[0];
+            st->done = (sum == 0.0 || st->slot[0] <= st->thr) ? 1 : 0;
+            break;
+        case FIN_PRES:  // sum = |r|^2 after the update of iteration it
+            lazy_step(st, false, slot);
+            st->slot[1] = sum;
+            st->it += 1;
+            if (sum != sum) {
+                st->done = 1;
+                st->status = PRX_ERR_BREAKDOWN;
+                st->iters = st->max_iters;
+            } else if (sum <= st->thr) {
+                st->done = 1;
+                st->iters = st->it;
+            } else if (st->it >= st->max_iters) {
+                st->done = 1;
+                st->iters = st->max_iters;
+                st->status = PRX_ERR_NOT_CONVERGED;
+            }
+            break;
+        case FIN_PBETA: {  // sum = r.z of the new residual
+            const double beta = sum / st->delta;
+            if (beta != beta || beta == INFINITY || beta == -INFINITY) {
+                st->done = 1;
+                st->status = PRX_ERR_BREAKDOWN;
+                st->iters = st->max_iters;
+                break;
+            }
+            st->beta = beta;
+            st->delta_prev = st->delta;
+            st->delta = sum;
             break;
         }
     }
@@ -225,7 +274,7 @@ __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) 
 // exactly as solve_cg does (src/solvers.rs:36-64).
 __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
     __shared__ double ws[16];
-    if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && st->done) return;
+    if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && op != FIN_PINIT && st->done) return;
     const double sum = fin_sum(partial, np, ws);
     if (threadIdx.x != 0) return;
     fin_step(st, sum, op, slot);

@@ -501,6 +501,22 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 tacc += q[e] * q[e];
             }
             *(NV*)(io.y + g) = q;
+        } else if constexpr (OP == OP_JACOBI) {
+            const NV bb = ld_v(io.b + g);
+            const T jw = (T)io.jw;
+            NV q;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                q[e] = xin[v * VEC + e] + jw * (bb[e] - tile[stage_at<B>(m) + e]);
+                tacc += bb[e] * q[e];
+            }
+            *(NV*)(io.y + g) = q;
+        } else if constexpr (OP == OP_JACOBI2) {
+            const T j2 = (T)(2.0 * io.jw), jj = (T)(io.jw * io.jw);
+            NV q;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) q[e] = j2 * xin[v * VEC + e] - jj * tile[stage_at<B>(m) + e];
+            *(NV*)(io.y + g) = q;
         } else {
             const nd2 bb = *(const nd2*)((const double*)io.b + g);
             nd2 xo;
@@ -514,6 +530,10 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             __builtin_nontemporal_store(rr, (nf2*)(io.r32 + g));
             tacc += q0 * q0 + q1 * q1;
         }
+    }
+    if constexpr (OP == OP_JACOBI2) return;
+    if constexpr (OP == OP_JACOBI) {
+        if (!io.partial) return;  // uniform over the block
     }
     const double sacc = block_sum_red((double)tacc, wsum);
     if (tid == 0 && io.partial) io.partial[bid] = sacc;
@@ -601,6 +621,12 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
             if constexpr (OP == OP_APPLY) {
                 io.y[i] = a;
                 acc = (double)io.x[i] * (double)a;
+            } else if constexpr (OP == OP_JACOBI) {
+                const T y = io.x[i] + (T)io.jw * (io.b[i] - a);
+                io.y[i] = y;
+                acc = (double)io.b[i] * (double)y;
+            } else if constexpr (OP == OP_JACOBI2) {
+                io.y[i] = (T)(2.0 * io.jw) * io.x[i] - (T)(io.jw * io.jw) * a;
             } else {
                 const T r = io.b[i] - a;
                 io.y[i] = r;
@@ -627,6 +653,8 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
                 hipLaunchKernelGGL((k_stencil_node<T, OP_DIR_APPLY>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N);
                 break;
             case OP_RESID: hipLaunchKernelGGL((k_stencil_node<T, OP_RESID>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
+            case OP_JACOBI: hipLaunchKernelGGL((k_stencil_node<T, OP_JACOBI>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
+            case OP_JACOBI2: hipLaunchKernelGGL((k_stencil_node<T, OP_JACOBI2>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N); break;
             default:
                 if constexpr (sizeof(T) == 8)
                     hipLaunchKernelGGL((k_stencil_node<T, OP_RESID_UPD>), dim3(nb), dim3(256), 0, s, a, io, (uint32_t)N);
@@ -645,11 +673,15 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         if (op == OP_APPLY) PRX_TILE_LAUNCH(16, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_TILE_LAUNCH(16, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_TILE_LAUNCH(16, OP_RESID);
+        else if (op == OP_JACOBI) PRX_TILE_LAUNCH(16, OP_JACOBI);
+        else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(16, OP_JACOBI2);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(16, OP_RESID_UPD);
     } else {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(8, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_TILE_LAUNCH(8, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_TILE_LAUNCH(8, OP_RESID);
+        else if (op == OP_JACOBI) PRX_TILE_LAUNCH(8, OP_JACOBI);
+        else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(8, OP_JACOBI2);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(8, OP_RESID_UPD);
     }
 #undef PRX_TILE_LAUNCH

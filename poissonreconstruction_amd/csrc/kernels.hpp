@@ -23,8 +23,15 @@ struct CgState {
     int32_t xpend, dir_buf;
 };
 
-// FIN_BETA_FULL: as FIN_BETA after an iteration that left x fully updated
-enum FinOp { FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4, FIN_BETA_FULL = 5 };
+// FIN_BETA_FULL: as FIN_BETA after an iteration that left x fully updated.
+// Preconditioned CG (delta = r.z): FIN_PINIT starts from delta = r0.z0,
+// FIN_PRES takes |r|^2 after the vector update (lazy-x bookkeeping, the
+// iteration count and the stopping test on |r|^2 <= thr), FIN_PBETA takes the
+// new r.z and forms beta.
+enum FinOp {
+    FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4, FIN_BETA_FULL = 5,
+    FIN_PINIT = 6, FIN_PRES = 7, FIN_PBETA = 8
+};
 
 // Points on the device.  Input order: positions/normals (SoA, FP64).  Sorted
 // order (by the Morton code of the screening cell): cell corner + weights.
@@ -48,7 +55,11 @@ struct LevelView {
     const DevPoints* pts = nullptr;
 };
 
-enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3 };
+// OP_JACOBI: y = x + jw * (b - A x), one damped-Jacobi step (jw = omega / a_ii,
+// a_ii constant), with partial sums of b.y when io.partial is set.
+// OP_JACOBI2: the first two such steps from x = 0 in one pass over b (the
+// operand io.x = b): y = 2 jw b - jw^2 A b.
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5 };
 
 // ---- k_grid.hip
 void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
@@ -62,6 +73,16 @@ template <typename T> void launch_axpy_alpha(T* x, const T* dprev, const T* d, c
 // x += alpha_pend * d{dir_buf} when a lazy term is pending (the caller then clears xpend)
 template <typename T> void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s);
 void launch_clear_xpend(CgState* st, hipStream_t s);
+
+// ---- k_mg.hip: geometric multigrid on the dense Morton grids
+// x = w * b
+template <typename T> void launch_mg_scale(T* x, const T* b, double w, int64_t n, hipStream_t s);
+// b_c = P^T r_f from depth dc + 1 to dc (trilinear hats, nodes outside the grid absent)
+template <typename T> void launch_mg_restrict(uint32_t dc, const T* rf, T* bc, hipStream_t s);
+// x_f += P x_c from depth df - 1 to df
+template <typename T> void launch_mg_prolong_add(uint32_t df, const T* xc, T* xf, hipStream_t s);
+// x = Ainv b on a grid of n <= 512 nodes (Ainv row-major, FP64)
+template <typename T> void launch_mg_dense(int n, const double* ainv, const T* b, T* x, hipStream_t s);
 template <typename T> void launch_xpby_beta(T* d, const T* r, const CgState* st, int64_t n, hipStream_t s);
 template <typename T> void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s);
 void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s);
@@ -125,6 +146,7 @@ struct StencilIO {
     const T* s = nullptr;  // per-point screening values wh3*sum(phi*x)
     double* partial = nullptr;
     const int32_t* done = nullptr;
+    double jw = 0.0;  // OP_JACOBI weight
 };
 template <typename T>
 void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);

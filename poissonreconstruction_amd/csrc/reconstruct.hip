@@ -228,6 +228,7 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     o.refresh_period = p.refresh_period;
     o.gs_iters = p.gs_iters;
     o.inner_tol = p.inner_tol > 0.0 ? p.inner_tol : 0.1;
+    o.precond = p.precond;
     o.max_outer = p.max_outer > 0 ? p.max_outer : 64;
     prx_solve_report rep;
     rc = prx_solve_cascadic(t, base, (uint32_t)p.gs_iters, p.cg_iters, p.cg_accuracy, &o, &rep);

@@ -424,9 +424,395 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     rep.status = status;
 }
 
+// ------------------------------------------------------------------ multigrid-preconditioned CG
+// Preconditioner M ~ (L + wg I)^-1: one V-cycle over the dense grids of
+// depths dc..D.  Each level smooths with damped Jacobi (nu sweeps from x = 0
+// before the coarse correction, nu after), restricts the residual with P^T,
+// prolongates the coarse correction with P (trilinear hats) and solves
+// depth dc exactly with a dense inverse.  Level d uses the hat stiffness of
+// its own h, which is the Galerkin product P^T L_{d+1} P of nested hat
+// spaces up to the truncated boundary rows, and the lumped mass wg * 8^(D-d).
+// The per-point screening is left out of M (its entries are ~wh3 = pw h^3/1.5,
+// about 1e-5 of the stencil diagonal at pw = 4) and enters through the exact
+// operator of the outer CG.  Equal pre- and post-smoothing, R = P^T and an
+// exact coarse solve make M symmetric positive definite.
+template <typename T>
+static DArr<T>& mg_pool(prx_tree* t);
+template <>
+DArr<double>& mg_pool<double>(prx_tree* t) { return t->mg64; }
+template <>
+DArr<float>& mg_pool<float>(prx_tree* t) { return t->mg32; }
+
+template <typename T>
+struct Mg {
+    uint32_t D = 0, dc = 0;
+    int nu = 2;
+    std::vector<LevelView> lv;  // [d - dc]
+    std::vector<double> jw;     // omega / a_ii per level
+    std::vector<T*> xa, xb, bv, rv;
+    int nc = 0;                 // nodes of the coarsest grid
+    uint64_t cycles = 0;
+};
+
+// Dense inverse of L + wg I on depth dc (nodes in Morton order), entries
+// scaled by `scale`; Cholesky factorisation on the host, FP64.
+static void mg_dense_inverse(uint32_t dc, double s, double wg, std::vector<double>& inv) {
+    const int r = 1 << dc, n = r * r * r;
+    std::vector<double> a((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        uint32_t x, y, z;
+        demorton3((uint32_t)i, x, y, z);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int jx = (int)x + dx, jy = (int)y + dy, jz = (int)z + dz;
+                    if (jx < 0 || jy < 0 || jz < 0 || jx >= r || jy >= r || jz >= r) continue;
+                    const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+                    const double w = nz == 0 ? 32.0 * s + wg : nz == 2 ? -2.0 * s : nz == 3 ? -s : 0.0;
+                    a[(size_t)i * n + morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz)] = w;
+                }
+    }
+    // a = C C^T (lower C in place)
+    for (int j = 0; j < n; ++j) {
+        double d = a[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= a[(size_t)j * n + k] * a[(size_t)j * n + k];
+        if (!(d > 0.0)) throw Error(PRX_ERR_BREAKDOWN, "coarsest multigrid operator is not positive definite");
+        const double cjj = std::sqrt(d);
+        a[(size_t)j * n + j] = cjj;
+        for (int i = j + 1; i < n; ++i) {
+            double v = a[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) v -= a[(size_t)i * n + k] * a[(size_t)j * n + k];
+            a[(size_t)i * n + j] = v / cjj;
+        }
+    }
+    // inverse column by column: C y = e_c, C^T x = y
+    inv.assign((size_t)n * n, 0.0);
+    std::vector<double> y(n);
+    for (int c = 0; c < n; ++c) {
+        for (int i = 0; i < n; ++i) {
+            double v = i == c ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= a[(size_t)i * n + k] * y[k];
+            y[i] = v / a[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double v = y[i];
+            for (int k = i + 1; k < n; ++k) v -= a[(size_t)k * n + i] * inv[(size_t)k * n + c];
+            inv[(size_t)i * n + c] = v / a[(size_t)i * n + i];
+        }
+    }
+}
+
+// Level views, Jacobi weights, vectors (one pool) and the coarsest inverse.
+// The FP32 system is the FP64 one scaled by 1/h (as in the mixed solve).
+template <typename T>
+static Mg<T> mg_build(prx_tree* t, int nu, double omega) {
+    Mg<T> m;
+    m.D = t->D;
+    m.dc = std::min<uint32_t>(t->D, 2);
+    m.nu = std::max(1, nu);
+    const int nl = (int)(m.D - m.dc) + 1;
+    const double ih = 1.0 / t->h;
+    const double scale = sizeof(T) == 4 ? ih : 1.0;
+    m.lv.resize(nl);
+    m.jw.resize(nl);
+    m.xa.assign(nl, nullptr); m.xb.assign(nl, nullptr); m.bv.assign(nl, nullptr); m.rv.assign(nl, nullptr);
+    int64_t total = 0;
+    std::vector<int64_t> off(4 * nl, -1);
+    for (int l = 0; l < nl; ++l) {
+        const uint32_t d = m.dc + (uint32_t)l;
+        const int64_t n = (int64_t)1 << (3 * d);
+        const double h = 1.0 / (double)(1u << d);
+        const double K0 = stiffness_entry(h, 0, 0, 0), Ke = stiffness_entry(h, 1, 1, 0), Kc = stiffness_entry(h, 1, 1, 1);
+        const double wg = t->wg * (double)((uint64_t)1 << (3 * (m.D - d)));
+        LevelView& L = m.lv[l];
+        L.D = d;
+        L.res = 1u << d;
+        L.N = n;
+        L.k64 = {K0, Ke, Kc, wg, 0.0};
+        L.k32 = {(float)(K0 * ih), (float)(Ke * ih), (float)(Kc * ih), (float)(wg * ih), 0.0f};
+        L.scr = nullptr;
+        L.pts = nullptr;
+        m.jw[l] = omega / ((32.0 * -Kc + wg) * scale);
+        // fine: xa = smoothing partner, rv = residual (input and output are the caller's)
+        // middle: xa, xb, bv, rv ; coarsest: bv, xa
+        const int cnt = l == nl - 1 ? 2 : (l == 0 ? 2 : 4);
+        for (int k = 0; k < cnt; ++k) {
+            off[4 * l + k] = total;
+            total += n;
+        }
+    }
+    DArr<T>& pool = mg_pool<T>(t);
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    pool.ensure(total);
+    for (int l = 0; l < nl; ++l) {
+        T* p = pool.p;
+        if (l == nl - 1) {
+            m.xa[l] = p + off[4 * l];
+            m.rv[l] = p + off[4 * l + 1];
+        } else if (l == 0) {
+            m.bv[l] = p + off[4 * l];
+            m.xa[l] = p + off[4 * l + 1];
+        } else {
+            m.xa[l] = p + off[4 * l];
+            m.xb[l] = p + off[4 * l + 1];
+            m.bv[l] = p + off[4 * l + 2];
+            m.rv[l] = p + off[4 * l + 3];
+        }
+    }
+    m.nc = 1 << (3 * m.dc);
+    const double hc = 1.0 / (double)(1u << m.dc);
+    std::vector<double> inv;
+    mg_dense_inverse(m.dc, -stiffness_entry(hc, 1, 1, 1) * scale, m.lv[0].k64.wg * scale, inv);
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    t->mg_ainv.ensure((int64_t)inv.size());
+    PRX_HIP(hipMemcpyAsync(t->mg_ainv.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, t->stream));
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    return m;
+}
+
+// z = M b on the finest level (b and z are N-vectors of the caller).  The
+// first two pre-smoothing steps from x = 0 are one pass over b (OP_JACOBI2);
+// the last post-smoothing pass of the finest level also leaves the partial
+// sums of b.z in t->partial.  Returns their count (0: none were formed).
+template <typename T>
+static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
+    const hipStream_t s = t->stream;
+    const int top = (int)m.lv.size() - 1;
+    std::vector<T*> cur(top + 1, nullptr), oth(top + 1, nullptr);
+    int nb_rz = 0;
+    auto jacobi = [&](int l, const T* b, T*& c, T*& o, bool rz) {
+        StencilIO<T> io;
+        io.x = c;
+        io.b = b;
+        io.y = o;
+        io.jw = m.jw[l];
+        io.partial = rz ? t->partial.p : nullptr;
+        int nb = 0;
+        launch_stencil<T>(m.lv[l], OP_JACOBI, io, &nb, t->tile, s);
+        if (rz) nb_rz = nb;
+        std::swap(c, o);
+    };
+    // writes per level: pre nu-1 (nu >= 2) or 1, post nu; the first goes to
+    // the buffer that makes the last one land in the level's output
+    const int writes = m.nu >= 2 ? 2 * m.nu - 1 : 2;
+    for (int l = top; l > 0; --l) {
+        const LevelView& L = m.lv[l];
+        const T* b = l == top ? b_fine : m.bv[l];
+        T* out = l == top ? z_fine : m.xb[l];
+        T* c = (writes & 1) ? out : m.xa[l];
+        T* o = (writes & 1) ? m.xa[l] : out;
+        if (m.nu >= 2) {
+            StencilIO<T> io;
+            io.x = b;
+            io.y = c;
+            io.jw = m.jw[l];
+            int nb = 0;
+            launch_stencil<T>(L, OP_JACOBI2, io, &nb, t->tile, s);
+            for (int i = 2; i < m.nu; ++i) jacobi(l, b, c, o, false);
+        } else {
+            launch_mg_scale<T>(c, b, m.jw[l], L.N, s);
+        }
+        StencilIO<T> io;
+        io.x = c;
+        io.b = b;
+        io.y = m.rv[l];
+        int nb = 0;
+        launch_stencil<T>(L, OP_RESID, io, &nb, t->tile, s);
+        launch_mg_restrict<T>(L.D - 1, m.rv[l], m.bv[l - 1], s);
+        cur[l] = c;
+        oth[l] = o;
+    }
+    launch_mg_dense<T>(m.nc, t->mg_ainv.p, top == 0 ? b_fine : m.bv[0], top == 0 ? z_fine : m.xa[0], s);
+    cur[0] = top == 0 ? z_fine : m.xa[0];
+    for (int l = 1; l <= top; ++l) {
+        const T* b = l == top ? b_fine : m.bv[l];
+        launch_mg_prolong_add<T>(m.lv[l].D, cur[l - 1], cur[l], s);
+        for (int i = 0; i < m.nu; ++i) jacobi(l, b, cur[l], oth[l], l == top && i == m.nu - 1);
+    }
+    if (top > 0 && cur[top] != z_fine) throw Error(PRX_ERR_STATE, "multigrid buffer parity");
+    m.cycles++;
+    return nb_rz;
+}
+
+// Preconditioned CG (M = one V-cycle) on level L, the exact operator with
+// screening; x is updated in place.  Stops when |r|^2 <= bnorm2 eps^2 (r the
+// recursive residual) or after max_iters iterations.  The update
+// d = z + beta d is fused into the operator pass as in the plain CG, and x
+// is updated lazily in pairs.  One host round trip per iteration decides
+// whether the next V-cycle is needed.
+template <typename T>
+static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b, bool x_zero, T* r, T* z, T* dd, T* q,
+                     double eps, double bnorm2, uint64_t max_iters) {
+    const hipStream_t s = t->stream;
+    const int64_t n = L.N;
+    CgOut out;
+    init_state(t, eps, max_iters, bnorm2);
+    int nb = 0;
+    if (x_zero) {
+        if (r != b) PRX_HIP(hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+        launch_dot<T>(r, r, n, t->partial.p, &nb, s);
+    } else {
+        nb = op_pass<T>(t, L, OP_RESID, x, b, r, nullptr, t->partial.p);
+        out.applies++;
+    }
+    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+    nb = mg_vcycle<T>(t, m, r, z);
+    if (nb == 0) launch_dot<T>(r, z, n, t->partial.p, &nb, s);
+    launch_fin(t->st, t->partial.p, nb, FIN_PINIT, 0, s);
+    launch_fill<T>(dd, T(0), n, s);
+    read_state(t);
+    uint64_t k = 0;
+    while (!t->st_host->done) {
+        T* dold = dd + (k & 1) * n;
+        T* dnew = dd + ((k + 1) & 1) * n;
+        StencilIO<T> io;
+        io.r = z;
+        io.dold = dold;
+        io.dnew = dnew;
+        io.beta = &t->st->beta;
+        io.y = q;
+        io.partial = t->partial.p;
+        io.done = &t->st->done;
+        if (L.scr) {
+            T* sb = sbuf_for<T>(t);
+            launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, z, dold, &t->st->beta);
+            io.s = sb;
+        }
+        launch_stencil<T>(L, OP_DIR_APPLY, io, &nb, t->tile, s);
+        out.applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_ALPHA, 0, s);
+        launch_cg_update<T>(x, r, dold, dnew, q, t->st, n, t->partial.p, &nb, s);
+        launch_fin(t->st, t->partial.p, nb, FIN_PRES, (int)((k + 1) & 1), s);
+        ++k;
+        read_state(t);
+        if (t->st_host->done) break;
+        nb = mg_vcycle<T>(t, m, r, z);
+        if (nb == 0) launch_dot<T>(r, z, n, t->partial.p, &nb, s);
+        launch_fin(t->st, t->partial.p, nb, FIN_PBETA, 0, s);
+    }
+    if (t->st_host->xpend) {
+        launch_flush_x<T>(x, dd, dd + n, t->st, n, s);
+        launch_clear_xpend(t->st, s);
+        t->st_host->xpend = 0;
+    }
+    const CgState& h = *t->st_host;
+    out.iters = (uint64_t)h.it;
+    out.status = h.status;
+    out.rel = bnorm2 > 0.0 ? std::sqrt(h.slot[1] / bnorm2) : 0.0;
+    return out;
+}
+
+static int mg_nu(const prx_solve_opts& o) { return o.mg_smooth > 0 ? o.mg_smooth : 2; }
+static double mg_omega(const prx_solve_opts& o) { return o.mg_omega > 0.0 && o.mg_omega < 1.3 ? o.mg_omega : 0.8; }
+
+// FP64 PCG from x = 0 to ||b - A x|| <= eps ||b||, confirmed on the true
+// residual (a PCG restart from the current x if the recursive residual drifted).
+static void mg_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    t->r64.ensure(n); t->d64.ensure(2 * n); t->q64.ensure(n); t->tmp64.ensure(n);
+    Timer ts(s);
+    Mg<double> m = mg_build<double>(t, mg_nu(c.o), mg_omega(c.o));
+    rep.ms_coarse = ts.ms();
+    Timer tm(s);
+    launch_fill<double>(t->x.p, 0.0, n, s);
+    const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
+    const double thr = (bb * c.eps) * c.eps;
+    uint64_t iters = 0;
+    int status = PRX_OK;
+    bool first = true;
+    for (int pass = 0; pass < 4; ++pass) {
+        CgOut o = run_pcg<double>(t, m, L, t->x.p, t->b.p, first, t->r64.p, t->tmp64.p, t->d64.p, t->q64.p, c.eps, bb,
+                                  cap > iters ? cap - iters : 1);
+        first = false;
+        iters += o.iters;
+        rep.applies += o.applies;
+        status = o.status;
+        if (o.status != PRX_OK) break;
+        int nb = op_pass<double>(t, L, OP_RESID, t->x.p, t->b.p, t->r64.p, nullptr, t->partial.p);
+        rep.applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+        read_state(t);
+        if (t->st_host->slot[0] <= thr) break;
+        if (iters >= cap) { status = PRX_ERR_NOT_CONVERGED; break; }
+    }
+    rep.ms_fine_cg = tm.ms();
+    rep.fine_iters = iters;
+    rep.outer_iters = 1;
+    rep.coarse_iters = m.cycles;
+    rep.fine_rel = 0.0;
+    rep.status = status;
+}
+
+// Mixed precision: FP64 refinement x += e around FP32 PCG solves of
+// (A/h) e = (b - A x)/h, each to a residual reduction mg_inner_tol (or less
+// when less reaches the target); the true residual b - A x is formed in FP64
+// (in the same pass that folds e into x) and decides convergence.
+static void mg_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
+    const LevelView L = t->fine_view();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    t->e32.ensure(n); t->ri32.ensure(n); t->d32.ensure(2 * n); t->q32.ensure(n); t->r32.ensure(n);
+    const double ih = 1.0 / t->h;
+    const double tol = c.o.mg_inner_tol > 0.0 && c.o.mg_inner_tol < 1.0 ? c.o.mg_inner_tol : 1e-4;
+    Timer ts(s);
+    Mg<float> m = mg_build<float>(t, mg_nu(c.o), mg_omega(c.o));
+    rep.ms_coarse = ts.ms();
+    Timer tm(s);
+    launch_fill<double>(t->x.p, 0.0, n, s);
+    const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
+    const double thr = (bb * c.eps) * c.eps;
+    launch_fill<float>(t->e32.p, 0.0f, n, s);
+    int nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
+    rep.applies++;
+    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+    read_state(t);
+    double rr = t->st_host->slot[0];
+    uint64_t iters = 0, outer = 0;
+    int status = PRX_OK;
+    while (rr > thr) {
+        if (outer >= c.o.max_outer || iters >= cap) { status = PRX_ERR_NOT_CONVERGED; break; }
+        launch_fill<float>(t->e32.p, 0.0f, n, s);
+        // reduce by mg_inner_tol, or only by what the target still needs (with margin 2)
+        const double need = 0.5 * std::sqrt(thr / rr);
+        CgOut o = run_pcg<float>(t, m, L, t->e32.p, t->ri32.p, true, t->ri32.p, t->r32.p, t->d32.p, t->q32.p,
+                                 std::max(tol, need), rr * ih * ih, cap - iters);
+        iters += o.iters;
+        rep.applies += o.applies;
+        if (o.status == PRX_ERR_BREAKDOWN) { status = o.status; break; }
+        nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
+        rep.applies++;
+        launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
+        read_state(t);
+        const double rn = t->st_host->slot[0];
+        ++outer;
+        if (!(rn < rr)) { rr = rn; status = PRX_ERR_NOT_CONVERGED; break; }
+        rr = rn;
+    }
+    rep.ms_fine_cg = tm.ms();
+    rep.fine_iters = iters;
+    rep.outer_iters = outer;
+    rep.coarse_iters = m.cycles;
+    rep.fine_rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    rep.status = status;
+}
+
+static void solve_mg(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
+    Timer total(t->stream);
+    if (c.o.precision == PRX_PREC_MIXED) mg_solve_mixed(t, c, c.cg_iters, rep);
+    else mg_solve_fp64(t, c, c.cg_iters, rep);
+    rep.ms_total = total.ms();
+    rep.true_rel = true_rel(t);
+}
+
 static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
     std::memset(&rep, 0, sizeof rep);
     t->ensure_state();
+    if (c.o.precond == PRX_PRECOND_MG) {
+        solve_mg(t, c, rep);
+        return;
+    }
     const hipStream_t s = t->stream;
     Timer total(s);
     if (c.base >= t->D) {
@@ -743,6 +1129,10 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->gs_iters = 20;
     o->inner_tol = 0.1;
     o->max_outer = 64;
+    o->precond = PRX_PRECOND_NONE;
+    o->mg_smooth = 2;
+    o->mg_omega = 0.8;
+    o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
 }
 
@@ -905,6 +1295,41 @@ int prx_apply(prx_tree* t, int precision, const double* x, double* y) {
     } else {
         prx::op_pass<double>(t, L, OP_APPLY, t->r64.p, nullptr, t->q64.p, nullptr, t->partial.p);
         PRX_HIP(hipMemcpyAsync(y, t->q64.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+    }
+    PRX_API_END
+}
+
+int prx_mg_vcycle(prx_tree* t, int precision, int smooth, double omega, const double* in, double* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && in && out, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->assembled, PRX_ERR_STATE, "assemble not called");
+    set_device(t);
+    t->ensure_state();
+    const int64_t n = t->N;
+    const hipStream_t s = t->stream;
+    prx_solve_opts o;
+    prx_solve_opts_default(&o);
+    o.mg_smooth = smooth;
+    o.mg_omega = omega;
+    t->r64.ensure(n);
+    t->q64.ensure(n);
+    PRX_HIP(hipMemcpyAsync(t->r64.p, in, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (precision == PRX_PREC_MIXED) {
+        // the FP32 system is A/h: z = (A/h)^-1 (b/h)
+        t->d32.ensure(n);
+        t->q32.ensure(n);
+        prx::Mg<float> m = prx::mg_build<float>(t, prx::mg_nu(o), prx::mg_omega(o));
+        launch_cast_scale_d2f(t->d32.p, t->r64.p, 1.0 / t->h, n, s);
+        prx::mg_vcycle<float>(t, m, t->d32.p, t->q32.p);
+        std::vector<float> hz(n);
+        PRX_HIP(hipMemcpyAsync(hz.data(), t->q32.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) out[i] = (double)hz[i];
+    } else {
+        prx::Mg<double> m = prx::mg_build<double>(t, prx::mg_nu(o), prx::mg_omega(o));
+        prx::mg_vcycle<double>(t, m, t->r64.p, t->q64.p);
+        PRX_HIP(hipMemcpyAsync(out, t->q64.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         PRX_HIP(hipStreamSynchronize(s));
     }
     PRX_API_END

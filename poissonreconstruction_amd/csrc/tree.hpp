@@ -122,6 +122,11 @@ struct prx_tree {
     uint32_t cd = 0;
     DArr<double> cb, cx, cr, cdd, cq, cpart;
 
+    // multigrid preconditioner: level vectors (one pool per precision) and
+    // the inverse of the coarsest operator
+    DArr<double> mg64, mg_ainv;
+    DArr<float> mg32;
+
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
         free_list(floor_list);

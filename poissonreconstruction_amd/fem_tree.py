@@ -33,6 +33,16 @@ class MatrixFreeSystem:
         check(L.lib().prx_apply(self._tree._h, int(precision), ptr(x), ptr(y)))
         return y
 
+    def mg_vcycle(self, b, precision: int = L.PRX_PREC_FP64, smooth: int = 0, omega: float = 0.0) -> np.ndarray:
+        """z = M b, one V-cycle of the multigrid preconditioner (M ~ A^-1
+        without the per-point screening; 0 selects the default smoothing)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (self.num_rows(),):
+            raise ValueError("b has the wrong length")
+        z = np.empty_like(b)
+        check(L.lib().prx_mg_vcycle(self._tree._h, int(precision), int(smooth), float(omega), ptr(b), ptr(z)))
+        return z
+
     def nnz(self) -> int:
         v = ctypes.c_int64(0)
         check(L.lib().prx_csr_nnz(self._tree._h, ctypes.byref(v)))
@@ -154,8 +164,12 @@ class FEMTree:
     # ------------------------------------------------------------------ solve
     def solve_cascadic(self, points, pw: float, base_depth: int, gs: int, cg: int, eps: float,
                        precision: int = L.PRX_PREC_FP64, residual_mode: int = L.PRX_RESID_REFERENCE,
-                       refresh_period: int = 10, inner_tol: float = 0.1, max_outer: int = 64) -> dict:
-        """src/fem_tree.rs:458-487; returns the solve report."""
+                       refresh_period: int = 10, inner_tol: float = 0.1, max_outer: int = 64,
+                       precond: int = L.PRX_PRECOND_NONE, mg_smooth: int = 2, mg_omega: float = 0.8,
+                       mg_inner_tol: float = 1e-4) -> dict:
+        """src/fem_tree.rs:458-487; returns the solve report.  precond
+        PRX_PRECOND_MG solves the same system with multigrid-preconditioned
+        CG from x = 0 instead (stopping at ||b - A x|| <= eps ||b||)."""
         self._assemble(points, pw)
         o = L.default_solve_opts()
         o.precision = int(precision)
@@ -164,6 +178,10 @@ class FEMTree:
         o.gs_iters = int(gs)
         o.inner_tol = float(inner_tol)
         o.max_outer = int(max_outer)
+        o.precond = int(precond)
+        o.mg_smooth = int(mg_smooth)
+        o.mg_omega = float(mg_omega)
+        o.mg_inner_tol = float(mg_inner_tol)
         rep = L.SolveReport()
         check(L.lib().prx_solve_cascadic(self._h, int(base_depth), int(gs), int(cg), float(eps), ctypes.byref(o),
                                          ctypes.byref(rep)))

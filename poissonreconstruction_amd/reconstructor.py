@@ -33,6 +33,7 @@ class ReconstructParams:
     refresh_period: int = 10
     inner_tol: float = 0.1
     max_outer: int = 64
+    precond: int = L.PRX_PRECOND_NONE
 
     def to_c(self) -> L.Params:
         p = L.Params()
