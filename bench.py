@@ -11,8 +11,9 @@ Inputs (transformed points, point lists, splatted normal field) are resident
 in HBM before the timed region.
 
 Prints one JSON line.  `value` is the mixed-precision ms per solve; the FP64
-GPU solve, the field error against an FP64 solve to 1e-12, the operator
-roofline and the CPU baseline are reported beside it.
+GPU solve, the same system solved by multigrid-preconditioned CG (mixed and
+FP64), the field error against an FP64 solve to 1e-12, the operator roofline
+and the CPU baseline are reported beside it.
 """
 from __future__ import annotations
 
@@ -51,6 +52,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-mg", action="store_true", help="skip the multigrid-preconditioned solves")
     ap.add_argument("--batch-clouds", type=int, default=256)
     ap.add_argument("--op-reps", type=int, default=20)
     ap.add_argument("--large", action="store_true",
@@ -95,10 +97,21 @@ def main():
         L.check(L.lib().prx_solve_cascadic(tree._h, D - 2, 20, 4 * cap, tol, ctypes.byref(o), ctypes.byref(rep)))
         return (time.perf_counter() - t) * 1e3, rep.as_dict()
 
-    def run(prec, name):
+    def solve_mg(prec, tol):
+        o = L.default_solve_opts()
+        o.precision = prec
+        o.residual_mode = L.PRX_RESID_TRUE
+        o.precond = L.PRX_PRECOND_MG
+        rep = L.SolveReport()
+        t = time.perf_counter()
+        L.check(L.lib().prx_assemble(tree._h, pw))
+        L.check(L.lib().prx_solve_cascadic(tree._h, D - 2, 20, 10000, tol, ctypes.byref(o), ctypes.byref(rep)))
+        return (time.perf_counter() - t) * 1e3, rep.as_dict()
+
+    def run(prec, name, fn=solve):
         reps = []
         for i in range(args.warmup + args.steps):
-            ms, rep = solve(prec, args.tol)
+            ms, rep = fn(prec, args.tol)
             log(f"{name} step {i}: {ms:.1f} ms  fine_iters={rep['fine_iters']} outer={rep['outer_iters']} "
                 f"coarse_iters={rep['coarse_iters']} true_rel={rep['true_rel']:.3e} status={rep['status']}")
             if i >= args.warmup:
@@ -128,6 +141,24 @@ def main():
     ms_fp64 = float(np.mean([m for m, _ in fp64])) if fp64 else None
     x_fp64 = tree.solution if fp64 else None
 
+    # the same system solved by multigrid-preconditioned CG (an extension)
+    mg, x_mg = None, {}
+    if not args.skip_mg:
+        mg_runs = {}
+        for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):
+            mg_runs[name] = run(prec, f"mg {name}", fn=solve_mg)
+            x_mg[name] = tree.solution
+        mgm = {k: float(np.mean([m for m, _ in v])) for k, v in mg_runs.items()}
+        keys = ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse", "ms_fine_cg", "ms_total", "applies")
+        mg = {"note": "multigrid-preconditioned CG (V-cycle: damped Jacobi, trilinear transfers, dense coarsest "
+                      "inverse) from x = 0 on the same system to the same true residual; coarse_iters = V-cycles",
+              "mixed_ms": round(mgm["mixed"], 2), "fp64_ms": round(mgm["fp64"], 2),
+              "mixed_speedup_vs_fp64": round(mgm["fp64"] / mgm["mixed"], 3),
+              "speedup_vs_cascadic_mixed": round(ms_mixed / mgm["mixed"], 2),
+              "mixed": {k: mg_runs["mixed"][-1][1][k] for k in keys},
+              "fp64": {k: mg_runs["fp64"][-1][1][k] for k in keys}}
+        log(f"multigrid: {mg}")
+
     field = {}
     if not args.skip_ref:
         ms_ref, rep_ref = solve(L.PRX_PREC_FP64, 1e-12, cap=6000)
@@ -142,6 +173,10 @@ def main():
                  "mixed_err_points": float(np.abs(f_mix - f_ref).max() / rng_f)}
         if x_fp64 is not None:
             field["fp64_err_nodes"] = float(np.abs(x_fp64 - x_ref).max() / rng_f)
+        for name, xs in x_mg.items():
+            tree.solution = xs
+            field[f"mg_{name}_err_nodes"] = float(np.abs(xs - x_ref).max() / rng_f)
+            field[f"mg_{name}_err_points"] = float(np.abs(tree.evaluate_many(q) - f_ref).max() / rng_f)
         log(f"field error vs FP64 1e-12 solve: {field}")
 
     roofline = operator_roofline(L, tree, N, P, args.op_reps)
@@ -168,6 +203,7 @@ def main():
         "fp64": ({k2: fp64[-1][1][k2] for k2 in ("fine_iters", "coarse_iters", "true_rel", "ms_coarse", "ms_fine_gs",
                                                  "ms_fine_cg", "ms_total", "applies")} if fp64 else None),
         "field_error": field or None,
+        "multigrid": mg,
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
         "roofline": roofline,
         "marching_cubes": mc,
@@ -214,6 +250,18 @@ def large_bench(pr, L, synthetic, tol, with_fp64):
         return ms, r
 
     fp64 = solve(L.PRX_PREC_FP64) if with_fp64 else None
+    mg = {}
+    for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):
+        o = L.default_solve_opts()
+        o.precision = prec
+        o.residual_mode = L.PRX_RESID_TRUE
+        o.precond = L.PRX_PRECOND_MG
+        r = L.SolveReport()
+        ts = time.perf_counter()
+        L.check(L.lib().prx_solve_cascadic(tree._h, D - 2, 20, 10000, tol, ctypes.byref(o), ctypes.byref(r)))
+        mg[name] = {"ms": round((time.perf_counter() - ts) * 1e3, 1), "iters": r.fine_iters, "outer": r.outer_iters,
+                    "true_rel": r.true_rel, "status": r.status}
+        log(f"large: multigrid {name} {mg[name]}")
     ms_mixed, rep = solve(L.PRX_PREC_MIXED)
     cms, css = ctypes.c_double(0), ctypes.c_double(0)
     L.check(L.lib().prx_time_evaluate_corners(tree._h, 3, 0, ctypes.byref(cms), ctypes.byref(css)))
@@ -228,6 +276,7 @@ def large_bench(pr, L, synthetic, tol, with_fp64):
                                   f"{tol:g}", "depth": D, "points": P, "nodes": N},
            "mixed": {k: rep[k] for k in ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse",
                                          "ms_fine_gs", "ms_fine_cg", "ms_total", "status")},
+           "multigrid": mg,
            "fp64_ms": round(fp64[0], 1) if fp64 else None,
            "mixed_speedup_vs_fp64": round(fp64[0] / ms_mixed, 3) if fp64 else None,
            "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
@@ -324,35 +373,45 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
 
 
 def operator_roofline(L, tree, N, P, reps):
-    """The dominant kernel's achieved HBM bandwidth: the FP32 stencil apply,
-    8N algorithmic bytes per launch (x read once, y written once), averaged
-    over `reps` launches timed with hipEvents on the library's stream."""
-    g = ctypes.c_double(0)
-    k = ctypes.c_double(0)
-    L.check(L.lib().prx_time_operator(tree._h, L.PRX_PREC_MIXED, reps, ctypes.byref(g), ctypes.byref(k)))
-    g64 = ctypes.c_double(0)
-    k64 = ctypes.c_double(0)
-    L.check(L.lib().prx_time_operator(tree._h, L.PRX_PREC_FP64, reps, ctypes.byref(g64), ctypes.byref(k64)))
+    """The dominant kernel's achieved HBM bandwidth.  The CG iteration's
+    operator pass is the FP32 fused direction update + apply (OP_DIR_APPLY):
+    it reads r and d_old, writes d_new and q = A d_new (16N bytes) and reads
+    the P screening points' positions once (12P).  Launches are timed with
+    hipEvents on the library's stream and averaged over `reps`.  The plain
+    apply y = A x (8N + 12P) and the FP64 forms are reported beside it."""
+    def timed(prec, op):
+        g, k = ctypes.c_double(0), ctypes.c_double(0)
+        L.check(L.lib().prx_time_stencil_op(tree._h, prec, op, reps, ctypes.byref(g), ctypes.byref(k)))
+        return g.value, k.value
+    gd, kd = timed(L.PRX_PREC_MIXED, 1)
+    ga, ka = timed(L.PRX_PREC_MIXED, 0)
+    gd64, kd64 = timed(L.PRX_PREC_FP64, 1)
+    ga64, ka64 = timed(L.PRX_PREC_FP64, 0)
     cms = ctypes.c_double(0)
     copy_bytes = 2 * 4 * N
     L.check(L.lib().prx_copy_bandwidth(copy_bytes, reps, 0, ctypes.byref(cms)))
     copy_gbs = copy_bytes / (cms.value * 1e-3) / 1e9
-    stencil_bytes = 8 * N
-    achieved = stencil_bytes / (k.value * 1e-3) / 1e9
+    dir_bytes, apply_bytes = 16 * N + 12 * P, 8 * N + 12 * P
+    achieved = dir_bytes / (kd * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stencil_apply_depth9_torus1M_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "stencil_dir_apply_depth9_torus1M_traffic.json")
     if N == 8 ** 9 and os.path.exists(tpath):  # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
         with open(tpath) as fh:
             traffic = int(json.load(fh)["traffic_bytes_per_launch"])
+    gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_stencil_tile<float,16,OP_APPLY>", "bytes_per_launch": stencil_bytes,
-                "ms_per_launch": round(k.value, 4), "gather_ms": round(g.value, 4),
-                "apply_fp32_GBs_incl_gather": round((8 * N + 12 * P) / ((k.value + g.value) * 1e-3) / 1e9, 1),
-                "stencil_fp64_ms": round(k64.value, 4),
-                "stencil_fp64_GBs": round(16 * N / (k64.value * 1e-3) / 1e9, 1),
-                "copy_bytes": copy_bytes, "copy_ms": round(cms.value, 4),
-                "copy_GBs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)}
+                "kernel": "k_stencil_tile<float,16,OP_DIR_APPLY>", "bytes_per_launch": dir_bytes,
+                "ms_per_launch": round(kd, 4), "gather_ms": round(gd, 4),
+                "frac_of_copy": round(achieved / copy_gbs, 4),
+                "apply_fp32": {"kernel": "k_stencil_tile<float,16,OP_APPLY>", "bytes": apply_bytes, "ms": round(ka, 4),
+                               "GBs": gbs(apply_bytes, ka), "frac": round(gbs(apply_bytes, ka) / HBM_PEAK_GBS, 4),
+                               "gather_ms": round(ga, 4)},
+                "dir_apply_fp64": {"bytes": 32 * N + 24 * P, "ms": round(kd64, 4), "GBs": gbs(32 * N + 24 * P, kd64),
+                                   "gather_ms": round(gd64, 4)},
+                "apply_fp64": {"bytes": 16 * N + 24 * P, "ms": round(ka64, 4), "GBs": gbs(16 * N + 24 * P, ka64),
+                               "gather_ms": round(ga64, 4)},
+                "copy_bytes": copy_bytes, "copy_ms": round(cms.value, 4), "copy_GBs": round(copy_gbs, 1)}
     log(f"roofline {roofline}")
     return roofline
 

@@ -117,7 +117,7 @@ typedef struct prx_solve_opts {
      * same stopping test as PRX_RESID_TRUE) */
     int32_t precond;        /* PRX_PRECOND_NONE (default) or PRX_PRECOND_MG */
     int32_t mg_smooth;      /* damped-Jacobi sweeps before and after each coarse correction (default 2) */
-    double mg_omega;        /* Jacobi damping (default 0.8) */
+    double mg_omega;        /* Jacobi damping (default 0.9) */
     double mg_inner_tol;    /* mixed MG: FP32 residual reduction per FP64 refinement step (default 1e-4) */
 } prx_solve_opts;
 
@@ -244,6 +244,10 @@ int prx_unit_transform_device(const prx_point* points, size_t n, double scale, i
  * times a streaming device copy of `bytes` (read + write) and returns the
  * average ms per copy. */
 int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
+/* As prx_time_operator for one pass of the CG iteration: op 0 = y = A x,
+ * op 1 = the fused direction update and apply (d_new = r + beta d_old,
+ * q = A d_new, beta = 0.5), each with its screening gather. */
+int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
 /* Measurement: evaluate the current solution at every marching-cubes corner
  * (as prx_evaluate_corners, src/fem_tree.rs:596-605) into a device buffer
@@ -258,7 +262,9 @@ int prx_time_evaluate_corners(prx_tree* t, int reps, int general, double* ms, do
  * converged solution, through the splat, RHS, screening and solve.  Given
  * dL/dF [P] (input point order) it writes dL/dq [3P] and dL/dn [3P] (AoS,
  * unit-cube coordinates, the point transform held fixed).  The adjoint system
- * A lam = dL/dx is solved to ||r|| <= eps ||dL/dx|| in `precision`. */
+ * A lam = dL/dx is solved to ||r|| <= eps ||dL/dx|| in `precision`, with
+ * the solver of the tree's last prx_solve_cascadic (multigrid-preconditioned
+ * CG when that used PRX_PRECOND_MG, CG otherwise). */
 typedef struct prx_vjp_report {
     uint64_t iters;
     double rel;

@@ -27,6 +27,8 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--residual", choices=("reference", "true"), default="reference")
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--multigrid", action="store_true",
+                    help="solve with multigrid-preconditioned CG instead of the cascadic solve (an extension)")
     a = ap.parse_args(argv)
 
     from . import ply
@@ -47,7 +49,7 @@ def main(argv=None) -> int:
                           cg_iters=a.cg_iters, point_weight=a.point_weight, verbose=a.verbose,
                           precision=L.PRX_PREC_MIXED if a.precision == "mixed" else L.PRX_PREC_FP64,
                           residual_mode=L.PRX_RESID_TRUE if a.residual == "true" else L.PRX_RESID_REFERENCE,
-                          device=a.device)
+                          device=a.device, precond=L.PRX_PRECOND_MG if a.multigrid else L.PRX_PRECOND_NONE)
     mesh, s = reconstruct(pts, p)
     if a.verbose:
         print(f"Timing [s]: xform={s.time_transform:.3f} tree={s.time_tree:.3f} sys={s.time_system:.3f} "

@@ -154,7 +154,7 @@ void launch_point_keys_eval(uint32_t D, const double* pos, int64_t P, uint32_t* 
 }
 
 __global__ void k_list_entries(uint32_t D, int mode, const uint32_t* code, int64_t M, const double* __restrict__ pos, int64_t P,
-                               const uint32_t* __restrict__ inv, uint32_t* sp, double* phi, float* phi32) {
+                               const uint32_t* __restrict__ inv, uint32_t* sp, uint32_t* spc, double* phi, float* phi32) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= M) return;
     const uint32_t c = code[j], p = c >> 3;
@@ -163,6 +163,7 @@ __global__ void k_list_entries(uint32_t D, int mode, const uint32_t* code, int64
     phi[j] = w;
     phi32[j] = (float)w;
     sp[j] = inv[p];
+    spc[j] = (inv[p] << 3) | (uint32_t)k;
 }
 __global__ void k_list_diag(const uint32_t* off, int64_t T, const double* phi, double* diag) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,6 +236,7 @@ void build_node_list(NodeList& L, uint32_t D, const uint32_t* key, const uint32_
     dmalloc(&L.off, T + 1);
     dmalloc(&L.code, M);
     dmalloc(&L.sp, M);
+    dmalloc(&L.spc, M);
     dmalloc(&L.phi, M);
     dmalloc(&L.phi32, M);
     dmalloc(&L.diag, T);
@@ -242,7 +244,7 @@ void build_node_list(NodeList& L, uint32_t D, const uint32_t* key, const uint32_
         hipLaunchKernelGGL(k_scatter_heads, dim3(nblk(M)), dim3(256), 0, s, key, head, pos, M, L.node, L.off);
         PRX_HIP(hipMemcpyAsync(L.code, val, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_list_entries, dim3(nblk(M)), dim3(256), 0, s, D, L.mode, L.code, M, pts.pos, pts.P, inv, L.sp,
-                           L.phi, L.phi32);
+                           L.spc, L.phi, L.phi32);
     }
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, L.off + T, (uint32_t)M);
     if (T > 0) hipLaunchKernelGGL(k_list_diag, dim3(nblk(T)), dim3(256), 0, s, L.off, T, L.phi, L.diag);
@@ -326,9 +328,70 @@ __global__ void k_rhs_laplacian(uint32_t D, const double* __restrict__ nf, doubl
             }
     rhs[i] = sum;
 }
+// The same sums with the operands staged in LDS: one workgroup per aligned
+// 8^3 brick (a contiguous Morton range) loads the brick's three normal-field
+// components and the one-node halo, then each thread forms two nodes with
+// exactly the per-node expression and neighbour order above, so the result is
+// bit-identical.
+__global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const double* __restrict__ nf, double* rhs) {
+    constexpr int E = 10, E3 = E * E * E;
+    __shared__ double v[3][E3];
+    const uint32_t res = 1u << D;
+    const uint64_t N = 1ull << (3 * D);
+    const int tid = threadIdx.x;
+    const uint32_t bb = blockIdx.x;
+    const uint64_t base = (uint64_t)bb << 9;
+    uint32_t bx, by, bz;
+    demorton3(bb, bx, by, bz);
+    const int ox = (int)bx * 8, oy = (int)by * 8, oz = (int)bz * 8;
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    // interior: thread t holds local Morton nodes 2t and 2t+1 (same x pair)
+    const uint32_t m0 = 2u * (uint32_t)tid;
+    uint32_t lx, ly, lz;
+    demorton3(m0, lx, ly, lz);
+    const int li = ((int)(lz + 1) * E + (int)ly + 1) * E + (int)lx + 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const d2 q = *(const d2*)(nf + (uint64_t)c * N + base + m0);
+        v[c][li] = q[0];
+        v[c][li + 1] = q[1];
+    }
+    // halo shell (10^3 - 8^3 = 488 nodes); values outside the grid are never read
+    for (int i = tid; i < E3; i += 256) {
+        const int hx = i % E - 1, hy = (i / E) % E - 1, hz = i / (E * E) - 1;
+        if (hx >= 0 && hx < 8 && hy >= 0 && hy < 8 && hz >= 0 && hz < 8) continue;
+        const int gx = ox + hx, gy = oy + hy, gz = oz + hz;
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= (int)res || gy >= (int)res || gz >= (int)res) continue;
+        const uint32_t j = morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c][i] = nf[(uint64_t)c * N + j];
+    }
+    __syncthreads();
+    const double h = 1.0 / (double)res;
+    d2 out;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int x = (int)lx + e, gxc = ox + x, gyc = oy + (int)ly, gzc = oz + (int)lz;
+        double sum = 0.0;
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int jx = gxc + dx, jy = gyc + dy, jz = gzc + dz;
+                    if (jx < 0 || jx >= (int)res || jy < 0 || jy >= (int)res || jz < 0 || jz >= (int)res) continue;
+                    const int k = (((int)lz + 1 + dz) * E + (int)ly + 1 + dy) * E + x + 1 + dx;
+                    const double vx = v[0][k], vy = v[1][k], vz = v[2][k];
+                    sum += h * h * (vx * deriv_mass_1d(dx) * mass_1d(dy) * mass_1d(dz) + vy * mass_1d(dx) * deriv_mass_1d(dy) * mass_1d(dz) +
+                                    vz * mass_1d(dx) * mass_1d(dy) * deriv_mass_1d(dz));
+                }
+        out[e] = sum;
+    }
+    *(d2*)(rhs + base + m0) = out;
+}
+
 void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s) {
     const int64_t N = (int64_t)1 << (3 * D);
-    hipLaunchKernelGGL(k_rhs_laplacian, dim3(nblk(N)), dim3(256), 0, s, D, nf, rhs);
+    if (D >= 3) hipLaunchKernelGGL(k_rhs_laplacian_tile, dim3((unsigned)(N >> 9)), dim3(256), 0, s, D, nf, rhs);
+    else hipLaunchKernelGGL(k_rhs_laplacian, dim3(nblk(N)), dim3(256), 0, s, D, nf, rhs);
     PRX_LAUNCH_CHECK();
 }
 

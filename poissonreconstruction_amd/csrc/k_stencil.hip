@@ -42,6 +42,11 @@ struct LevelArgs {
     const T* phi;
     const double* diag;  // sum phi^2 per touched node
     T wh3;
+    // when set, the tile kernels recompute each entry's weight from its point
+    // (spc = 8*sorted point + corner, tt = the points' in-cell coordinates)
+    const uint32_t* spc;
+    const T* tt;
+    int64_t P;
 };
 
 template <typename T>
@@ -69,6 +74,12 @@ static LevelArgs<T> make_args(const LevelView& L, int tile_log) {
         if constexpr (sizeof(T) == 8) a.phi = (const T*)S.phi;
         else a.phi = (const T*)S.phi32;
         a.diag = S.diag;
+        if (S.spc && L.pts && L.pts->P > 0) {
+            a.spc = S.spc;
+            if constexpr (sizeof(T) == 8) a.tt = (const T*)L.pts->t64;
+            else a.tt = (const T*)L.pts->t32;
+            a.P = L.pts->P;
+        }
     }
     return a;
 }
@@ -450,14 +461,33 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 for (uint32_t c0 = e0; c0 < e1; c0 += CH) {
                     uint32_t spv[4];
                     T phv[4], pv[4];
+                    if (L.spc) {
+                        // phi = bx*by*bz from the point's in-cell coordinates, as the gather forms it
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t j = c0 + u * NT + tid;
-                        spv[u] = j < e1 ? L.sp[j] : 0u;
-                        phv[u] = j < e1 ? L.phi[j] : T(0);
+                        for (int u = 0; u < 4; ++u) {
+                            const uint32_t j = c0 + u * NT + tid;
+                            spv[u] = j < e1 ? L.spc[j] : 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const uint32_t q = spv[u] >> 3, k = spv[u] & 7u;
+                            const T tx = L.tt[q], ty = L.tt[L.P + q], tz = L.tt[2 * L.P + q];
+                            const T bx = (k & 1u) ? tx : T(1) - tx;
+                            const T by = (k & 2u) ? ty : T(1) - ty;
+                            const T bz = (k & 4u) ? tz : T(1) - tz;
+                            phv[u] = bx * by * bz;
+                            pv[u] = c0 + u * NT + tid < e1 ? phv[u] * io.s[q] : T(0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const uint32_t j = c0 + u * NT + tid;
+                            spv[u] = j < e1 ? L.sp[j] : 0u;
+                            phv[u] = j < e1 ? L.phi[j] : T(0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) pv[u] = c0 + u * NT + tid < e1 ? phv[u] * io.s[spv[u]] : T(0);
                     }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) pv[u] = c0 + u * NT + tid < e1 ? phv[u] * io.s[spv[u]] : T(0);
                     __syncthreads();  // the previous chunk has been read
 #pragma unroll
                     for (int u = 0; u < 4; ++u) prod[u * NT + tid] = pv[u];

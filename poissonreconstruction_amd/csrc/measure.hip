@@ -9,7 +9,7 @@
 namespace prx {
 void host_unit_transform(const prx_point* p, size_t n, double scale, double m[16], double mi[16]);
 void host_apply_transform(const prx_point* in, size_t n, const double m[16], prx_point* out);
-int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
+int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil, int op);
 
 // Streaming copy: one 16-byte vector per thread over a flat grid,
 // non-temporal loads and stores (the fastest copy shape measured on gfx950).
@@ -33,7 +33,12 @@ int prx_unit_transform(const prx_point* points, size_t n, double scale, double* 
 
 int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil) {
     if (!t || reps <= 0) return PRX_ERR_INVALID;
-    return prx::time_operator(t, precision, reps, ms_gather, ms_stencil);
+    return prx::time_operator(t, precision, reps, ms_gather, ms_stencil, prx::OP_APPLY);
+}
+
+int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil) {
+    if (!t || reps <= 0 || (op != 0 && op != 1)) return PRX_ERR_INVALID;
+    return prx::time_operator(t, precision, reps, ms_gather, ms_stencil, op == 0 ? prx::OP_APPLY : prx::OP_DIR_APPLY);
 }
 
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms) {

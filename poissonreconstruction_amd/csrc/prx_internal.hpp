@@ -121,6 +121,7 @@ struct NodeList {
     uint32_t* off = nullptr;      // [T+1] entry offsets
     uint32_t* code = nullptr;     // [M] 8*p + corner, p = input point index
     uint32_t* sp = nullptr;       // [M] sorted position of the point (screening lookups)
+    uint32_t* spc = nullptr;      // [M] 8*sp + corner (the weight is recomputed from the point)
     double* phi = nullptr;        // [M] trilinear weight (bx*by)*bz
     float* phi32 = nullptr;       // [M]
     double* diag = nullptr;       // [T] sum of phi^2 (screening diagonal without wh3)

@@ -703,7 +703,7 @@ static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b
 }
 
 static int mg_nu(const prx_solve_opts& o) { return o.mg_smooth > 0 ? o.mg_smooth : 2; }
-static double mg_omega(const prx_solve_opts& o) { return o.mg_omega > 0.0 && o.mg_omega < 1.3 ? o.mg_omega : 0.8; }
+static double mg_omega(const prx_solve_opts& o) { return o.mg_omega > 0.0 && o.mg_omega < 1.3 ? o.mg_omega : 0.9; }
 
 // FP64 PCG from x = 0 to ||b - A x|| <= eps ||b||, confirmed on the true
 // residual (a PCG restart from the current x if the recursive residual drifted).
@@ -809,6 +809,7 @@ static void solve_mg(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
 static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
     std::memset(&rep, 0, sizeof rep);
     t->ensure_state();
+    t->precond = c.o.precond;
     if (c.o.precond == PRX_PRECOND_MG) {
         solve_mg(t, c, rep);
         return;
@@ -1000,8 +1001,15 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
         c.o.refresh_period = 0;
         c.o.inner_tol = 0.1;
         c.o.max_outer = 200;
-        if (precision == PRX_PREC_MIXED) fine_solve_mixed(t, c, 200000, r);
-        else fine_solve_fp64(t, c, 200000, r);
+        if (t->precond == PRX_PRECOND_MG) {
+            c.o.precond = PRX_PRECOND_MG;
+            if (precision == PRX_PREC_MIXED) mg_solve_mixed(t, c, 10000, r);
+            else mg_solve_fp64(t, c, 10000, r);
+        } else if (precision == PRX_PREC_MIXED) {
+            fine_solve_mixed(t, c, 200000, r);
+        } else {
+            fine_solve_fp64(t, c, 200000, r);
+        }
     } catch (...) {
         t->x.swap(t->adj_lam);
         t->b.swap(t->adj_g);
@@ -1028,7 +1036,7 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
     }
 }
 
-int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil) {
+int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil, int op) {
     try {
         if (!t->assembled) return PRX_ERR_STATE;
         set_device(t);
@@ -1041,19 +1049,27 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
         PRX_HIP(hipEventCreate(&e1));
         PRX_HIP(hipEventCreate(&e2));
         double g = 0.0, k = 0.0;
-        auto run = [&](auto* xin, auto* yout, auto* sb) {
+        // OP_DIR_APPLY: r = xin, d_old = dold, d_new = dnew (beta from the CG state)
+        auto run = [&](auto* xin, auto* yout, auto* sb, auto* dold, auto* dnew) {
             using T = std::remove_pointer_t<decltype(xin)>;
             StencilIO<T> io;
             io.x = xin;
             io.y = yout;
             io.partial = t->partial.p;
             io.s = sb;
+            io.r = xin;
+            io.dold = dold;
+            io.dnew = dnew;
+            io.beta = &t->st->beta;
             int nb = 0;
             for (int r = -1; r < reps; ++r) {  // r = -1 warms up
                 PRX_HIP(hipEventRecord(e0, s));
-                if (L.scr) launch_screen_gather<T>(L, xin, nullptr, nullptr, sb, s);
+                if (L.scr) {
+                    if (op == OP_DIR_APPLY) launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, xin, dold, &t->st->beta);
+                    else launch_screen_gather<T>(L, xin, nullptr, nullptr, sb, s);
+                }
                 PRX_HIP(hipEventRecord(e1, s));
-                launch_stencil<T>(L, OP_APPLY, io, &nb, t->tile, s);
+                launch_stencil<T>(L, op, io, &nb, t->tile, s);
                 PRX_HIP(hipEventRecord(e2, s));
                 PRX_HIP(hipEventSynchronize(e2));
                 float a = 0.f, b = 0.f;
@@ -1062,16 +1078,33 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
                 if (r >= 0) { g += a; k += b; }
             }
         };
+        if (op != OP_APPLY && op != OP_DIR_APPLY) return PRX_ERR_INVALID;
+        if (op == OP_DIR_APPLY) {  // beta = 0.5, not done
+            CgState h{};
+            h.beta = 0.5;
+            *t->st_host = h;
+            PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
+        }
         if (precision == PRX_PREC_MIXED) {
-            t->d32.ensure(n);
+            t->d32.ensure(2 * n);
             t->q32.ensure(n);
-            launch_fill<float>(t->d32.p, 1.0f, n, s);
-            run(t->d32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr);
+            t->ri32.ensure(n);
+            launch_fill<float>(t->d32.p, 1.0f, 2 * n, s);
+            launch_fill<float>(t->ri32.p, 0.5f, n, s);
+            if (op == OP_DIR_APPLY)
+                run(t->ri32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr, t->d32.p, t->d32.p + n);
+            else
+                run(t->d32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr, (float*)nullptr, (float*)nullptr);
         } else {
-            t->d64.ensure(n);
+            t->d64.ensure(2 * n);
             t->q64.ensure(n);
-            launch_fill<double>(t->d64.p, 1.0, n, s);
-            run(t->d64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr);
+            t->r64.ensure(n);
+            launch_fill<double>(t->d64.p, 1.0, 2 * n, s);
+            launch_fill<double>(t->r64.p, 0.5, n, s);
+            if (op == OP_DIR_APPLY)
+                run(t->r64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr, t->d64.p, t->d64.p + n);
+            else
+                run(t->d64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr, (double*)nullptr, (double*)nullptr);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1131,7 +1164,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->max_outer = 64;
     o->precond = PRX_PRECOND_NONE;
     o->mg_smooth = 2;
-    o->mg_omega = 0.8;
+    o->mg_omega = 0.9;
     o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
 }

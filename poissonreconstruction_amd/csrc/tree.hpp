@@ -38,7 +38,7 @@ struct DArr {
 };
 
 inline void free_list(NodeList& L) {
-    void* ptrs[] = {L.node, L.off, L.code, L.sp, L.phi, L.phi32, L.diag, L.brick, L.brick8};
+    void* ptrs[] = {L.node, L.off, L.code, L.sp, L.spc, L.phi, L.phi32, L.diag, L.brick, L.brick8};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     int mode = L.mode;
@@ -126,6 +126,7 @@ struct prx_tree {
     // the inverse of the coarsest operator
     DArr<double> mg64, mg_ainv;
     DArr<float> mg32;
+    int precond = 0;  // solver of the last solve (the adjoint solve uses the same)
 
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
