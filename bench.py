@@ -313,6 +313,20 @@ def adjoint_bench(pr, L, synthetic, tol):
     out.update({"forward_ms": round(float(np.median(fwd[1:])), 2), "vjp_ms": round(float(np.median(bwd[1:])), 2),
                 "vjp_device_ms": round(vrep["ms"], 2), "vjp_iters": vrep["iters"], "vjp_rel": vrep["rel"],
                 "forward_fine_iters": rep["fine_iters"], "tol": tol})
+    # the same with multigrid-preconditioned CG for the forward and the adjoint solve
+    fwd, bwd = [], []
+    for i in range(3):
+        ts = time.perf_counter()
+        rep = t.solve_cascadic(unit, pw, D - 2, 20, 10000, tol, precision=L.PRX_PREC_MIXED,
+                               residual_mode=L.PRX_RESID_TRUE, precond=L.PRX_PRECOND_MG)
+        fwd.append((time.perf_counter() - ts) * 1e3)
+        F = t.evaluate_many(q)
+        ts = time.perf_counter()
+        gpos, gnrm, vrep = t.vjp_points(2.0 * (F - F.mean()), precision=L.PRX_PREC_MIXED, eps=tol)
+        bwd.append((time.perf_counter() - ts) * 1e3)
+    out["multigrid"] = {"forward_ms": round(float(np.median(fwd[1:])), 2), "vjp_ms": round(float(np.median(bwd[1:])), 2),
+                        "vjp_device_ms": round(vrep["ms"], 2), "vjp_iters": vrep["iters"], "vjp_rel": vrep["rel"],
+                        "forward_iters": rep["fine_iters"]}
     log(f"adjoint {out}")
     return out
 
