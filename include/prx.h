@@ -284,7 +284,10 @@ int prx_vjp_points(prx_tree* t, const double* dL_dF, double* dL_dpos, double* dL
  * concatenated node array, with the CG iterations and GS sweeps replayed as
  * captured HIP graphs.  FP64 only.  With autotune != 0 the first solve times
  * the operator brick size, the GS block size and the CG chunk length on the
- * batch and keeps the fastest (reported).  Instance trees (prx_batch_tree)
+ * batch and keeps the fastest (reported).  With opts->precond =
+ * PRX_PRECOND_MG every instance is instead solved by multigrid-preconditioned
+ * CG (FP64 or mixed) to ||b-Ax|| <= eps ||b||, the instances spread over up to
+ * four streams; gs and autotune are then unused.  Instance trees (prx_batch_tree)
  * are borrowed: evaluate / get_solution / vjp work on them; never destroy
  * them.  per_instance (optional) receives one report per instance. */
 typedef struct prx_batch prx_batch;

@@ -20,6 +20,9 @@
 #include <chrono>
 #include <cmath>
 #include <functional>
+#include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "tree.hpp"
@@ -559,10 +562,92 @@ static void autotune(prx_batch* B) {
 }
 
 // ------------------------------------------------------------------ the batched cascadic solve
+// Multigrid-preconditioned CG (PRX_PRECOND_MG) on every instance.  Each
+// instance is a complete assembled tree, solved by the single-tree path to
+// ||b - A x|| <= eps ||b||; the instances are spread over up to 4 streams
+// (one host thread each, so one instance's host round trips overlap the
+// others' kernels), largest first onto the least-loaded stream.
+static void solve_batched_mg(prx_batch* B, uint64_t cg_iters, double eps, const prx_solve_opts& o, prx_batch_report* rep,
+                             prx_solve_report* per) {
+    const int K = (int)B->inst.size();
+    const int S = std::min(K, 4);
+    for (prx_tree* t : B->inst)
+        if (!t->assembled) throw Error(PRX_ERR_STATE, "batch instance not assembled");
+    std::vector<int> order(K);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->inst[a]->N > B->inst[b]->N; });
+    std::vector<std::vector<int>> work(S);
+    std::vector<int64_t> load(S, 0);
+    for (int k : order) {
+        const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        work[s].push_back(k);
+        load[s] += B->inst[k]->N;
+    }
+    std::vector<hipStream_t> streams(S, nullptr);
+    for (auto& st : streams) PRX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::vector<prx_solve_report> reps(K);
+    std::vector<int> rc(K, PRX_OK);
+    std::vector<std::string> errs(S);
+    PRX_HIP(hipStreamSynchronize(B->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> threads;
+    for (int s = 0; s < S; ++s)
+        threads.emplace_back([&, s] {
+            for (int k : work[s]) {
+                prx_tree* t = B->inst[k];
+                const hipStream_t keep = t->stream;
+                t->stream = streams[s];
+                prx_solve_opts oo = o;
+                oo.precond = PRX_PRECOND_MG;
+                oo.residual_mode = PRX_RESID_TRUE;
+                rc[k] = prx_solve_cascadic(t, t->D > 2 ? t->D - 2 : 0, 20, cg_iters, eps, &oo, &reps[k]);
+                if (rc[k] != PRX_OK) errs[s] = prx_last_error();
+                (void)hipStreamSynchronize(streams[s]);
+                t->stream = keep;
+                if (rc[k] != PRX_OK) break;
+            }
+        });
+    for (auto& th : threads) th.join();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (auto& st : streams) (void)hipStreamDestroy(st);
+    for (int k = 0; k < K; ++k)
+        if (rc[k] != PRX_OK) {
+            std::string msg;
+            for (const auto& e : errs) if (!e.empty()) { msg = e; break; }
+            throw Error(rc[k], "batch instance " + std::to_string(k) + ": " + msg);
+        }
+    prx_batch_report R{};
+    R.instances = (uint32_t)K;
+    R.status = PRX_OK;
+    R.ms_total = ms;
+    R.tile = B->tile;
+    R.chunk = B->chunk;
+    R.gs_block = B->gs_block;
+    for (int k = 0; k < K; ++k) {
+        const prx_solve_report& r = reps[k];
+        R.total_nodes += (uint64_t)B->inst[k]->N;
+        R.total_points += (uint64_t)B->inst[k]->P;
+        R.max_true_rel = std::max(R.max_true_rel, r.true_rel);
+        R.max_fine_iters = std::max<uint64_t>(R.max_fine_iters, r.fine_iters);
+        R.sum_fine_iters += r.fine_iters;
+        R.max_coarse_iters = std::max<uint64_t>(R.max_coarse_iters, r.coarse_iters);
+        R.ms_coarse += r.ms_coarse;
+        R.ms_fine_cg += r.ms_fine_cg;
+        if (r.status == PRX_OK) R.converged++;
+        else if (R.status == PRX_OK) R.status = r.status;
+        if (per) per[k] = r;
+    }
+    if (rep) *rep = R;
+}
+
 static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double eps, const prx_solve_opts& o, int tune,
                           prx_batch_report* rep, prx_solve_report* per) {
     if (B->inst.empty()) throw Error(PRX_ERR_STATE, "empty batch");
-    if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched solve runs in FP64");
+    if (o.precond == PRX_PRECOND_MG) {
+        solve_batched_mg(B, cg_iters, eps, o, rep, per);
+        return;
+    }
+    if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched cascadic solve runs in FP64");
     build(B);
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;

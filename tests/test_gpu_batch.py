@@ -93,3 +93,24 @@ def test_batch_rejects_what_it_does_not_do(gpu):
     o.precision = L.PRX_PREC_MIXED
     rc = L.lib().prx_solve_batched(B._h, 20, 1000, 1e-6, ctypes.byref(o), 0, None, None)
     assert rc == L.PRX_ERR_INVALID
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_batch_multigrid_matches_single_tree_multigrid(gpu, prec):
+    # each instance runs the single-tree multigrid solve on one of several
+    # streams: the same arithmetic, so the same bits and iteration counts
+    cs = clouds()
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    for rnd in range(2):
+        rep, per = B.solve(eps=1e-10, precond=pr.PRX_PRECOND_MG, precision=prec)
+        assert rep["instances"] == len(cs) and rep["converged"] == len(cs) and rep["status"] == pr.PRX_OK
+        assert rep["max_true_rel"] <= 1e-10 and rep["max_fine_iters"] <= 80
+        for k, (D, pts, pw) in enumerate(cs):
+            t = pr.FEMTree(D)
+            t.splat_normal_field(pts)
+            rs = t.solve_cascadic(pts, pw, D - 2, 20, 800000, 1e-10, precision=prec, residual_mode=pr.PRX_RESID_TRUE,
+                                  precond=pr.PRX_PRECOND_MG)
+            assert per[k]["fine_iters"] == rs["fine_iters"], (rnd, k)
+            assert np.array_equal(B.tree(k).solution, t.solution), (rnd, k)
