@@ -72,7 +72,7 @@ typedef struct prx_params {
     int32_t gs_iters;       /* smoothing sweeps per level (reference: 20) */
     int32_t base_depth;     /* -1: D-2 rule of src/reconstructor.rs:51 */
     int32_t refresh_period; /* CG true-residual refresh period (reference: 10; 0 = never) */
-    double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.1) */
+    double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.02) */
     uint64_t max_outer;     /* mixed: cap on FP64 residual replacements (default 64) */
     int32_t precond;        /* PRX_PRECOND_NONE (default: the reference's cascadic solve) or PRX_PRECOND_MG */
 } prx_params;
@@ -297,7 +297,7 @@ typedef struct prx_batch_report {
     uint64_t max_fine_iters, sum_fine_iters, max_coarse_iters, graph_launches;
     double ms_total, ms_coarse, ms_fine_gs, ms_fine_cg, ms_tune;
     double max_true_rel;   /* max over instances of ||b-Ax||/||b|| */
-    int32_t tile, chunk, gs_block, status;
+    int32_t tile, chunk, gs_block, status; /* operator brick edge, CG iterations per graph, GS brick edge */
 } prx_batch_report;
 int prx_batch_create(int device, prx_batch** out);
 void prx_batch_destroy(prx_batch* b);

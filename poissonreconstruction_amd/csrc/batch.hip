@@ -102,7 +102,7 @@ struct prx_batch {
     prx::DArr<double> partial;
     prx::DArr<int32_t> flags, ones;
     std::vector<int32_t> flags_h;
-    int tile = 16, chunk = 16, gs_block = 256;
+    int tile = 16, chunk = 16, gs_block = 16;  // gs_block: brick edge of the GS passes
     bool tuned = false;
     double tune_ms = 0.0;
     uint64_t graph_launches = 0;
@@ -309,12 +309,20 @@ static void resid_pass(prx_batch* B, BLevel& L, const BatchTab& tab) {
     launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_RESID, io, B->tile, B->stream);
 }
 
+// Multicolour GS sweeps of every instance: one brick-tiled pass per colour,
+// the same pass (and arithmetic) as the single-tree sweep.
 static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
     const BatchTab tab = tab_all(B, L);
+    StencilIO<double> io;
+    io.x = L.x.p;
+    io.y = L.x.p;
+    io.b = L.b.p;
+    io.s = L.sbuf.p;
     for (int k = 0; k < sweeps; ++k)
         for (int c = 0; c < 8; ++c) {
             if (L.has_scr) launch_bscreen_gather<double>(tab, L.pts(), L.x.p, nullptr, nullptr, L.sbuf.p, nullptr, nullptr, B->stream);
-            launch_bgs_color<double>(tab, L.N, L.scr_ptr(), c, L.x.p, L.b.p, L.sbuf.p, B->gs_block, B->stream);
+            io.color = c;
+            launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_GS, io, B->gs_block, B->stream);
         }
 }
 
@@ -532,12 +540,17 @@ static void autotune(prx_batch* B) {
         const double ms = time_ms(s, 3, [&] { launch_bstencil<double>(all, L.N, L.scr_ptr(), OP_APPLY, io, tile, s); });
         if (ms < best) { best = ms; B->tile = tile; }
     }
-    // GS colour pass on a scratch copy of x
+    // GS colour pass (tiled, in place) on a scratch copy of x: its brick edge
     PRX_HIP(hipMemcpyAsync(L.q.p, L.x.p, sizeof(double) * L.N, hipMemcpyDeviceToDevice, s));
+    StencilIO<double> gio;
+    gio.x = L.q.p;
+    gio.y = L.q.p;
+    gio.b = L.b.p;
+    gio.s = L.sbuf.p;
     best = INFINITY;
-    for (int bs : {64, 128, 256, 512}) {
-        const double ms = time_ms(s, 3, [&] { launch_bgs_color<double>(all, L.N, L.scr_ptr(), 0, L.q.p, L.b.p, L.sbuf.p, bs, s); });
-        if (ms < best) { best = ms; B->gs_block = bs; }
+    for (int gt : {16, 8}) {
+        const double ms = time_ms(s, 3, [&] { launch_bstencil<double>(all, L.N, L.scr_ptr(), OP_GS, gio, gt, s); });
+        if (ms < best) { best = ms; B->gs_block = gt; }
     }
     // host round trip and the cost of an iteration whose instances are all done
     const double t_sync = time_ms(s, 5, [&] {
@@ -665,7 +678,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     }
     if (!tune && !B->tuned) {
         B->tile = 16;
-        B->gs_block = 256;
+        B->gs_block = 16;
         B->chunk = 16;
     }
     const int mode = o.residual_mode == PRX_RESID_TRUE ? PRX_RESID_TRUE : PRX_RESID_REFERENCE;

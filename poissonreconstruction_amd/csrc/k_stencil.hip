@@ -208,7 +208,10 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     constexpr int VEC = 16 / (int)sizeof(T), PER = B3 / NT, NVEC = PER / VEC, NH = (S + NT - 1) / NT;
     __shared__ T tile[E3];
     __shared__ double wsum[(NT + 63) / 64];
+    __shared__ T gdiag[OP == OP_GS ? B3 / 8 : 1];  // screening diagonal of the colour's touched nodes
     const int tid = threadIdx.x;
+    if constexpr (OP == OP_GS)
+        for (int i = tid; i < B3 / 8; i += NT) gdiag[i] = T(0);
     uint32_t bid = blockIdx.x;
     if (swz) {
         const uint32_t per = nbricks >> 3;  // XCD-contiguous brick ranges
@@ -223,8 +226,8 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
         const BCoef c = tab.coef[inst];
         res = c.res;
-        if constexpr (sizeof(T) == 8) { L.s = (T)c.s64; L.wg = (T)c.wg64; }
-        else { L.s = (T)c.s32; L.wg = (T)c.wg32; }
+        if constexpr (sizeof(T) == 8) { L.s = (T)c.s64; L.wg = (T)c.wg64; L.wh3 = (T)c.wh3_64; }
+        else { L.s = (T)c.s32; L.wg = (T)c.wg32; L.wh3 = (T)c.wh3_32; }
         ib = tab.off[inst];
         lb = bid - (uint32_t)(ib / B3);
         if constexpr (OP == OP_DIR_APPLY) bt = tab.st[inst].beta;
@@ -234,6 +237,12 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     uint32_t ox, oy, oz;
     demorton3(lb, ox, oy, oz);
     ox *= B; oy *= B; oz *= B;
+    // the brick's touched-node range (screening), needed early by the prefetch below
+    uint32_t st0 = 0, st1 = 0;
+    if (L.has_screen) {
+        st0 = L.brick[bid];
+        st1 = L.brick[bid + 1];
+    }
 
     // ---- 1. issue the brick's interior loads (16-byte, coalesced)
     NV qa[NVEC], qb[NVEC];
@@ -345,6 +354,31 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             }
         }
     }
+    // ---- 2b. screening prefetch (bricks with touched nodes): the first
+    // group's node slots and the first chunk of its entries are requested
+    // now, their point gathers before the sweep, so the chain of dependent
+    // loads overlaps the stencil work instead of following it
+    constexpr int CH = 4 * NT;
+    const bool pre = L.has_screen && L.spc && st0 < st1;
+    uint32_t pob[4], poe[4], pnd[4], pspv[4], pe0 = 0, pe1 = 0;
+    T ppv[4];
+    if (pre) {
+        const uint32_t g1 = min(st1, st0 + (uint32_t)CH);
+        pe0 = L.off[st0];
+        pe1 = L.off[g1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t t = st0 + k * NT + tid;
+            pob[k] = t < g1 ? L.off[t] : 0u;
+            poe[k] = t < g1 ? L.off[t + 1] : 0u;
+            pnd[k] = t < g1 ? L.node[t] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t j = pe0 + u * NT + tid;
+            pspv[u] = j < pe1 ? L.spc[j] : 0u;
+        }
+    }
     // brick reference value r0 (the brick's first node, formed as the node is)
     T r0;
     if constexpr (OP == OP_RESID_UPD) r0 = io.x64[base] + (double)io.e32[base];
@@ -397,6 +431,18 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         for (int k = 0; k < NH; ++k)
             if (hl[k] >= 0) tile[hl[k]] = hin[k] ? hv[k] - r0 : -r0;
     }
+    if (pre) {  // the prefetched chunk's point gathers (products phi * s_p)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t q = pspv[u] >> 3, k = pspv[u] & 7u;
+            const T tx = L.tt[q], ty = L.tt[L.P + q], tz = L.tt[2 * L.P + q];
+            const T bx = (k & 1u) ? tx : T(1) - tx;
+            const T by = (k & 2u) ? ty : T(1) - ty;
+            const T bz = (k & 4u) ? tz : T(1) - tz;
+            const T ph = bx * by * bz;
+            ppv[u] = pe0 + u * NT + tid < pe1 ? ph * io.s[q] : T(0);
+        }
+    }
     __syncthreads();
 
     // ---- 4. z sweep with a 3-plane register window (values are x - r0)
@@ -441,27 +487,30 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     // ascending order: the sum is deterministic and no thread waits on a
     // chain of dependent global loads.
     if (L.has_screen) {
-        const uint32_t t0 = L.brick[bid], t1 = L.brick[bid + 1];
+        const uint32_t t0 = st0, t1 = st1;
         if (t0 < t1) {
-            constexpr int CH = 4 * NT;
             static_assert(E3 - B3 >= CH, "the spare LDS holds one chunk");
             T* prod = tile + B3;
             for (uint32_t g0 = t0; g0 < t1; g0 += CH) {
                 const uint32_t g1 = min(t1, g0 + (uint32_t)CH);
+                const bool first = pre && g0 == t0;  // uniform
                 uint32_t ob[4], oe[4];
                 T acc[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t t = g0 + k * NT + tid;
-                    ob[k] = t < g1 ? L.off[t] : 0u;
-                    oe[k] = t < g1 ? L.off[t + 1] : 0u;
+                    ob[k] = first ? pob[k] : (t < g1 ? L.off[t] : 0u);
+                    oe[k] = first ? poe[k] : (t < g1 ? L.off[t + 1] : 0u);
                     acc[k] = T(0);
                 }
-                const uint32_t e0 = L.off[g0], e1 = L.off[g1];
+                const uint32_t e0 = first ? pe0 : L.off[g0], e1 = first ? pe1 : L.off[g1];
                 for (uint32_t c0 = e0; c0 < e1; c0 += CH) {
                     uint32_t spv[4];
                     T phv[4], pv[4];
-                    if (L.spc) {
+                    if (first && c0 == e0) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) pv[u] = ppv[u];
+                    } else if (L.spc) {
                         // phi = bx*by*bz from the point's in-cell coordinates, as the gather forms it
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
@@ -500,7 +549,12 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t t = g0 + k * NT + tid;
-                    if (t < g1) tile[stage_at<B>((uint32_t)(L.node[t] - base))] += acc[k];
+                    if (t < g1) {
+                        const uint32_t loc = (uint32_t)((first ? pnd[k] : L.node[t]) - base);
+                        tile[stage_at<B>(loc)] += acc[k];
+                        if constexpr (OP == OP_GS)
+                            if ((int)(loc & 7u) == io.color) gdiag[loc >> 3] = L.wh3 * (T)L.diag[t];
+                    }
                 }
             }
             __syncthreads();
@@ -541,6 +595,17 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 tacc += bb[e] * q[e];
             }
             *(NV*)(io.y + g) = q;
+        } else if constexpr (OP == OP_GS) {
+            // the colour's nodes only (local Morton bits 0..2 are the coordinate parities)
+            const T dg0 = T(32) * L.s + L.wg;  // the diagonal of L + wg I
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const uint32_t me = m + (uint32_t)e;
+                if ((int)(me & 7u) == io.color) {
+                    const T dg = dg0 + gdiag[me >> 3];
+                    if (dg != T(0)) io.y[g + e] = xin[v * VEC + e] + (io.b[g + e] - tile[stage_at<B>(m) + e]) / dg;
+                }
+            }
         } else if constexpr (OP == OP_JACOBI2) {
             const T j2 = (T)(2.0 * io.jw), jj = (T)(io.jw * io.jw);
             NV q;
@@ -561,7 +626,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             tacc += q0 * q0 + q1 * q1;
         }
     }
-    if constexpr (OP == OP_JACOBI2) return;
+    if constexpr (OP == OP_JACOBI2 || OP == OP_GS) return;
     if constexpr (OP == OP_JACOBI) {
         if (!io.partial) return;  // uniform over the block
     }
@@ -705,6 +770,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         else if (op == OP_RESID) PRX_TILE_LAUNCH(16, OP_RESID);
         else if (op == OP_JACOBI) PRX_TILE_LAUNCH(16, OP_JACOBI);
         else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(16, OP_JACOBI2);
+        else if (op == OP_GS) PRX_TILE_LAUNCH(16, OP_GS);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(16, OP_RESID_UPD);
     } else {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(8, OP_APPLY);
@@ -712,6 +778,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         else if (op == OP_RESID) PRX_TILE_LAUNCH(8, OP_RESID);
         else if (op == OP_JACOBI) PRX_TILE_LAUNCH(8, OP_JACOBI);
         else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(8, OP_JACOBI2);
+        else if (op == OP_GS) PRX_TILE_LAUNCH(8, OP_GS);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(8, OP_RESID_UPD);
     }
 #undef PRX_TILE_LAUNCH
@@ -782,11 +849,13 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
         if (op == OP_APPLY) PRX_BTILE_LAUNCH(16, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(16, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(16, OP_RESID);
+        else if (op == OP_GS) PRX_BTILE_LAUNCH(16, OP_GS);
         else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     } else {
         if (op == OP_APPLY) PRX_BTILE_LAUNCH(8, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(8, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(8, OP_RESID);
+        else if (op == OP_GS) PRX_BTILE_LAUNCH(8, OP_GS);
         else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     }
 #undef PRX_BTILE_LAUNCH
@@ -795,46 +864,5 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
 template void launch_bstencil<double>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<double>&, int, hipStream_t);
 template void launch_bstencil<float>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<float>&, int, hipStream_t);
 
-// Multicolour GS over a batch: thread j handles node j & 511 of colour
-// `color` inside 16^3 brick j >> 9 (each brick holds 512 nodes per colour).
-template <typename T>
-__global__ void k_bgs_color(LevelArgs<T> L, BatchTab tab, int color, T* x, const T* b, const T* s, uint64_t nthreads) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nthreads) return;
-    const uint64_t gb = j >> 9;
-    const uint32_t inst = tab.brick_inst[gb];
-    if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
-    const BCoef c = tab.coef[inst];
-    L.res = c.res;
-    T sc;
-    if constexpr (sizeof(T) == 8) { sc = (T)c.s64; L.wg = (T)c.wg64; L.wh3 = (T)c.wh3_64; }
-    else { sc = (T)c.s32; L.wg = (T)c.wg32; L.wh3 = (T)c.wh3_32; }
-    L.s = sc;
-    L.cc = -sc;
-    L.ce = T(-2) * sc;
-    L.c0 = T(32) * sc;
-    const uint64_t ib = tab.off[inst];
-    uint32_t bx, by, bz, X, Y, Z;
-    demorton3((uint32_t)(gb - (ib >> 12)), bx, by, bz);
-    demorton3((uint32_t)(j & 511u), X, Y, Z);
-    const uint32_t ix = 16 * bx + 2 * X + (color & 1), iy = 16 * by + 2 * Y + ((color >> 1) & 1), iz = 16 * bz + 2 * Z + (color >> 2);
-    const uint64_t i = ib + morton3(ix, iy, iz);
-    T dg;
-    const T a = apply_node<T, 0>(L, x, nullptr, nullptr, s, ix, iy, iz, (uint32_t)i, &dg, nullptr, 0.0, ib);
-    if (dg != T(0)) x[i] = x[i] + (b[i] - a) / dg;
-}
-template <typename T>
-void launch_bgs_color(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int color, T* x, const T* b, const T* sbuf,
-                      int block, hipStream_t s) {
-    const LevelArgs<T> a = batch_args<T>(scr, 12);
-    const uint64_t nt = (uint64_t)Ntot / 8;
-    const int bs = (block == 64 || block == 128 || block == 512) ? block : 256;
-    hipLaunchKernelGGL((k_bgs_color<T>), dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0, s, a, tab, color, x, b, sbuf, nt);
-    PRX_LAUNCH_CHECK();
-}
-template void launch_bgs_color<double>(const BatchTab&, int64_t, const NodeList*, int, double*, const double*, const double*, int,
-                                       hipStream_t);
-template void launch_bgs_color<float>(const BatchTab&, int64_t, const NodeList*, int, float*, const float*, const float*, int,
-                                      hipStream_t);
 
 }  // namespace prx

@@ -59,7 +59,10 @@ struct LevelView {
 // a_ii constant), with partial sums of b.y when io.partial is set.
 // OP_JACOBI2: the first two such steps from x = 0 in one pass over b (the
 // operand io.x = b): y = 2 jw b - jw^2 A b.
-enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5 };
+// OP_GS: one colour (io.color = coordinate parities x | y<<1 | z<<2) of the
+// multicolour Gauss-Seidel sweep, in place: x_i += (b_i - (A x)_i) / a_ii for
+// the nodes of that colour (io.x = io.y = x).
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5, OP_GS = 6 };
 
 // ---- k_grid.hip
 void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
@@ -147,6 +150,7 @@ struct StencilIO {
     double* partial = nullptr;
     const int32_t* done = nullptr;
     double jw = 0.0;  // OP_JACOBI weight
+    int color = 0;    // OP_GS colour
 };
 template <typename T>
 void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);
@@ -191,9 +195,6 @@ struct BPoints {
 template <typename T>
 void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int op, const StencilIO<T>& io, int tile,
                      hipStream_t s);
-template <typename T>
-void launch_bgs_color(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int color, T* x, const T* b, const T* sbuf,
-                      int block, hipStream_t s);
 template <typename T>
 void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
                            const T* r, const T* d, hipStream_t s);

@@ -69,13 +69,26 @@ static int op_resid_update(prx_tree* t, const LevelView& L, const float* e32, co
     return nb;
 }
 
+// Multicolour GS sweeps: one brick-tiled pass per colour (depth >= 3), the
+// per-node colour kernel on smaller grids.
 template <typename T>
 static void gs_sweeps(prx_tree* t, const LevelView& L, T* x, const T* b, int sweeps) {
     T* sb = L.scr ? sbuf_for<T>(t) : nullptr;
     for (int k = 0; k < sweeps; ++k)
         for (int c = 0; c < 8; ++c) {
             if (L.scr) launch_screen_gather<T>(L, x, nullptr, nullptr, sb, t->stream);
-            launch_gs_color<T>(L, c, x, b, sb, t->stream);
+            if (L.D >= 3) {
+                StencilIO<T> io;
+                io.x = x;
+                io.y = x;
+                io.b = b;
+                io.s = sb;
+                io.color = c;
+                int nb = 0;
+                launch_stencil<T>(L, OP_GS, io, &nb, t->tile, t->stream);
+            } else {
+                launch_gs_color<T>(L, c, x, b, sb, t->stream);
+            }
         }
 }
 
@@ -363,7 +376,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const hipStream_t s = t->stream;
     t->e32.ensure(n); t->ri32.ensure(n); t->d32.ensure(2 * n); t->q32.ensure(n);
     const double ih = 1.0 / t->h;
-    const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.1;
+    const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.02;
     const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
     // smoothing on the correction: e = GS^k(0; A/h, r/h), x += e
     {
@@ -999,7 +1012,7 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
         c.o.precision = precision;
         c.o.residual_mode = PRX_RESID_TRUE;
         c.o.refresh_period = 0;
-        c.o.inner_tol = 0.1;
+        c.o.inner_tol = 0.02;
         c.o.max_outer = 200;
         if (t->precond == PRX_PRECOND_MG) {
             c.o.precond = PRX_PRECOND_MG;
@@ -1150,7 +1163,7 @@ void prx_params_default(prx_params* p) {
     p->gs_iters = 20;
     p->base_depth = -1;
     p->refresh_period = 10;
-    p->inner_tol = 0.1;
+    p->inner_tol = 0.02;
     p->max_outer = 64;
 }
 
@@ -1160,7 +1173,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->residual_mode = PRX_RESID_REFERENCE;
     o->refresh_period = 10;
     o->gs_iters = 20;
-    o->inner_tol = 0.1;
+    o->inner_tol = 0.02;
     o->max_outer = 64;
     o->precond = PRX_PRECOND_NONE;
     o->mg_smooth = 2;

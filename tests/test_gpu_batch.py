@@ -59,7 +59,7 @@ def test_batch_graph_replays_and_autotune(gpu):
     ref = [single(D, pts, pw, 1e-10, pr.PRX_RESID_TRUE, 0)[0] for D, pts, pw in cs]
     for rnd in range(3):  # the captured graphs are replayed by every solve
         rep, per = B.solve(eps=1e-10, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=True)
-        assert rep["tile"] in (8, 16) and rep["gs_block"] in (64, 128, 256, 512)
+        assert rep["tile"] in (8, 16) and rep["gs_block"] in (8, 16)
         assert rep["chunk"] % 2 == 0 and 4 <= rep["chunk"] <= 64
         assert rep["graph_launches"] > 0
         for k in range(len(cs)):

@@ -170,6 +170,9 @@ def test_multicolour_gs_equals_colour_ordered_sweep(gpu, D):
         A.gs_sweep_order(b, xr, order)
     xg = t.gauss_seidel(b, x0, sweeps=3)
     assert np.abs(xg - xr).max() <= 1e-12 * np.abs(xr).max()
+    # the FP32 sweep (the mixed solve's smoother) to FP32 accuracy
+    xf = t.gauss_seidel(b, x0, sweeps=3, precision=pr.PRX_PREC_MIXED)
+    assert np.abs(xf - xr).max() <= 1e-4 * np.abs(xr).max()
 
 
 def test_cg_matches_direct_solve(gpu):
