@@ -116,8 +116,8 @@ typedef struct prx_solve_opts {
      * ||b-Ax|| <= eps ||b|| (no counterpart in the reference; same system,
      * same stopping test as PRX_RESID_TRUE) */
     int32_t precond;        /* PRX_PRECOND_NONE (default) or PRX_PRECOND_MG */
-    int32_t mg_smooth;      /* damped-Jacobi sweeps before and after each coarse correction (default 2) */
-    double mg_omega;        /* Jacobi damping (default 0.9) */
+    int32_t mg_smooth;      /* Jacobi sweeps before and after each coarse correction (default 2) */
+    double mg_omega;        /* smoothing: > 0 damped Jacobi with this weight; 0 (default) Chebyshev weights on [lmax/6, lmax]; < 0 Chebyshev on [lmax/-mg_omega, lmax] */
     double mg_inner_tol;    /* mixed MG: FP32 residual reduction per FP64 refinement step (default 1e-4) */
 } prx_solve_opts;
 
@@ -182,9 +182,10 @@ int prx_export_csr(prx_tree* t, int64_t* off, int64_t* col, double* val);
 int prx_apply(prx_tree* t, int precision, const double* x, double* y);
 
 /* z = M in, one multigrid V-cycle of the PRX_PRECOND_MG solver (M ~ A^-1
- * without the per-point screening; smooth <= 0 and omega <= 0 select the
- * defaults).  MIXED applies the FP32 V-cycle of the scaled system.  No
- * counterpart in the reference: a diagnostic of the preconditioner. */
+ * without the per-point screening; smooth <= 0 selects 2 sweeps, omega as
+ * prx_solve_opts.mg_omega).  MIXED applies the FP32 V-cycle of the scaled
+ * system.  No counterpart in the reference: a diagnostic of the
+ * preconditioner. */
 int prx_mg_vcycle(prx_tree* t, int precision, int smooth, double omega, const double* in, double* out);
 
 /* build_system_at_depth (src/fem_tree.rs:346-404) at depth cdepth: RHS in

@@ -607,7 +607,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 }
             }
         } else if constexpr (OP == OP_JACOBI2) {
-            const T j2 = (T)(2.0 * io.jw), jj = (T)(io.jw * io.jw);
+            const T j2 = (T)(io.jw + io.jw2), jj = (T)(io.jw * io.jw2);
             NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) q[e] = j2 * xin[v * VEC + e] - jj * tile[stage_at<B>(m) + e];
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
                 io.y[i] = y;
                 acc = (double)io.b[i] * (double)y;
             } else if constexpr (OP == OP_JACOBI2) {
-                io.y[i] = (T)(2.0 * io.jw) * io.x[i] - (T)(io.jw * io.jw) * a;
+                io.y[i] = (T)(io.jw + io.jw2) * io.x[i] - (T)(io.jw * io.jw2) * a;
             } else {
                 const T r = io.b[i] - a;
                 io.y[i] = r;

@@ -57,8 +57,8 @@ struct LevelView {
 
 // OP_JACOBI: y = x + jw * (b - A x), one damped-Jacobi step (jw = omega / a_ii,
 // a_ii constant), with partial sums of b.y when io.partial is set.
-// OP_JACOBI2: the first two such steps from x = 0 in one pass over b (the
-// operand io.x = b): y = 2 jw b - jw^2 A b.
+// OP_JACOBI2: the first two such steps from x = 0 (weights jw, jw2) in one
+// pass over b (the operand io.x = b): y = (jw + jw2) b - jw jw2 A b.
 // OP_GS: one colour (io.color = coordinate parities x | y<<1 | z<<2) of the
 // multicolour Gauss-Seidel sweep, in place: x_i += (b_i - (A x)_i) / a_ii for
 // the nodes of that colour (io.x = io.y = x).
@@ -149,8 +149,9 @@ struct StencilIO {
     const T* s = nullptr;  // per-point screening values wh3*sum(phi*x)
     double* partial = nullptr;
     const int32_t* done = nullptr;
-    double jw = 0.0;  // OP_JACOBI weight
-    int color = 0;    // OP_GS colour
+    double jw = 0.0;   // OP_JACOBI weight; OP_JACOBI2: the first step's
+    double jw2 = 0.0;  // OP_JACOBI2: the second step's weight
+    int color = 0;     // OP_GS colour
 };
 template <typename T>
 void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);

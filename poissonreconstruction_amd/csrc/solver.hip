@@ -461,7 +461,7 @@ struct Mg {
     uint32_t D = 0, dc = 0;
     int nu = 2;
     std::vector<LevelView> lv;  // [d - dc]
-    std::vector<double> jw;     // omega / a_ii per level
+    std::vector<std::vector<double>> jw;  // per level: the nu smoothing weights / a_ii
     std::vector<T*> xa, xb, bv, rv;
     int nc = 0;                 // nodes of the coarsest grid
     uint64_t cycles = 0;
@@ -528,6 +528,20 @@ static Mg<T> mg_build(prx_tree* t, int nu, double omega) {
     const double scale = sizeof(T) == 4 ? ih : 1.0;
     m.lv.resize(nl);
     m.jw.resize(nl);
+    // Step weights: omega > 0 is damped Jacobi (the same weight every step);
+    // omega <= 0 takes the nu steps at the reciprocals of the roots of the
+    // degree-nu Chebyshev polynomial on [lmax/alpha, lmax] (alpha = -omega, or
+    // 6), which damps that upper part of the spectrum of D^-1 (L + wg I)
+    // uniformly.  lmax = 1.5 is the maximum of the stencil's symbol over
+    // D = 32 s (at frequency (pi, 0, 0)); the truncated rows only lower it.
+    // The polynomial stays within [-1, 1] on [0, lmax], so the smoother
+    // converges and the V-cycle is positive definite.
+    std::vector<double> omega_i(m.nu, omega);
+    if (!(omega > 0.0)) {
+        const double lmax = 1.5, alpha = omega < 0.0 ? -omega : 6.0, lmin = lmax / alpha;
+        const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin);
+        for (int i = 0; i < m.nu; ++i) omega_i[i] = 1.0 / (th + de * std::cos(M_PI * (2 * i + 1) / (2.0 * m.nu)));
+    }
     m.xa.assign(nl, nullptr); m.xb.assign(nl, nullptr); m.bv.assign(nl, nullptr); m.rv.assign(nl, nullptr);
     int64_t total = 0;
     std::vector<int64_t> off(4 * nl, -1);
@@ -545,7 +559,8 @@ static Mg<T> mg_build(prx_tree* t, int nu, double omega) {
         L.k32 = {(float)(K0 * ih), (float)(Ke * ih), (float)(Kc * ih), (float)(wg * ih), 0.0f};
         L.scr = nullptr;
         L.pts = nullptr;
-        m.jw[l] = omega / ((32.0 * -Kc + wg) * scale);
+        m.jw[l].resize(m.nu);
+        for (int i = 0; i < m.nu; ++i) m.jw[l][i] = omega_i[i] / ((32.0 * -Kc + wg) * scale);
         // fine: xa = smoothing partner, rv = residual (input and output are the caller's)
         // middle: xa, xb, bv, rv ; coarsest: bv, xa
         const int cnt = l == nl - 1 ? 2 : (l == 0 ? 2 : 4);
@@ -593,12 +608,12 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
     const int top = (int)m.lv.size() - 1;
     std::vector<T*> cur(top + 1, nullptr), oth(top + 1, nullptr);
     int nb_rz = 0;
-    auto jacobi = [&](int l, const T* b, T*& c, T*& o, bool rz) {
+    auto jacobi = [&](int l, const T* b, T*& c, T*& o, double w, bool rz) {
         StencilIO<T> io;
         io.x = c;
         io.b = b;
         io.y = o;
-        io.jw = m.jw[l];
+        io.jw = w;
         io.partial = rz ? t->partial.p : nullptr;
         int nb = 0;
         launch_stencil<T>(m.lv[l], OP_JACOBI, io, &nb, t->tile, s);
@@ -618,12 +633,13 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
             StencilIO<T> io;
             io.x = b;
             io.y = c;
-            io.jw = m.jw[l];
+            io.jw = m.jw[l][0];
+            io.jw2 = m.jw[l][1];
             int nb = 0;
             launch_stencil<T>(L, OP_JACOBI2, io, &nb, t->tile, s);
-            for (int i = 2; i < m.nu; ++i) jacobi(l, b, c, o, false);
+            for (int i = 2; i < m.nu; ++i) jacobi(l, b, c, o, m.jw[l][i], false);
         } else {
-            launch_mg_scale<T>(c, b, m.jw[l], L.N, s);
+            launch_mg_scale<T>(c, b, m.jw[l][0], L.N, s);
         }
         StencilIO<T> io;
         io.x = c;
@@ -640,7 +656,8 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
     for (int l = 1; l <= top; ++l) {
         const T* b = l == top ? b_fine : m.bv[l];
         launch_mg_prolong_add<T>(m.lv[l].D, cur[l - 1], cur[l], s);
-        for (int i = 0; i < m.nu; ++i) jacobi(l, b, cur[l], oth[l], l == top && i == m.nu - 1);
+        // post-smoothing: the same steps in reverse order (M stays symmetric)
+        for (int i = 0; i < m.nu; ++i) jacobi(l, b, cur[l], oth[l], m.jw[l][m.nu - 1 - i], l == top && i == m.nu - 1);
     }
     if (top > 0 && cur[top] != z_fine) throw Error(PRX_ERR_STATE, "multigrid buffer parity");
     m.cycles++;
@@ -716,7 +733,7 @@ static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b
 }
 
 static int mg_nu(const prx_solve_opts& o) { return o.mg_smooth > 0 ? o.mg_smooth : 2; }
-static double mg_omega(const prx_solve_opts& o) { return o.mg_omega > 0.0 && o.mg_omega < 1.3 ? o.mg_omega : 0.9; }
+static double mg_omega(const prx_solve_opts& o) { return o.mg_omega < 1.3 ? o.mg_omega : 0.9; }
 
 // FP64 PCG from x = 0 to ||b - A x|| <= eps ||b||, confirmed on the true
 // residual (a PCG restart from the current x if the recursive residual drifted).
@@ -1177,7 +1194,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->max_outer = 64;
     o->precond = PRX_PRECOND_NONE;
     o->mg_smooth = 2;
-    o->mg_omega = 0.9;
+    o->mg_omega = 0.0;
     o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
 }

@@ -35,7 +35,8 @@ class MatrixFreeSystem:
 
     def mg_vcycle(self, b, precision: int = L.PRX_PREC_FP64, smooth: int = 0, omega: float = 0.0) -> np.ndarray:
         """z = M b, one V-cycle of the multigrid preconditioner (M ~ A^-1
-        without the per-point screening; 0 selects the default smoothing)."""
+        without the per-point screening; smooth 0 = 2 sweeps, omega 0 =
+        Chebyshev weights, > 0 = damped Jacobi with that weight)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         if b.shape != (self.num_rows(),):
             raise ValueError("b has the wrong length")
@@ -165,7 +166,7 @@ class FEMTree:
     def solve_cascadic(self, points, pw: float, base_depth: int, gs: int, cg: int, eps: float,
                        precision: int = L.PRX_PREC_FP64, residual_mode: int = L.PRX_RESID_REFERENCE,
                        refresh_period: int = 10, inner_tol: float = 0.02, max_outer: int = 64,
-                       precond: int = L.PRX_PRECOND_NONE, mg_smooth: int = 2, mg_omega: float = 0.9,
+                       precond: int = L.PRX_PRECOND_NONE, mg_smooth: int = 2, mg_omega: float = 0.0,
                        mg_inner_tol: float = 1e-4) -> dict:
         """src/fem_tree.rs:458-487; returns the solve report.  precond
         PRX_PRECOND_MG solves the same system with multigrid-preconditioned
