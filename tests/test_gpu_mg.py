@@ -172,3 +172,19 @@ def test_command_line_multigrid_flag(gpu, tmp_path):
     assert st.solve_status == pr.PRX_OK
     lines = open(out).read().split("\n")
     assert (int(lines[2].split()[-1]), int(lines[6].split()[-1])) == (len(mesh.vertices), len(mesh.triangles))
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+@pytest.mark.parametrize("D,pw", [(1, 4.0), (2, 4.0), (3, 4.0), (3, 0.0), (5, 0.0)])
+def test_mg_small_grids_and_no_screening(gpu, D, pw, prec):
+    # depth <= 2 is the dense coarsest solve alone; depth 3 one smoothed level
+    # on 8^3 bricks above it; pw = 0 leaves the operator without screening
+    pts = unit_points("sphere", 1500, D)
+    A, b = direct(D, pts, pw)
+    xd = sla.spsolve(A.tocsc(), b)
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    rep = t.solve_cascadic(pts, pw, max(D - 2, 0), 20, 10000, 1e-10, precision=prec, residual_mode=pr.PRX_RESID_TRUE,
+                           precond=pr.PRX_PRECOND_MG)
+    assert rep["status"] == pr.PRX_OK and rep["true_rel"] <= 1e-10
+    assert np.abs(t.solution - xd).max() <= 1e-8 * max(np.abs(xd).max(), 1e-300)
