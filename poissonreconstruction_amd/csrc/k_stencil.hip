@@ -214,8 +214,10 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         for (int i = tid; i < B3 / 8; i += NT) gdiag[i] = T(0);
     uint32_t bid = blockIdx.x;
     if (swz) {
-        const uint32_t per = nbricks >> 3;  // XCD-contiguous brick ranges
-        bid = (bid & 7u) * per + (bid >> 3);
+        // workgroups go to the 8 XCDs round-robin: XCD x takes the chunks of
+        // swz consecutive bricks numbered x, x+8, x+16, ... in turn
+        const uint32_t j = bid >> 3, c = (uint32_t)swz;
+        bid = (j / c) * 8u * c + (bid & 7u) * c + j % c;
     }
     const uint64_t base = (uint64_t)bid * B3;
     uint32_t lb = bid, res = L.res;
@@ -733,6 +735,24 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
     if (threadIdx.x == 0 && io.partial) io.partial[blockIdx.x] = sum;
 }
 
+// bricks per XCD chunk of the tile launch (0: no remapping).  Chunks of
+// PRX_XCD_CHUNK consecutive Morton bricks keep a brick's near neighbours in
+// its XCD's L2, while the eight XCDs sweep the grid together so that a far
+// neighbour (the next brick row or plane in Morton order) was read shortly
+// before by some XCD and is still in the last-level cache.  Measured on the
+// depth-9 grid (32768 16^3 bricks), FP32 direction pass: chunk 1 0.557 ms,
+// 8 0.462, 32 0.440, 64 0.435, 128 0.440, 4096 (one contiguous octant per
+// XCD) 0.465.
+#ifndef PRX_XCD_CHUNK
+#define PRX_XCD_CHUNK 64
+#endif
+static int xcd_chunk(uint32_t nbricks) {
+    if (nbricks < 64 || (nbricks & 7u)) return 0;
+    uint32_t c = nbricks >> 3;
+    while (c > (uint32_t)PRX_XCD_CHUNK && (c & 1u) == 0) c >>= 1;
+    return (int)c;
+}
+
 template <typename T>
 void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s) {
     const int64_t N = Lv.N;
@@ -761,7 +781,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
     const int tl = B == 16 ? 12 : 9;
     LevelArgs<T> a = make_args<T>(Lv, tl);
     const uint32_t nbricks = (uint32_t)(N >> tl);
-    const int swz = (nbricks >= 64 && (nbricks & 7u) == 0) ? 1 : 0;
+    const int swz = xcd_chunk(nbricks);
 #define PRX_TILE_LAUNCH(BB, OPV) \
     hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz, BatchTab{})
     if (B == 16) {
@@ -842,7 +862,7 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
     const int B = tile == 8 ? 8 : 16, tl = B == 16 ? 12 : 9;
     const LevelArgs<T> a = batch_args<T>(scr, tl);
     const uint32_t nbricks = (uint32_t)(Ntot >> tl);
-    const int swz = (nbricks >= 64 && (nbricks & 7u) == 0) ? 1 : 0;
+    const int swz = xcd_chunk(nbricks);
 #define PRX_BTILE_LAUNCH(BB, OPV) \
     hipLaunchKernelGGL((k_stencil_tile<T, BB, OPV, true>), dim3(nbricks), dim3(BB * BB), 0, s, a, io, nbricks, swz, tab)
     if (B == 16) {
