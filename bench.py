@@ -41,7 +41,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--depth", type=int, default=9)
     ap.add_argument("--points", type=int, default=1_000_000)
@@ -197,6 +197,8 @@ def main():
                                f"base depth {D - 2}, mixed FP32/FP64 refinement to {args.tol:g} true residual",
                    "depth": D, "points": P, "nodes": N, "tol": args.tol, "inner_tol": args.inner_tol},
         "fp64_ms_per_solve": round(ms_fp64, 2) if ms_fp64 else None,
+        "median_ms": {"mixed": round(float(np.median([m for m, _ in mixed])), 2),
+                      "fp64": round(float(np.median([m for m, _ in fp64])), 2) if fp64 else None},
         "mixed_speedup_vs_fp64": round(ms_fp64 / ms_mixed, 3) if ms_fp64 else None,
         "mixed": {k2: rep_mixed[k2] for k2 in ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse",
                                              "ms_fine_gs", "ms_fine_cg", "ms_total", "applies")},
@@ -402,43 +404,46 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
 
 def operator_roofline(L, tree, N, P, reps):
     """The dominant kernel's achieved HBM bandwidth.  The CG iteration's
-    operator pass is the FP32 fused direction update + apply (OP_DIR_APPLY):
-    it reads r and d_old, writes d_new and q = A d_new (16N bytes) and reads
-    the P screening points' positions once (12P).  Launches are timed with
-    hipEvents on the library's stream and averaged over `reps`.  The plain
-    apply y = A x (8N + 12P) and the FP64 forms are reported beside it."""
+    operator pass is the FP32 fused CG step (OP_CG): it reads r, q and d_old,
+    writes r' = r - alpha q, d_new = r' + beta d_old and q' = A d_new (24N
+    bytes) and reads the P screening points' positions once (12P).  Launches
+    are timed with hipEvents on the library's stream and averaged over
+    `reps`.  The two-pass form's direction pass (16N + 12P), the plain apply
+    y = A x (8N + 12P) and the FP64 forms are reported beside it."""
     def timed(prec, op):
         g, k = ctypes.c_double(0), ctypes.c_double(0)
         L.check(L.lib().prx_time_stencil_op(tree._h, prec, op, reps, ctypes.byref(g), ctypes.byref(k)))
         return g.value, k.value
+    gc, kc = timed(L.PRX_PREC_MIXED, 2)
     gd, kd = timed(L.PRX_PREC_MIXED, 1)
     ga, ka = timed(L.PRX_PREC_MIXED, 0)
+    gc64, kc64 = timed(L.PRX_PREC_FP64, 2)
     gd64, kd64 = timed(L.PRX_PREC_FP64, 1)
     ga64, ka64 = timed(L.PRX_PREC_FP64, 0)
     cms = ctypes.c_double(0)
     copy_bytes = 2 * 4 * N
     L.check(L.lib().prx_copy_bandwidth(copy_bytes, reps, 0, ctypes.byref(cms)))
     copy_gbs = copy_bytes / (cms.value * 1e-3) / 1e9
-    dir_bytes, apply_bytes = 16 * N + 12 * P, 8 * N + 12 * P
-    achieved = dir_bytes / (kd * 1e-3) / 1e9
+    cg_bytes, dir_bytes, apply_bytes = 24 * N + 12 * P, 16 * N + 12 * P, 8 * N + 12 * P
+    achieved = cg_bytes / (kc * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stencil_dir_apply_depth9_torus1M_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "stencil_cg_step_depth9_torus1M_traffic.json")
     if N == 8 ** 9 and os.path.exists(tpath):  # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
         with open(tpath) as fh:
             traffic = int(json.load(fh)["traffic_bytes_per_launch"])
     gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
+    sub = lambda kern, b, ms, g: {"kernel": kern, "bytes": b, "ms": round(ms, 4), "GBs": gbs(b, ms),
+                                  "frac": round(gbs(b, ms) / HBM_PEAK_GBS, 4), "gather_ms": round(g, 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_stencil_tile<float,16,OP_DIR_APPLY>", "bytes_per_launch": dir_bytes,
-                "ms_per_launch": round(kd, 4), "gather_ms": round(gd, 4),
+                "kernel": "k_stencil_tile<float,16,OP_CG>", "bytes_per_launch": cg_bytes,
+                "ms_per_launch": round(kc, 4), "gather_ms": round(gc, 4),
                 "frac_of_copy": round(achieved / copy_gbs, 4),
-                "apply_fp32": {"kernel": "k_stencil_tile<float,16,OP_APPLY>", "bytes": apply_bytes, "ms": round(ka, 4),
-                               "GBs": gbs(apply_bytes, ka), "frac": round(gbs(apply_bytes, ka) / HBM_PEAK_GBS, 4),
-                               "gather_ms": round(ga, 4)},
-                "dir_apply_fp64": {"bytes": 32 * N + 24 * P, "ms": round(kd64, 4), "GBs": gbs(32 * N + 24 * P, kd64),
-                                   "gather_ms": round(gd64, 4)},
-                "apply_fp64": {"bytes": 16 * N + 24 * P, "ms": round(ka64, 4), "GBs": gbs(16 * N + 24 * P, ka64),
-                               "gather_ms": round(ga64, 4)},
+                "cg_step_fp64": sub("k_stencil_tile<double,16,OP_CG>", 48 * N + 24 * P, kc64, gc64),
+                "dir_apply_fp32": sub("k_stencil_tile<float,16,OP_DIR_APPLY>", dir_bytes, kd, gd),
+                "dir_apply_fp64": sub("k_stencil_tile<double,16,OP_DIR_APPLY>", 32 * N + 24 * P, kd64, gd64),
+                "apply_fp32": sub("k_stencil_tile<float,16,OP_APPLY>", apply_bytes, ka, ga),
+                "apply_fp64": sub("k_stencil_tile<double,16,OP_APPLY>", 16 * N + 24 * P, ka64, ga64),
                 "copy_bytes": copy_bytes, "copy_ms": round(cms.value, 4), "copy_GBs": round(copy_gbs, 1)}
     log(f"roofline {roofline}")
     return roofline

@@ -119,6 +119,13 @@ typedef struct prx_solve_opts {
     int32_t mg_smooth;      /* Jacobi sweeps before and after each coarse correction (default 2) */
     double mg_omega;        /* smoothing: > 0 damped Jacobi with this weight; 0 (default) Chebyshev weights on [lmax/6, lmax]; < 0 Chebyshev on [lmax/-mg_omega, lmax] */
     double mg_inner_tol;    /* mixed MG: FP32 residual reduction per FP64 refinement step (default 1e-4) */
+    /* CG iteration form (no counterpart in the reference; the same
+     * iteration): 0 (default) one fused pass per iteration on grids of depth
+     * >= 3 (the residual update r -= alpha q joins the next operator pass,
+     * whose |r|^2 for the stopping test and beta is expanded from the pass's
+     * d.q, r.q, q.q and r.r); 1 the two-pass form
+     * (operator pass, then the vector update with its own |r|^2) */
+    int32_t cg_two_pass;
 } prx_solve_opts;
 
 typedef struct prx_solve_report {
@@ -247,7 +254,9 @@ int prx_unit_transform_device(const prx_point* points, size_t n, double scale, i
 int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, double* ms_stencil);
 /* As prx_time_operator for one pass of the CG iteration: op 0 = y = A x,
  * op 1 = the fused direction update and apply (d_new = r + beta d_old,
- * q = A d_new, beta = 0.5), each with its screening gather. */
+ * q = A d_new, beta = 0.5), op 2 = the fused CG step (r' = r - alpha q,
+ * d_new = r' + beta d_old, q' = A d_new, alpha = 0.25, beta = 0.5; depth >= 3),
+ * each with its screening gather. */
 int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
 /* Measurement: evaluate the current solution at every marching-cubes corner

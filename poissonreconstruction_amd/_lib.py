@@ -67,7 +67,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
                 ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
                 ("coarse_eps", ctypes.c_double), ("precond", ctypes.c_int32), ("mg_smooth", ctypes.c_int32),
-                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double)]
+                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double), ("cg_two_pass", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

@@ -25,7 +25,7 @@ static inline int nblocks_for(int64_t n, int per_thread = 1, int threads = 256, 
 static inline int red_blocks(int64_t n) { return nblocks_for(n, 8, 256, 2048); }
 
 int partial_capacity(int64_t N) {
-    int64_t b = (N + 511) / 512;    // one partial per brick (8^3 or 16^3), per 512 nodes of k_cg_update
+    int64_t b = 4 * ((N + 511) / 512);  // four per brick (8^3 or 16^3; fused CG step), one per 512 nodes of k_cg_update
     int64_t r = red_blocks(N);
     int64_t nv = (N + 255) / 256;   // naive stencil: one partial per block
     int64_t m = b > r ? b : r;
@@ -125,18 +125,34 @@ template void launch_dot<float>(const float*, const float*, int64_t, double*, in
 template <typename T>
 __device__ __forceinline__ void cg_upd(T& x, T& r, T d, T q, double a, double& acc) {
     x = (T)((double)x + a * (double)d);
-    r = (T)((double)r - a * (double)q);
+    r = r_form(a, q, r);
     acc += (double)r * (double)r;
 }
 template <typename T>
 __device__ __forceinline__ void cg_upd_r(T& r, T q, double a, double& acc) {
-    r = (T)((double)r - a * (double)q);
+    r = r_form(a, q, r);
     acc += (double)r * (double)r;
 }
 // the two pending terms of a lazy pair: x + alpha_pend * dprev + alpha * d
 template <typename T>
 __device__ __forceinline__ T x_pair(T x, T dp, T d, double ap, double a) {
     return (T)(((double)x + ap * (double)dp) + a * (double)d);
+}
+
+// beta of the next direction, with the breakdown and iteration-cap tests
+__device__ inline void fin_beta(CgState* st, double beta) {
+    if (beta != beta || beta == INFINITY || beta == -INFINITY) {
+        st->done = 1;
+        st->status = PRX_ERR_BREAKDOWN;
+        st->iters = st->max_iters;
+        return;
+    }
+    st->beta = beta;
+    if (st->it >= st->max_iters) {
+        st->done = 1;
+        st->iters = st->max_iters;
+        st->status = PRX_ERR_NOT_CONVERGED;
+    }
 }
 
 // The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
@@ -247,12 +263,68 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
             st->delta = sum;
             break;
         }
+        case FIN_FRESH:  // sum = |b - A x|^2 of the refreshed residual (x complete)
+            st->rpend = 0;
+            st->xn = 0;
+            st->delta = sum;
+            if (sum <= st->thr) {
+                st->done = 1;
+                st->iters = st->it;
+                break;
+            }
+            fin_beta(st, sum / st->delta_prev);
+            break;
     }
+}
+
+// The fused CG step (src/solvers.rs:36-64 with the two reductions of an
+// iteration merged): the pass formed r_k (the pending update of the previous
+// iteration), d_k = r_k + beta d_{k-1} and q_k = A d_k and summed d_k.q_k,
+// r_k.q_k, q_k.q_k and r_k.r_k.  delta_k = r_k.r_k and alpha_k = delta_k /
+// d_k.q_k as in the two-pass form; the update r_k - alpha_k q_k is left to
+// the next pass, so the stopping test and beta_k take its |r|^2 from the
+// expansion delta_k - 2 alpha_k r_k.q_k + alpha_k^2 q_k.q_k.  The next pass
+// sums its own r.r again, so the expansion's rounding never accumulates over
+// iterations (carried from one to the next, it would: its absolute error,
+// about eps * the early |r|^2, swamps |r|^2 once the residual has fallen by
+// ~1e-6).  x: a pair of pending terms was added before the pass (xn == 2 ->
+// 0), then alpha_k d_k is pending.  refresh: the caller recomputes r = b - A x
+// next (FIN_FRESH decides).
+__device__ inline void fin_cg(CgState* st, double dq, double rq, double qq, double rr, int refresh) {
+    if (st->xn == 2) st->xn = 0;
+    st->dq = dq;
+    st->delta = rr;
+    if (dq == 0.0) {
+        st->done = 1;
+        st->iters = st->it + 1;
+        return;
+    }
+    const double a = st->delta / dq;
+    st->alpha = a;
+    if (st->xn == 0) {
+        st->xn = 1;
+        st->apend = a;
+    } else {
+        st->xn = 2;
+    }
+    st->rpend = 1;
+    const double g = (st->delta - 2.0 * a * rq) + a * a * qq;
+    st->delta_prev = st->delta;
+    st->delta = g;
+    st->it += 1;
+    if (!refresh && g <= st->thr) {
+        st->done = 1;
+        st->iters = st->it;
+        return;
+    }
+    fin_beta(st, g / st->delta_prev);
 }
 
 // 1024 threads fold np partials in a fixed order (16 strided accumulators,
 // a fixed pairwise tree, then the block sum), so a reduction is reproducible
 // run to run; 16 independent loads per thread are in flight per round.
+// STRIDE: the partials of one sum are every STRIDE-th value.
+template <int STRIDE = 1>
 __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) {
     double a[16];
 #pragma unroll
@@ -260,9 +332,9 @@ __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) 
     int64_t i = threadIdx.x;
     for (; i + 15 * 1024 < np; i += 16 * 1024) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) a[u] += partial[i + u * 1024];
+        for (int u = 0; u < 16; ++u) a[u] += partial[STRIDE * (i + u * 1024)];
     }
-    for (; i < np; i += 1024) a[0] += partial[i];
+    for (; i < np; i += 1024) a[0] += partial[STRIDE * i];
 #pragma unroll
     for (int w = 8; w >= 1; w >>= 1)
 #pragma unroll
@@ -275,6 +347,16 @@ __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) 
 __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
     __shared__ double ws[16];
     if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && op != FIN_PINIT && st->done) return;
+    if (op == FIN_CG) {  // four interleaved partials per block
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[j] = fin_sum<4>(partial + j, np, ws);
+            __syncthreads();  // ws is reused by the next sum
+        }
+        if (threadIdx.x == 0) fin_cg(st, v[0], v[1], v[2], v[3], slot);
+        return;
+    }
     const double sum = fin_sum(partial, np, ws);
     if (threadIdx.x != 0) return;
     fin_step(st, sum, op, slot);
@@ -549,12 +631,70 @@ void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n
 template void launch_flush_x<double>(double*, const double*, const double*, const CgState*, int64_t, hipStream_t);
 template void launch_flush_x<float>(float*, const float*, const float*, const CgState*, int64_t, hipStream_t);
 __global__ void k_clear_xpend(CgState* st) {
-    if (threadIdx.x == 0) st->xpend = 0;
+    if (threadIdx.x == 0) {
+        st->xpend = 0;
+        st->xn = 0;
+    }
 }
 void launch_clear_xpend(CgState* st, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_xpend, dim3(1), dim3(64), 0, s, st);
     PRX_LAUNCH_CHECK();
 }
+
+// Fused CG x terms (n a multiple of the vector width: the fused step runs on
+// tiled levels only).  MODE 0: the pair before a pass (xn == 2, not done);
+// MODE 1: flush whatever is pending; MODE 2: the same while not done.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_x_terms(T* __restrict__ x, const T* __restrict__ da, const T* __restrict__ db,
+                                                 const CgState* st, int64_t nv) {
+    if (MODE != 1 && st->done) return;
+    const int xn = st->xn;
+    if (MODE == 0 ? xn != 2 : xn == 0) return;
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const double a = st->alpha, ap = st->apend;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t i = (int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x;
+        if (i < nv) {
+            const int64_t j = i * VEC;
+            NV xv = ld_nt(x + j);
+            const NV bv = ld_nt(db + j);
+            if (xn == 2) {
+                const NV av = ld_nt(da + j);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) xv[e] = x_pair<T>(xv[e], av[e], bv[e], ap, a);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) xv[e] = (T)((double)xv[e] + ap * (double)bv[e]);
+            }
+            st_nt(x + j, xv);
+        }
+    }
+}
+template <typename T>
+static void launch_x_terms(int mode, T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    if (n % VEC) throw Error(PRX_ERR_INVALID, "fused CG vectors must be a multiple of the vector width");
+    const int64_t nv = n / VEC;
+    const int nb = (int)std::max<int64_t>(1, (nv + 1023) / 1024);
+    if (mode == 0) hipLaunchKernelGGL((k_x_terms<T, 0>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
+    else if (mode == 1) hipLaunchKernelGGL((k_x_terms<T, 1>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
+    else hipLaunchKernelGGL((k_x_terms<T, 2>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
+    PRX_LAUNCH_CHECK();
+}
+template <typename T>
+void launch_x_pair(T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s) {
+    launch_x_terms<T>(0, x, da, db, st, n, s);
+}
+template <typename T>
+void launch_x_flush(T* x, const T* da, const T* db, const CgState* st, int64_t n, bool if_running, hipStream_t s) {
+    launch_x_terms<T>(if_running ? 2 : 1, x, da, db, st, n, s);
+}
+template void launch_x_pair<double>(double*, const double*, const double*, const CgState*, int64_t, hipStream_t);
+template void launch_x_pair<float>(float*, const float*, const float*, const CgState*, int64_t, hipStream_t);
+template void launch_x_flush<double>(double*, const double*, const double*, const CgState*, int64_t, bool, hipStream_t);
+template void launch_x_flush<float>(float*, const float*, const float*, const CgState*, int64_t, bool, hipStream_t);
 
 // d = r + beta d   (src/solvers.rs:64)
 template <typename T>

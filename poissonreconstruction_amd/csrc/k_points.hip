@@ -431,13 +431,16 @@ void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, d
 // per point in sorted order.  The operator then adds sum phi * s_p at every
 // touched node: together they apply sum_p wh3 phi_p phi_p^T
 // (src/fem_tree.rs:206-226).
-template <typename T, int MODE>  // MODE 0: x ; MODE 1: x64 + e32 ; MODE 2: r + beta*d
+template <typename T, int MODE>  // MODE 0: x ; MODE 1: x64 + e32 ; MODE 2: r + beta*d ; MODE 3: the OP_CG operand
 __global__ void k_screen_gather(uint32_t D, int64_t P, const uint32_t* __restrict__ cell, const T* __restrict__ tt,
                                 const T* __restrict__ x, const double* __restrict__ x64, const float* __restrict__ e32, T wh3,
-                                T* s, const T* __restrict__ r, const T* __restrict__ dd, const double* beta) {
+                                T* s, const T* __restrict__ r, const T* __restrict__ dd, const double* beta,
+                                const T* __restrict__ qq, const CgState* cst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    const double bt = MODE == 2 ? *beta : 0.0;
+    const double bt = MODE == 2 ? *beta : MODE == 3 ? cst->beta : 0.0;
+    const double al = MODE == 3 ? cst->alpha : 0.0;
+    const int rp = MODE == 3 ? cst->rpend : 0;
     const uint32_t res = 1u << D;
     const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
     const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
@@ -452,35 +455,39 @@ __global__ void k_screen_gather(uint32_t D, int64_t P, const uint32_t* __restric
         T v;
         if (MODE == 0) v = x[m];
         else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else v = dir_form(bt, dd[m], r[m]);
+        else if (MODE == 2) v = dir_form(bt, dd[m], r[m]);
+        else v = cg_form(bt, al, rp, dd[m], r[m], rp ? qq[m] : T(0));
         acc += bx * by * bz * v;
     }
     s[i] = wh3 * acc;
 }
 template <typename T>
 void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s,
-                          const T* r, const T* d, const double* beta) {
+                          const T* r, const T* d, const double* beta, const T* q, const CgState* cst) {
     const DevPoints& pts = *L.pts;
     if (pts.P <= 0) return;
     const T* tt;
     T wh3;
     if constexpr (sizeof(T) == 8) { tt = (const T*)pts.t64; wh3 = (T)L.k64.wh3; }
     else { tt = (const T*)pts.t32; wh3 = (T)L.k32.wh3; }
-    if (r)
+    if (cst)
+        hipLaunchKernelGGL((k_screen_gather<T, 3>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf,
+                           r, d, beta, q, cst);
+    else if (r)
         hipLaunchKernelGGL((k_screen_gather<T, 2>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf,
-                           r, d, beta);
+                           r, d, beta, q, cst);
     else if (x64)
         hipLaunchKernelGGL((k_screen_gather<T, 1>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf,
-                           r, d, beta);
+                           r, d, beta, q, cst);
     else
         hipLaunchKernelGGL((k_screen_gather<T, 0>), dim3(nblk(pts.P)), dim3(256), 0, s, L.D, pts.P, pts.cell, tt, x, x64, e32, wh3, sbuf,
-                           r, d, beta);
+                           r, d, beta, q, cst);
     PRX_LAUNCH_CHECK();
 }
 template void launch_screen_gather<double>(const LevelView&, const double*, const double*, const float*, double*, hipStream_t,
-                                           const double*, const double*, const double*);
+                                           const double*, const double*, const double*, const double*, const CgState*);
 template void launch_screen_gather<float>(const LevelView&, const float*, const double*, const float*, float*, hipStream_t,
-                                          const float*, const float*, const double*);
+                                          const float*, const float*, const double*, const float*, const CgState*);
 
 // The same gather over a batch: point i belongs to instance pts.inst[i]
 // (its grid size, wh3, first node and, in MODE 2, beta); it runs while the

@@ -95,6 +95,26 @@ __device__ inline double block_sum_red(double v, double* wsum) {
     return s;
 }
 
+// NS block sums at once (each folded in wave order, as block_sum_red)
+template <int NS>
+__device__ inline void block_sum_redn(const double* v, double* wsum, double* out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double u = v[j];
+        for (int o = 32; o > 0; o >>= 1) u += __shfl_xor(u, o, 64);
+        if (lane == 0) wsum[j * nw + wave] = u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double s = 0.0;
+        if (threadIdx.x == 0)
+            for (int w = 0; w < nw; ++w) s += wsum[j * nw + w];
+        out[j] = s;
+    }
+}
+
 // ------------------------------------------------------------------ tile index tables
 // Morton bit helpers usable at compile time.
 constexpr uint32_t spread3c(uint32_t v) {
@@ -174,8 +194,15 @@ __device__ __forceinline__ void ldrun(const U* p, U* o) {
 // the operand value of the pass at n nodes starting at g, formed as the
 // interior is formed (x ; x64 + e32 ; fma(beta, d_old, r))
 template <typename T, int OP, int n>
-__device__ __forceinline__ void halo_run(const StencilIO<T>& io, uint64_t g, double bt, T* o) {
-    if constexpr (OP == OP_RESID_UPD) {
+__device__ __forceinline__ void halo_run(const StencilIO<T>& io, uint64_t g, double bt, double al, int rp, T* o) {
+    if constexpr (OP == OP_CG) {
+        T r[n], d[n], q[n];
+        ldrun<T, n>(io.r + g, r);
+        ldrun<T, n>(io.dold + g, d);
+        if (rp) ldrun<T, n>(io.q + g, q);
+#pragma unroll
+        for (int k = 0; k < n; ++k) o[k] = cg_form(bt, al, rp, d[k], r[k], rp ? q[k] : T(0));
+    } else if constexpr (OP == OP_RESID_UPD) {
         double a[n];
         float e[n];
         ldrun<double, n>(io.x64 + g, a);
@@ -207,7 +234,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     typedef double nd2 __attribute__((ext_vector_type(2)));
     constexpr int VEC = 16 / (int)sizeof(T), PER = B3 / NT, NVEC = PER / VEC, NH = (S + NT - 1) / NT;
     __shared__ T tile[E3];
-    __shared__ double wsum[(NT + 63) / 64];
+    __shared__ double wsum[4 * ((NT + 63) / 64)];
     __shared__ T gdiag[OP == OP_GS ? B3 / 8 : 1];  // screening diagonal of the colour's touched nodes
     const int tid = threadIdx.x;
     if constexpr (OP == OP_GS)
@@ -222,7 +249,8 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     const uint64_t base = (uint64_t)bid * B3;
     uint32_t lb = bid, res = L.res;
     uint64_t ib = 0;  // first node of the brick's instance
-    double bt = 0.0;
+    double bt = 0.0, al = 0.0;
+    int rp = 0;
     if constexpr (BATCH) {
         const uint32_t inst = tab.brick_inst[base >> 12];
         if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
@@ -235,6 +263,11 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         if constexpr (OP == OP_DIR_APPLY) bt = tab.st[inst].beta;
     } else {
         if constexpr (OP == OP_DIR_APPLY) bt = *io.beta;
+        if constexpr (OP == OP_CG) {
+            bt = io.cst->beta;
+            al = io.cst->alpha;
+            rp = io.cst->rpend;
+        }
     }
     uint32_t ox, oy, oz;
     demorton3(lb, ox, oy, oz);
@@ -247,7 +280,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     }
 
     // ---- 1. issue the brick's interior loads (16-byte, coalesced)
-    NV qa[NVEC], qb[NVEC];
+    NV qa[NVEC], qb[NVEC], qc[OP == OP_CG ? NVEC : 1];
     nf2 ee[OP == OP_RESID_UPD ? NVEC : 1];
 #pragma unroll
     for (int v = 0; v < NVEC; ++v) {
@@ -258,6 +291,11 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         } else if constexpr (OP == OP_DIR_APPLY) {
             qa[v] = ld_v(io.r + g);
             qb[v] = ld_v(io.dold + g);
+        } else if constexpr (OP == OP_CG) {
+            qa[v] = ld_v(io.r + g);
+            qb[v] = ld_v(io.dold + g);
+            if (rp) qc[v] = ld_v(io.q + g);
+            else qc[v] = NV{};
         } else {
             qa[v] = ld_v(io.x + g);
         }
@@ -289,7 +327,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             bool ok;
             const uint64_t sb = nb_base(0, 0, side ? 1 : -1, ok);
             const uint32_t m0 = spread3(2 * a) | (spread3(2 * b) << 1) | (side ? 0u : (spread3(15u) << 2));
-            if (ok) halo_run<T, OP, 4>(io, sb + m0, bt, zf);
+            if (ok) halo_run<T, OP, 4>(io, sb + m0, bt, al, rp, zf);
             zi = tix(2 * a, 2 * b, side ? B : -1);
         }
 #pragma unroll
@@ -301,7 +339,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 bool ok;
                 const uint64_t sb = nb_base(0, sd ? 1 : -1, 0, ok);
                 const uint32_t m0 = spread3(2 * a) | ((sd ? 0u : spread3(15u)) << 1) | (spread3((uint32_t)c) << 2);
-                if (ok) halo_run<T, OP, 2>(io, sb + m0, bt, yf[sd]);
+                if (ok) halo_run<T, OP, 2>(io, sb + m0, bt, al, rp, yf[sd]);
             }
         }
 #pragma unroll
@@ -311,7 +349,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             bool ok;
             const uint64_t sb = nb_base(sd ? 1 : -1, 0, 0, ok);
             const uint32_t m = (sd ? 0u : spread3(15u)) | (spread3((uint32_t)yy) << 1) | (spread3((uint32_t)zz) << 2);
-            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, &xf[sd]);
+            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, al, rp, &xf[sd]);
             xi[sd] = tix(sd ? B : -1, yy, zz);
         }
         ev = T(0);
@@ -331,7 +369,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
             bool ok;
             const uint64_t sb = nb_base(hx < 0 ? -1 : (hx >= B ? 1 : 0), hy < 0 ? -1 : (hy >= B ? 1 : 0), hz < 0 ? -1 : (hz >= B ? 1 : 0), ok);
             const uint32_t m = spread3((uint32_t)(hx & 15)) | (spread3((uint32_t)(hy & 15)) << 1) | (spread3((uint32_t)(hz & 15)) << 2);
-            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, &ev);
+            if (ok) halo_run<T, OP, 1>(io, sb + m, bt, al, rp, &ev);
             ei = tix(hx, hy, hz);
         }
     } else {
@@ -351,6 +389,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                     hin[k] = true;
                     if constexpr (OP == OP_RESID_UPD) hv[k] = io.x64[g] + (double)io.e32[g];
                     else if constexpr (OP == OP_DIR_APPLY) hv[k] = dir_form(bt, io.dold[g], io.r[g]);
+                    else if constexpr (OP == OP_CG) hv[k] = cg_form(bt, al, rp, io.dold[g], io.r[g], rp ? io.q[g] : T(0));
                     else hv[k] = io.x[g];
                 }
             }
@@ -385,9 +424,11 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     T r0;
     if constexpr (OP == OP_RESID_UPD) r0 = io.x64[base] + (double)io.e32[base];
     else if constexpr (OP == OP_DIR_APPLY) r0 = dir_form(bt, io.dold[base], io.r[base]);
+    else if constexpr (OP == OP_CG) r0 = cg_form(bt, al, rp, io.dold[base], io.r[base], rp ? io.q[base] : T(0));
     else r0 = io.x[base];
     // ---- 3. interior values into LDS (offset linear in the Morton bits)
     T xin[PER];
+    T rk[OP == OP_CG ? PER : 1];  // OP_CG: the updated residual, for r'.y
     const int lbase = (E * E + E + 1) + (int)(compact3(tid * VEC) + E * compact3((tid * VEC) >> 1) + E * E * compact3((tid * VEC) >> 2));
 #pragma unroll
     for (int v = 0; v < NVEC; ++v) {
@@ -403,6 +444,18 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
                 nq[e] = dir_form(bt, (T)qb[v][e], (T)qa[v][e]);
                 xin[v * VEC + e] = nq[e];
             }
+            st_nt(io.dnew + g, nq);
+        } else if constexpr (OP == OP_CG) {
+            // r' = r - alpha q (the previous iteration's update), d' = r' + beta d_old
+            NV nr, nq;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                nr[e] = rp ? r_form(al, (T)qc[v][e], (T)qa[v][e]) : (T)qa[v][e];
+                nq[e] = dir_form(bt, (T)qb[v][e], nr[e]);
+                rk[v * VEC + e] = nr[e];
+                xin[v * VEC + e] = nq[e];
+            }
+            st_nt(io.rout + g, nr);
             st_nt(io.dnew + g, nq);
         } else {
 #pragma unroll
@@ -566,11 +619,23 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     // ---- 6. coalesced stores + reduction partial (per-thread terms in T,
     // folded in FP64 across the block and the grid)
     T tacc = T(0);
+    double c3[4] = {0.0, 0.0, 0.0, 0.0};  // OP_CG: d'.y, r'.y, y.y, r'.r'
 #pragma unroll
     for (int v = 0; v < NVEC; ++v) {
         const uint32_t m = (uint32_t)(v * NT + tid) * VEC;
         const uint64_t g = base + m;
-        if constexpr (OP == OP_APPLY || OP == OP_DIR_APPLY) {
+        if constexpr (OP == OP_CG) {
+            NV q;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                q[e] = tile[stage_at<B>(m) + e];
+                c3[0] += (double)xin[v * VEC + e] * (double)q[e];
+                c3[1] += (double)rk[v * VEC + e] * (double)q[e];
+                c3[2] += (double)q[e] * (double)q[e];
+                c3[3] += (double)rk[v * VEC + e] * (double)rk[v * VEC + e];
+            }
+            *(NV*)(io.y + g) = q;
+        } else if constexpr (OP == OP_APPLY || OP == OP_DIR_APPLY) {
             NV q;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
@@ -629,6 +694,15 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         }
     }
     if constexpr (OP == OP_JACOBI2 || OP == OP_GS) return;
+    if constexpr (OP == OP_CG) {
+        double s4[4];
+        block_sum_redn<4>(c3, wsum, s4);
+        if (tid == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) io.partial[4 * (uint64_t)bid + j] = s4[j];
+        }
+        return;
+    }
     if constexpr (OP == OP_JACOBI) {
         if (!io.partial) return;  // uniform over the block
     }
@@ -760,6 +834,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
     if (Lv.D >= 4) B = (tile == 8) ? 8 : 16;
     else if (Lv.D == 3) B = 8;
     if (B == 0) {
+        if (op == OP_CG) throw Error(PRX_ERR_INVALID, "the fused CG pass needs a tiled level (depth >= 3)");
         LevelArgs<T> a = make_args<T>(Lv, 12);
         const int nb = (int)((N + 255) / 256);
         switch (op) {
@@ -791,6 +866,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         else if (op == OP_JACOBI) PRX_TILE_LAUNCH(16, OP_JACOBI);
         else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(16, OP_JACOBI2);
         else if (op == OP_GS) PRX_TILE_LAUNCH(16, OP_GS);
+        else if (op == OP_CG) PRX_TILE_LAUNCH(16, OP_CG);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(16, OP_RESID_UPD);
     } else {
         if (op == OP_APPLY) PRX_TILE_LAUNCH(8, OP_APPLY);
@@ -799,6 +875,7 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
         else if (op == OP_JACOBI) PRX_TILE_LAUNCH(8, OP_JACOBI);
         else if (op == OP_JACOBI2) PRX_TILE_LAUNCH(8, OP_JACOBI2);
         else if (op == OP_GS) PRX_TILE_LAUNCH(8, OP_GS);
+        else if (op == OP_CG) PRX_TILE_LAUNCH(8, OP_CG);
         else if constexpr (sizeof(T) == 8) PRX_TILE_LAUNCH(8, OP_RESID_UPD);
     }
 #undef PRX_TILE_LAUNCH

@@ -13,6 +13,10 @@ namespace prx {
 // iterations.  `xpend` marks a term alpha_pend * d[dir_buf] not yet in x;
 // solvers flush it before they read x.  `dir_buf` is the ping-pong index of
 // the latest direction (the FIN_BETA caller passes it as the slot).
+// Fused CG step (OP_CG + FIN_CG): `rpend` marks the update r -= alpha q of
+// the last iteration as still to be formed (by the next pass, from the r and
+// q it wrote); `xn` counts the terms alpha d not yet added to x (the older
+// one with coefficient apend, the newer with alpha).
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
     double bnorm2;
@@ -21,6 +25,8 @@ struct CgState {
     int32_t done, status, refresh_due, lazy;
     double alpha_pend;
     int32_t xpend, dir_buf;
+    double apend;
+    int32_t rpend, xn;
 };
 
 // FIN_BETA_FULL: as FIN_BETA after an iteration that left x fully updated.
@@ -28,9 +34,14 @@ struct CgState {
 // FIN_PRES takes |r|^2 after the vector update (lazy-x bookkeeping, the
 // iteration count and the stopping test on |r|^2 <= thr), FIN_PBETA takes the
 // new r.z and forms beta.
+// FIN_CG folds the four interleaved sums of an OP_CG pass (d.q, r.q, q.q,
+// r.r): delta = r.r, alpha = delta / d.q, the next |r|^2 = delta - 2 alpha
+// r.q + alpha^2 q.q and beta, in one step (slot 1: a refresh of r follows, so
+// the stopping test waits for FIN_FRESH).  FIN_FRESH takes |b - A x|^2 of a
+// refreshed residual.
 enum FinOp {
     FIN_INIT_REF = 0, FIN_INIT_TRUE = 1, FIN_ALPHA = 2, FIN_BETA = 3, FIN_SLOT = 4, FIN_BETA_FULL = 5,
-    FIN_PINIT = 6, FIN_PRES = 7, FIN_PBETA = 8
+    FIN_PINIT = 6, FIN_PRES = 7, FIN_PBETA = 8, FIN_CG = 9, FIN_FRESH = 10
 };
 
 // Points on the device.  Input order: positions/normals (SoA, FP64).  Sorted
@@ -62,7 +73,12 @@ struct LevelView {
 // OP_GS: one colour (io.color = coordinate parities x | y<<1 | z<<2) of the
 // multicolour Gauss-Seidel sweep, in place: x_i += (b_i - (A x)_i) / a_ii for
 // the nodes of that colour (io.x = io.y = x).
-enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5, OP_GS = 6 };
+// OP_CG: one fused CG step (tile kernels only): r' = r - alpha q when
+// rpend (else r' = r), d' = r' + beta dold, y = A d'; writes r' to rout, d'
+// to dnew and y; four partials per brick, interleaved (d'.y, r'.y, y.y, r'.r').
+// alpha, beta and rpend come from io.cst.  rout, dnew and y must not alias
+// r, dold and q: neighbouring workgroups still read those as halo.
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5, OP_GS = 6, OP_CG = 7 };
 
 // ---- k_grid.hip
 void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
@@ -76,6 +92,12 @@ template <typename T> void launch_axpy_alpha(T* x, const T* dprev, const T* d, c
 // x += alpha_pend * d{dir_buf} when a lazy term is pending (the caller then clears xpend)
 template <typename T> void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s);
 void launch_clear_xpend(CgState* st, hipStream_t s);
+// fused CG: x += apend * da + alpha * db while xn == 2 and not done (the pair
+// is added before the pass that overwrites da)
+template <typename T> void launch_x_pair(T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s);
+// fused CG: add the pending terms (xn == 1: apend * db; xn == 2: apend * da +
+// alpha * db); if_running: only while the solve is not done
+template <typename T> void launch_x_flush(T* x, const T* da, const T* db, const CgState* st, int64_t n, bool if_running, hipStream_t s);
 
 // ---- k_mg.hip: geometric multigrid on the dense Morton grids
 // x = w * b
@@ -118,9 +140,11 @@ void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t
 void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, double wh3, double* rhs, hipStream_t s);
 // s_p over the point's corners of x (x64 + e32 when x64 is given; r + beta*d
 // when r is given).
+// With cst (OP_CG): the operand r' + beta d with r' = r - alpha q when rpend.
 template <typename T>
 void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s,
-                          const T* r = nullptr, const T* d = nullptr, const double* beta = nullptr);
+                          const T* r = nullptr, const T* d = nullptr, const double* beta = nullptr, const T* q = nullptr,
+                          const CgState* cst = nullptr);
 void launch_csr_count(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, int64_t* rowcnt, hipStream_t s);
 void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, double K0, double Ke, double Kc,
                      double wg, double wh3, const int64_t* off, int64_t* col, double* val, hipStream_t s);
@@ -152,6 +176,9 @@ struct StencilIO {
     double jw = 0.0;   // OP_JACOBI weight; OP_JACOBI2: the first step's
     double jw2 = 0.0;  // OP_JACOBI2: the second step's weight
     int color = 0;     // OP_GS colour
+    const T* q = nullptr;  // OP_CG: the previous pass's A d
+    T* rout = nullptr;     // OP_CG: the updated residual
+    const CgState* cst = nullptr;  // OP_CG: alpha, beta, rpend
 };
 template <typename T>
 void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);

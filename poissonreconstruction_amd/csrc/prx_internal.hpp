@@ -112,6 +112,15 @@ __device__ inline typename NV16<T>::type ld_v(const T* p) {
 // to FP32 measurably costs CG iterations of the FP32 correction system).
 __device__ __forceinline__ double dir_form(double bt, double d, double r) { return fma(bt, d, r); }
 __device__ __forceinline__ float dir_form(double bt, float d, float r) { return (float)fma(bt, (double)d, (double)r); }
+// The CG residual update r - a q, in FP64 and rounded once, formed the same
+// way wherever a node's updated residual is needed.
+__device__ __forceinline__ double r_form(double a, double q, double r) { return fma(-a, q, r); }
+__device__ __forceinline__ float r_form(double a, float q, float r) { return (float)fma(-a, (double)q, (double)r); }
+// OP_CG operand: d' = r' + bt d with r' = r - a q when rp (else r' = r)
+template <typename T>
+__device__ __forceinline__ T cg_form(double bt, double a, int rp, T d, T r, T q) {
+    return dir_form(bt, d, rp ? r_form(a, q, r) : r);
+}
 
 struct NodeList {
     int mode = 0;                 // 0 = floor (splat), 1 = trunc (screening)

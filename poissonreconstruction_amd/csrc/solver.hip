@@ -125,6 +125,15 @@ static double dev_dot(prx_tree* t, const T* a, const T* b, int64_t n) {
 // so chunks of iterations are enqueued without a host round-trip and the
 // iteration count is exact.  Refresh r = b - A x on iterations refresh,
 // 2*refresh, ... as the reference does every 10th (refresh 0 = never).
+//
+// Fused form (tiled levels, when second r and q buffers are given): one
+// operator pass per iteration (OP_CG) also forms the previous iteration's
+// residual update r - alpha q, and one fold (FIN_CG) yields alpha, the new
+// |r|^2 and beta.  r and q then live in ping-pong pairs like d: pass k reads
+// rr[k&1], qq[k&1], dd[k&1] and writes rr[(k+1)&1], qq[(k+1)&1],
+// dd[(k+1)&1].  x takes its terms in pairs before the pass that overwrites
+// the older direction.  Per iteration this reads r, q, d and writes r, d, q
+// (24 bytes per node in FP32) where the two-pass form moves 28.
 template <typename T>
 struct Cg {
     prx_tree* t;
@@ -134,11 +143,16 @@ struct Cg {
     T* r;
     T* dd[2];
     T* q;
+    T* rr[2] = {nullptr, nullptr};
+    T* qq[2] = {nullptr, nullptr};
+    bool fused = false;
+    int xn = 0;  // fused: x terms pending on the device (as FIN_CG counts them)
     int refresh = 0;
     uint64_t it = 0;  // completed iterations (the device's count after run())
     uint64_t applies = 0;
 
-    T* dir() { return dd[it & 1]; }
+    // the residual the next iteration starts from (where a replacement goes)
+    T* rcur() { return fused ? rr[it & 1] : r; }
 
     // r = b - A x (or r = b when x is known to be 0); delta0 = r.r
     void start(bool x_zero, int mode, double eps, uint64_t max_iters, double bnorm2) {
@@ -155,6 +169,7 @@ struct Cg {
         launch_fin(t->st, t->partial.p, nb, mode == PRX_RESID_TRUE ? FIN_INIT_TRUE : FIN_INIT_REF, 0, s);
         launch_fill<T>(dd[0], T(0), L.N, s);
         it = 0;
+        xn = 0;
     }
     // start from a residual already in r (x = 0 implied): delta0 = r.r
     void start_from_residual(double eps, uint64_t max_iters) {
@@ -165,8 +180,48 @@ struct Cg {
         launch_fin(t->st, t->partial.p, nb, FIN_INIT_REF, 0, s);
         launch_fill<T>(dd[0], T(0), L.N, s);
         it = 0;
+        xn = 0;
+    }
+    void enqueue_fused(uint64_t k) {
+        const hipStream_t s = t->stream;
+        const int32_t* done = &t->st->done;
+        T* dold = dd[k & 1];  // d_{k-1}
+        T* dnew = dd[(k + 1) & 1];  // holds d_{k-2} until this pass writes d_k
+        if (xn == 2) launch_x_pair<T>(x, dnew, dold, t->st, L.N, s);
+        StencilIO<T> io;
+        io.r = rr[k & 1];
+        io.q = qq[k & 1];
+        io.rout = rr[(k + 1) & 1];
+        io.dold = dold;
+        io.dnew = dnew;
+        io.y = qq[(k + 1) & 1];
+        io.cst = t->st;
+        io.partial = t->partial.p;
+        io.done = done;
+        if (L.scr) {
+            T* sb = sbuf_for<T>(t);
+            launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, io.r, dold, nullptr, io.q, t->st);
+            io.s = sb;
+        }
+        int nb = 0;
+        launch_stencil<T>(L, OP_CG, io, &nb, t->tile, s);
+        applies++;
+        const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
+        launch_fin(t->st, t->partial.p, nb, FIN_CG, fresh ? 1 : 0, s);
+        xn = xn == 2 ? 1 : xn + 1;
+        if (fresh) {  // x completed, then r = b - A x replaces the pending update
+            launch_x_flush<T>(x, dold, dnew, t->st, L.N, true, s);
+            nb = op_pass<T>(t, L, OP_RESID, x, b, io.rout, done, t->partial.p);
+            applies++;
+            launch_fin(t->st, t->partial.p, nb, FIN_FRESH, 0, s);
+            xn = 0;
+        }
     }
     void enqueue(uint64_t k) {
+        if (fused) {
+            enqueue_fused(k);
+            return;
+        }
         const hipStream_t s = t->stream;
         const int32_t* done = &t->st->done;
         T* dold = dd[k & 1];
@@ -210,11 +265,16 @@ struct Cg {
             if (t->st_host->done || k >= cap) break;
         }
         it = (uint64_t)t->st_host->it;
-        if (t->st_host->xpend) {
+        if (fused && t->st_host->xn) {  // d_{it-1} in dd[it&1], d_{it-2} in the other
+            launch_x_flush<T>(x, dd[(it + 1) & 1], dd[it & 1], t->st, L.N, false, t->stream);
+            launch_clear_xpend(t->st, t->stream);
+            t->st_host->xn = 0;
+        } else if (t->st_host->xpend) {
             launch_flush_x<T>(x, dd[0], dd[1], t->st, L.N, t->stream);
             launch_clear_xpend(t->st, t->stream);
             t->st_host->xpend = 0;
         }
+        xn = 0;
     }
     // Replace the recursive residual by `delta_true` = |r|^2 of a true
     // residual already written to r.  While the two residuals still agree
@@ -228,13 +288,17 @@ struct Cg {
         h.thr = thr;
         h.done = 0;
         h.status = 0;
+        h.rpend = 0;  // the replacement is the residual of the next iteration
+        h.xn = 0;
         *t->st_host = h;
         PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, t->stream));
     }
 };
 
+// r1, q1: second residual and product buffers (the fused form on tiled levels)
 template <typename T>
-static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d0, T* d1, T* q, int refresh) {
+static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d0, T* d1, T* q, int refresh, T* r1 = nullptr,
+                     T* q1 = nullptr) {
     Cg<T> c;
     c.t = t;
     c.L = L;
@@ -245,6 +309,11 @@ static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T*
     c.dd[1] = d1;
     c.q = q;
     c.refresh = refresh;
+    c.fused = r1 && q1 && L.D >= 3 && t->fused_cg;
+    c.rr[0] = r;
+    c.rr[1] = r1;
+    c.qq[0] = q;
+    c.qq[1] = q1;
     return c;
 }
 
@@ -262,7 +331,7 @@ static CgOut run_cg(Cg<T>& cg, uint64_t max_iters, double eps, int mode, double 
         cg.run(max_iters);
         const CgState h = *t->st_host;
         if (mode != PRX_RESID_TRUE || !h.done || h.status != PRX_OK || h.iters == 0) break;
-        int nb = op_pass<T>(t, cg.L, OP_RESID, cg.x, cg.b, cg.r, nullptr, t->partial.p);
+        int nb = op_pass<T>(t, cg.L, OP_RESID, cg.x, cg.b, cg.rcur(), nullptr, t->partial.p);
         cg.applies++;
         launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 1, t->stream);
         read_state(t);
@@ -314,13 +383,14 @@ static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, d
     L.res = 1u << cdepth;
     L.N = nc;
     L.k64 = {stiffness_entry(hc, 0, 0, 0), stiffness_entry(hc, 1, 1, 0), stiffness_entry(hc, 1, 1, 1), epw > 0.0 ? wh3c * 1.005 : 0.0, 0.0};
-    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(nc); t->cdd.ensure(2 * nc); t->cq.ensure(nc);
+    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(2 * nc); t->cdd.ensure(2 * nc); t->cq.ensure(2 * nc);
     launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
     launch_fill<double>(t->cx.p, 0.0, nc, s);
     gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
     const bool tru = mode == PRX_RESID_TRUE;
     const double bb = tru ? dev_dot<double>(t, t->cb.p, t->cb.p, nc) : 0.0;
-    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, tru ? 0 : 10);
+    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, tru ? 0 : 10, t->cr.p + nc,
+                                    t->cq.p + nc);
     CgOut c = run_cg<double>(cg, iters, eps, tru ? PRX_RESID_TRUE : PRX_RESID_REFERENCE, bb);
     rep.coarse_iters = c.iters;
     rep.coarse_rel = c.rel;
@@ -342,7 +412,7 @@ static double true_rel(prx_tree* t) {
 static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
-    t->r64.ensure(n); t->d64.ensure(2 * n); t->q64.ensure(n);
+    t->r64.ensure(2 * n); t->d64.ensure(2 * n); t->q64.ensure(2 * n);
     {
         Timer tm(t->stream);
         gs_sweeps<double>(t, L, t->x.p, t->b.p, c.gs);
@@ -351,7 +421,8 @@ static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_so
     Timer tm(t->stream);
     double bb = 0.0;
     if (c.o.residual_mode == PRX_RESID_TRUE) bb = dev_dot<double>(t, t->b.p, t->b.p, n);
-    Cg<double> cg = make_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, c.o.refresh_period);
+    Cg<double> cg = make_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, c.o.refresh_period,
+                                    t->r64.p + n, t->q64.p + n);
     CgOut o = run_cg<double>(cg, cap, c.eps, c.o.residual_mode, bb);
     rep.ms_fine_cg = tm.ms();
     rep.fine_iters = o.iters;
@@ -374,7 +445,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
     const hipStream_t s = t->stream;
-    t->e32.ensure(n); t->ri32.ensure(n); t->d32.ensure(2 * n); t->q32.ensure(n);
+    t->e32.ensure(n); t->ri32.ensure(2 * n); t->d32.ensure(2 * n); t->q32.ensure(2 * n);
     const double ih = 1.0 / t->h;
     const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.02;
     const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
@@ -397,7 +468,8 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const double thr32 = thr * ih * ih;  // the same test on the scaled FP32 residual
     uint64_t outer = 0;
     int status = PRX_OK;
-    Cg<float> cg = make_cg<float>(t, L, t->e32.p, nullptr, t->ri32.p, t->d32.p, t->d32.p + n, t->q32.p, 0);
+    Cg<float> cg = make_cg<float>(t, L, t->e32.p, nullptr, t->ri32.p, t->d32.p, t->d32.p + n, t->q32.p, 0, t->ri32.p + n,
+                                  t->q32.p + n);
     if (rr > thr && rr > 0.0) {
         launch_fill<float>(t->e32.p, 0.0f, n, s);
         cg.start_from_residual(1.0, cap);
@@ -407,7 +479,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
             cg.run(cap);
             const CgState h = *t->st_host;
             // fold e into x and replace the residual by the FP64 true one
-            nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
+            nb = op_resid_update(t, L, t->e32.p, t->b.p, cg.rcur(), ih);
             rep.applies++;
             launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
             read_state(t);
@@ -423,7 +495,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
                 break;
             }
             launch_fill<float>(t->e32.p, 0.0f, n, s);
-            const double dt = dev_dot<float>(t, t->ri32.p, t->ri32.p, n);
+            const double dt = dev_dot<float>(t, cg.rcur(), cg.rcur(), n);
             *t->st_host = h;
             cg.replace_residual(dt, std::max(theta * theta * dt, thr32));
         }
@@ -840,6 +912,7 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
     std::memset(&rep, 0, sizeof rep);
     t->ensure_state();
     t->precond = c.o.precond;
+    t->fused_cg = c.o.cg_two_pass ? 0 : 1;
     if (c.o.precond == PRX_PRECOND_MG) {
         solve_mg(t, c, rep);
         return;
@@ -1079,8 +1152,9 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
         PRX_HIP(hipEventCreate(&e1));
         PRX_HIP(hipEventCreate(&e2));
         double g = 0.0, k = 0.0;
-        // OP_DIR_APPLY: r = xin, d_old = dold, d_new = dnew (beta from the CG state)
-        auto run = [&](auto* xin, auto* yout, auto* sb, auto* dold, auto* dnew) {
+        // OP_DIR_APPLY: r = xin, d_old = dold, d_new = dnew (beta from the CG
+        // state); OP_CG also q = qin and r' to rout (alpha, beta, rpend = 1)
+        auto run = [&](auto* xin, auto* yout, auto* sb, auto* dold, auto* dnew, auto* qin, auto* rout) {
             using T = std::remove_pointer_t<decltype(xin)>;
             StencilIO<T> io;
             io.x = xin;
@@ -1091,11 +1165,15 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
             io.dold = dold;
             io.dnew = dnew;
             io.beta = &t->st->beta;
+            io.q = qin;
+            io.rout = rout;
+            io.cst = t->st;
             int nb = 0;
             for (int r = -1; r < reps; ++r) {  // r = -1 warms up
                 PRX_HIP(hipEventRecord(e0, s));
                 if (L.scr) {
                     if (op == OP_DIR_APPLY) launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, xin, dold, &t->st->beta);
+                    else if (op == OP_CG) launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, xin, dold, nullptr, qin, t->st);
                     else launch_screen_gather<T>(L, xin, nullptr, nullptr, sb, s);
                 }
                 PRX_HIP(hipEventRecord(e1, s));
@@ -1108,33 +1186,38 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
                 if (r >= 0) { g += a; k += b; }
             }
         };
-        if (op != OP_APPLY && op != OP_DIR_APPLY) return PRX_ERR_INVALID;
-        if (op == OP_DIR_APPLY) {  // beta = 0.5, not done
+        if (op != OP_APPLY && op != OP_DIR_APPLY && op != OP_CG) return PRX_ERR_INVALID;
+        if (op == OP_CG && L.D < 3) return PRX_ERR_INVALID;
+        if (op == OP_DIR_APPLY || op == OP_CG) {  // beta = 0.5, alpha = 0.25, r update pending, not done
             CgState h{};
             h.beta = 0.5;
+            h.alpha = 0.25;
+            h.rpend = 1;
             *t->st_host = h;
             PRX_HIP(hipMemcpyAsync(t->st, t->st_host, sizeof(CgState), hipMemcpyHostToDevice, s));
         }
         if (precision == PRX_PREC_MIXED) {
             t->d32.ensure(2 * n);
-            t->q32.ensure(n);
-            t->ri32.ensure(n);
+            t->q32.ensure(2 * n);
+            t->ri32.ensure(2 * n);
             launch_fill<float>(t->d32.p, 1.0f, 2 * n, s);
             launch_fill<float>(t->ri32.p, 0.5f, n, s);
-            if (op == OP_DIR_APPLY)
-                run(t->ri32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr, t->d32.p, t->d32.p + n);
-            else
-                run(t->d32.p, t->q32.p, L.scr ? sbuf_for<float>(t) : (float*)nullptr, (float*)nullptr, (float*)nullptr);
+            launch_fill<float>(t->q32.p, 0.125f, n, s);
+            float* sb = L.scr ? sbuf_for<float>(t) : (float*)nullptr;
+            if (op == OP_DIR_APPLY) run(t->ri32.p, t->q32.p, sb, t->d32.p, t->d32.p + n, (float*)nullptr, (float*)nullptr);
+            else if (op == OP_CG) run(t->ri32.p, t->q32.p + n, sb, t->d32.p, t->d32.p + n, t->q32.p, t->ri32.p + n);
+            else run(t->d32.p, t->q32.p, sb, (float*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr);
         } else {
             t->d64.ensure(2 * n);
-            t->q64.ensure(n);
-            t->r64.ensure(n);
+            t->q64.ensure(2 * n);
+            t->r64.ensure(2 * n);
             launch_fill<double>(t->d64.p, 1.0, 2 * n, s);
             launch_fill<double>(t->r64.p, 0.5, n, s);
-            if (op == OP_DIR_APPLY)
-                run(t->r64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr, t->d64.p, t->d64.p + n);
-            else
-                run(t->d64.p, t->q64.p, L.scr ? sbuf_for<double>(t) : (double*)nullptr, (double*)nullptr, (double*)nullptr);
+            launch_fill<double>(t->q64.p, 0.125, n, s);
+            double* sb = L.scr ? sbuf_for<double>(t) : (double*)nullptr;
+            if (op == OP_DIR_APPLY) run(t->r64.p, t->q64.p, sb, t->d64.p, t->d64.p + n, (double*)nullptr, (double*)nullptr);
+            else if (op == OP_CG) run(t->r64.p, t->q64.p + n, sb, t->d64.p, t->d64.p + n, t->q64.p, t->r64.p + n);
+            else run(t->d64.p, t->q64.p, sb, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr);
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1197,6 +1280,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->mg_omega = 0.0;
     o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
+    o->cg_two_pass = 0;
 }
 
 int prx_tree_create(uint32_t depth, int device, prx_tree** out) {
@@ -1488,12 +1572,12 @@ int prx_cg(prx_tree* t, int precision, const double* b, double* x, uint64_t max_
     DArr<double> db, dx;
     db.ensure(n);
     dx.ensure(n);
-    t->r64.ensure(n); t->d64.ensure(n); t->q64.ensure(n);
+    t->r64.ensure(2 * n); t->d64.ensure(2 * n); t->q64.ensure(2 * n);
     PRX_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
     PRX_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
     double bb = residual_mode == PRX_RESID_TRUE ? prx::dev_dot<double>(t, db.p, db.p, n) : 0.0;
-    t->d64.ensure(2 * n);
-    prx::Cg<double> cg = prx::make_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, refresh_period);
+    prx::Cg<double> cg = prx::make_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, refresh_period,
+                                              t->r64.p + n, t->q64.p + n);
     prx::CgOut o = prx::run_cg<double>(cg, max_iters, eps, residual_mode, bb);
     PRX_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     PRX_HIP(hipStreamSynchronize(s));

@@ -25,10 +25,12 @@ def clouds():
 
 
 def single(D, pts, pw, eps, mode, refresh):
+    # the batched cascadic solve runs the two-pass CG iteration; the single-tree solve with the same form is the
+    # reference for round-off identity (the fused form is the same iteration with other rounding)
     t = pr.FEMTree(D)
     t.splat_normal_field(pts)
     rep = t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=pr.PRX_PREC_FP64, residual_mode=mode,
-                           refresh_period=refresh)
+                           refresh_period=refresh, cg_two_pass=True)
     return t.solution, rep, t
 
 

@@ -175,8 +175,9 @@ def test_multicolour_gs_equals_colour_ordered_sweep(gpu, D):
     assert np.abs(xf - xr).max() <= 1e-4 * np.abs(xr).max()
 
 
-def test_cg_matches_direct_solve(gpu):
-    D = 4
+@pytest.mark.parametrize("D", [2, 3, 4])
+def test_cg_matches_direct_solve(gpu, D):
+    # depth 2: two-pass CG on the per-node kernel; 3 and 4: the fused CG step on 8^3 and 16^3 tiles
     pts = unit_points("sphere", 2000, seed=4)
     g2i, i2o = oracle.tree(D)
     A = oracle.assemble(D, g2i, i2o, pts, 4.0)
@@ -213,6 +214,35 @@ def test_cascadic_c1_field_matches_oracle(gpu, prec):
     rng_f = corners_o.max() - corners_o.min()
     assert np.abs(fg - fo).max() <= 1e-5 * rng_f
     assert np.abs(x - xo).max() <= 1e-5 * rng_f
+
+
+@pytest.mark.parametrize("prec,mode,refresh", [(pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 10),
+                                               (pr.PRX_PREC_FP64, pr.PRX_RESID_REFERENCE, 10),
+                                               (pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 0),
+                                               (pr.PRX_PREC_MIXED, pr.PRX_RESID_TRUE, 0)])
+def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh):
+    # The fused CG iteration (the update r -= alpha q joined to the next operator pass, |r|^2 expanded from d.q, r.q
+    # and q.q, r and q in ping-pong pairs, x in pairs) is the two-pass iteration of src/solvers.rs:36-64 with other
+    # rounding: iteration counts within 2 % (10 % for the mixed solve, whose FP64 refinement points follow the FP32
+    # residual estimate) and solutions within the solve's accuracy.  refresh 10 exercises the periodic r = b - A x of
+    # the reference, the mixed solve the residual replacements.
+    D, P = 6, 20000
+    pts = unit_points("torus", P, seed=3)
+    sols, reps = [], []
+    for two_pass in (True, False):
+        t = make_tree(D, pts, 4.0)
+        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 40000, 1e-8, precision=prec, residual_mode=mode,
+                               refresh_period=refresh, cg_two_pass=two_pass)
+        assert rep["status"] == 0
+        sols.append(t.solution)
+        reps.append(rep)
+    slack = 10 if prec == pr.PRX_PREC_MIXED else 50
+    assert abs(int(reps[0]["fine_iters"]) - int(reps[1]["fine_iters"])) <= max(3, int(reps[0]["fine_iters"]) // slack)
+    assert abs(int(reps[0]["coarse_iters"]) - int(reps[1]["coarse_iters"])) <= max(3, int(reps[0]["coarse_iters"]) // 50)
+    if mode == pr.PRX_RESID_TRUE:
+        assert reps[1]["true_rel"] <= 1e-8
+    rng_f = sols[0].max() - sols[0].min()
+    assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
 
 
 def test_evaluate_bit_exact(gpu):
