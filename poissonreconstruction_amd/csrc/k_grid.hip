@@ -287,11 +287,11 @@ __device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
 // sums its own r.r again, so the expansion's rounding never accumulates over
 // iterations (carried from one to the next, it would: its absolute error,
 // about eps * the early |r|^2, swamps |r|^2 once the residual has fallen by
-// ~1e-6).  x: a pair of pending terms was added before the pass (xn == 2 ->
-// 0), then alpha_k d_k is pending.  refresh: the caller recomputes r = b - A x
-// next (FIN_FRESH decides).
+// ~1e-6).  x: CG_XM pending terms were added before the pass (xn == CG_XM
+// -> 0), then alpha_k d_k is pending.  refresh: the caller recomputes r = b -
+// A x next (FIN_FRESH decides).
 __device__ inline void fin_cg(CgState* st, double dq, double rq, double qq, double rr, int refresh) {
-    if (st->xn == 2) st->xn = 0;
+    if (st->xn == CG_XM) st->xn = 0;
     st->dq = dq;
     st->delta = rr;
     if (dq == 0.0) {
@@ -301,12 +301,8 @@ __device__ inline void fin_cg(CgState* st, double dq, double rq, double qq, doub
     }
     const double a = st->delta / dq;
     st->alpha = a;
-    if (st->xn == 0) {
-        st->xn = 1;
-        st->apend = a;
-    } else {
-        st->xn = 2;
-    }
+    st->xa[st->xn] = a;
+    st->xn += 1;
     st->rpend = 1;
     const double g = (st->delta - 2.0 * a * rq) + a * a * qq;
     st->delta_prev = st->delta;
@@ -642,59 +638,65 @@ void launch_clear_xpend(CgState* st, hipStream_t s) {
 }
 
 // Fused CG x terms (n a multiple of the vector width: the fused step runs on
-// tiled levels only).  MODE 0: the pair before a pass (xn == 2, not done);
-// MODE 1: flush whatever is pending; MODE 2: the same while not done.
+// tiled levels only): x + xa[0] d[0] + xa[1] d[1] + ... in FP64, rounded
+// once.  MODE 0: all CG_XM terms, only when all are pending and the solve
+// runs; MODE 1: whatever is pending; MODE 2: the same while not done.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_x_terms(T* __restrict__ x, const T* __restrict__ da, const T* __restrict__ db,
-                                                 const CgState* st, int64_t nv) {
+__global__ __launch_bounds__(256) void k_x_terms(T* __restrict__ x, XDirs<T> dd, const CgState* st, int64_t nv) {
     if (MODE != 1 && st->done) return;
     const int xn = st->xn;
-    if (MODE == 0 ? xn != 2 : xn == 0) return;
+    if (MODE == 0 ? xn != CG_XM : xn == 0) return;
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
-    const double a = st->alpha, ap = st->apend;
+    double a[CG_XM];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int64_t i = (int64_t)blockIdx.x * 1024 + u * 256 + threadIdx.x;
+    for (int j = 0; j < CG_XM; ++j) a[j] = st->xa[j];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int64_t i = (int64_t)blockIdx.x * 512 + u * 256 + threadIdx.x;
         if (i < nv) {
-            const int64_t j = i * VEC;
-            NV xv = ld_nt(x + j);
-            const NV bv = ld_nt(db + j);
-            if (xn == 2) {
-                const NV av = ld_nt(da + j);
+            const int64_t g = i * VEC;
+            const NV xv = ld_nt(x + g);
+            NV dv[CG_XM];
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) xv[e] = x_pair<T>(xv[e], av[e], bv[e], ap, a);
-            } else {
+            for (int j = 0; j < CG_XM; ++j)
+                if (j < xn) dv[j] = ld_nt(dd.d[j] + g);
+            NV xo;
 #pragma unroll
-                for (int e = 0; e < VEC; ++e) xv[e] = (T)((double)xv[e] + ap * (double)bv[e]);
+            for (int e = 0; e < VEC; ++e) {
+                double v = (double)xv[e];
+#pragma unroll
+                for (int j = 0; j < CG_XM; ++j)
+                    if (j < xn) v += a[j] * (double)dv[j][e];
+                xo[e] = (T)v;
             }
-            st_nt(x + j, xv);
+            st_nt(x + g, xo);
         }
     }
 }
 template <typename T>
-static void launch_x_terms(int mode, T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s) {
+static void launch_x_terms(int mode, T* x, const XDirs<T>& d, const CgState* st, int64_t n, hipStream_t s) {
     constexpr int VEC = 16 / (int)sizeof(T);
     if (n % VEC) throw Error(PRX_ERR_INVALID, "fused CG vectors must be a multiple of the vector width");
     const int64_t nv = n / VEC;
-    const int nb = (int)std::max<int64_t>(1, (nv + 1023) / 1024);
-    if (mode == 0) hipLaunchKernelGGL((k_x_terms<T, 0>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
-    else if (mode == 1) hipLaunchKernelGGL((k_x_terms<T, 1>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
-    else hipLaunchKernelGGL((k_x_terms<T, 2>), dim3(nb), dim3(256), 0, s, x, da, db, st, nv);
+    const int nb = (int)std::max<int64_t>(1, (nv + 511) / 512);
+    if (mode == 0) hipLaunchKernelGGL((k_x_terms<T, 0>), dim3(nb), dim3(256), 0, s, x, d, st, nv);
+    else if (mode == 1) hipLaunchKernelGGL((k_x_terms<T, 1>), dim3(nb), dim3(256), 0, s, x, d, st, nv);
+    else hipLaunchKernelGGL((k_x_terms<T, 2>), dim3(nb), dim3(256), 0, s, x, d, st, nv);
     PRX_LAUNCH_CHECK();
 }
 template <typename T>
-void launch_x_pair(T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s) {
-    launch_x_terms<T>(0, x, da, db, st, n, s);
+void launch_x_terms_full(T* x, const XDirs<T>& d, const CgState* st, int64_t n, hipStream_t s) {
+    launch_x_terms<T>(0, x, d, st, n, s);
 }
 template <typename T>
-void launch_x_flush(T* x, const T* da, const T* db, const CgState* st, int64_t n, bool if_running, hipStream_t s) {
-    launch_x_terms<T>(if_running ? 2 : 1, x, da, db, st, n, s);
+void launch_x_flush(T* x, const XDirs<T>& d, const CgState* st, int64_t n, bool if_running, hipStream_t s) {
+    launch_x_terms<T>(if_running ? 2 : 1, x, d, st, n, s);
 }
-template void launch_x_pair<double>(double*, const double*, const double*, const CgState*, int64_t, hipStream_t);
-template void launch_x_pair<float>(float*, const float*, const float*, const CgState*, int64_t, hipStream_t);
-template void launch_x_flush<double>(double*, const double*, const double*, const CgState*, int64_t, bool, hipStream_t);
-template void launch_x_flush<float>(float*, const float*, const float*, const CgState*, int64_t, bool, hipStream_t);
+template void launch_x_terms_full<double>(double*, const XDirs<double>&, const CgState*, int64_t, hipStream_t);
+template void launch_x_terms_full<float>(float*, const XDirs<float>&, const CgState*, int64_t, hipStream_t);
+template void launch_x_flush<double>(double*, const XDirs<double>&, const CgState*, int64_t, bool, hipStream_t);
+template void launch_x_flush<float>(float*, const XDirs<float>&, const CgState*, int64_t, bool, hipStream_t);
 
 // d = r + beta d   (src/solvers.rs:64)
 template <typename T>

@@ -15,8 +15,10 @@ namespace prx {
 // the latest direction (the FIN_BETA caller passes it as the slot).
 // Fused CG step (OP_CG + FIN_CG): `rpend` marks the update r -= alpha q of
 // the last iteration as still to be formed (by the next pass, from the r and
-// q it wrote); `xn` counts the terms alpha d not yet added to x (the older
-// one with coefficient apend, the newer with alpha).
+// q it wrote); `xn` counts the terms alpha_j d_j not yet added to x, with
+// coefficients xa[0..xn) (oldest first).  x takes CG_XM of them at once, just
+// before the pass that overwrites the oldest direction of the ring.
+constexpr int CG_XM = 4;
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
     double bnorm2;
@@ -25,7 +27,7 @@ struct CgState {
     int32_t done, status, refresh_due, lazy;
     double alpha_pend;
     int32_t xpend, dir_buf;
-    double apend;
+    double xa[CG_XM];
     int32_t rpend, xn;
 };
 
@@ -92,12 +94,17 @@ template <typename T> void launch_axpy_alpha(T* x, const T* dprev, const T* d, c
 // x += alpha_pend * d{dir_buf} when a lazy term is pending (the caller then clears xpend)
 template <typename T> void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s);
 void launch_clear_xpend(CgState* st, hipStream_t s);
-// fused CG: x += apend * da + alpha * db while xn == 2 and not done (the pair
-// is added before the pass that overwrites da)
-template <typename T> void launch_x_pair(T* x, const T* da, const T* db, const CgState* st, int64_t n, hipStream_t s);
-// fused CG: add the pending terms (xn == 1: apend * db; xn == 2: apend * da +
-// alpha * db); if_running: only while the solve is not done
-template <typename T> void launch_x_flush(T* x, const T* da, const T* db, const CgState* st, int64_t n, bool if_running, hipStream_t s);
+// fused CG: x += sum_j xa[j] * d[j] over the pending terms (d[j] the
+// direction of term j, oldest first).  launch_x_terms_full adds them only
+// when all CG_XM are pending and the solve runs (before the pass that
+// overwrites d[0]); launch_x_flush adds whatever is pending (if_running: only
+// while the solve is not done).
+template <typename T>
+struct XDirs {
+    const T* d[CG_XM];
+};
+template <typename T> void launch_x_terms_full(T* x, const XDirs<T>& d, const CgState* st, int64_t n, hipStream_t s);
+template <typename T> void launch_x_flush(T* x, const XDirs<T>& d, const CgState* st, int64_t n, bool if_running, hipStream_t s);
 
 // ---- k_mg.hip: geometric multigrid on the dense Morton grids
 // x = w * b

@@ -129,11 +129,13 @@ static double dev_dot(prx_tree* t, const T* a, const T* b, int64_t n) {
 // Fused form (tiled levels, when second r and q buffers are given): one
 // operator pass per iteration (OP_CG) also forms the previous iteration's
 // residual update r - alpha q, and one fold (FIN_CG) yields alpha, the new
-// |r|^2 and beta.  r and q then live in ping-pong pairs like d: pass k reads
-// rr[k&1], qq[k&1], dd[k&1] and writes rr[(k+1)&1], qq[(k+1)&1],
-// dd[(k+1)&1].  x takes its terms in pairs before the pass that overwrites
-// the older direction.  Per iteration this reads r, q, d and writes r, d, q
-// (24 bytes per node in FP32) where the two-pass form moves 28.
+// |r|^2 and beta.  r and q then live in ping-pong pairs: pass k reads
+// rr[k&1], qq[k&1] and writes rr[(k+1)&1], qq[(k+1)&1]; the directions live
+// in a ring of CG_XM: pass k reads dd[k % CG_XM] and writes dd[(k+1) % CG_XM]
+// (d_j sits in slot (j+1) % CG_XM).  x takes CG_XM terms at once, before the
+// pass that overwrites the oldest direction.  Per iteration this reads r, q,
+// d and writes r, d, q (24 bytes per node in FP32) plus 6 for x, where the
+// two-pass form moves 28 plus 8.
 template <typename T>
 struct Cg {
     prx_tree* t;
@@ -141,7 +143,7 @@ struct Cg {
     T* x;
     const T* b;
     T* r;
-    T* dd[2];
+    T* dd[CG_XM] = {};
     T* q;
     T* rr[2] = {nullptr, nullptr};
     T* qq[2] = {nullptr, nullptr};
@@ -182,12 +184,18 @@ struct Cg {
         it = 0;
         xn = 0;
     }
+    // the directions of the xn terms pending after `done_it` iterations
+    XDirs<T> pending_dirs(uint64_t done_it, int n_terms) const {
+        XDirs<T> d{};
+        for (int i = 0; i < n_terms; ++i) d.d[i] = dd[(done_it - (uint64_t)n_terms + (uint64_t)i + 1) % CG_XM];
+        return d;
+    }
     void enqueue_fused(uint64_t k) {
         const hipStream_t s = t->stream;
         const int32_t* done = &t->st->done;
-        T* dold = dd[k & 1];  // d_{k-1}
-        T* dnew = dd[(k + 1) & 1];  // holds d_{k-2} until this pass writes d_k
-        if (xn == 2) launch_x_pair<T>(x, dnew, dold, t->st, L.N, s);
+        T* dold = dd[k % CG_XM];  // d_{k-1}
+        T* dnew = dd[(k + 1) % CG_XM];  // holds d_{k-CG_XM} until this pass writes d_k
+        if (xn == CG_XM) launch_x_terms_full<T>(x, pending_dirs(k, CG_XM), t->st, L.N, s);
         StencilIO<T> io;
         io.r = rr[k & 1];
         io.q = qq[k & 1];
@@ -208,9 +216,9 @@ struct Cg {
         applies++;
         const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
         launch_fin(t->st, t->partial.p, nb, FIN_CG, fresh ? 1 : 0, s);
-        xn = xn == 2 ? 1 : xn + 1;
+        xn = xn == CG_XM ? 1 : xn + 1;
         if (fresh) {  // x completed, then r = b - A x replaces the pending update
-            launch_x_flush<T>(x, dold, dnew, t->st, L.N, true, s);
+            launch_x_flush<T>(x, pending_dirs(k + 1, xn), t->st, L.N, true, s);
             nb = op_pass<T>(t, L, OP_RESID, x, b, io.rout, done, t->partial.p);
             applies++;
             launch_fin(t->st, t->partial.p, nb, FIN_FRESH, 0, s);
@@ -265,8 +273,8 @@ struct Cg {
             if (t->st_host->done || k >= cap) break;
         }
         it = (uint64_t)t->st_host->it;
-        if (fused && t->st_host->xn) {  // d_{it-1} in dd[it&1], d_{it-2} in the other
-            launch_x_flush<T>(x, dd[(it + 1) & 1], dd[it & 1], t->st, L.N, false, t->stream);
+        if (fused && t->st_host->xn) {
+            launch_x_flush<T>(x, pending_dirs(it, t->st_host->xn), t->st, L.N, false, t->stream);
             launch_clear_xpend(t->st, t->stream);
             t->st_host->xn = 0;
         } else if (t->st_host->xpend) {
@@ -295,9 +303,10 @@ struct Cg {
     }
 };
 
-// r1, q1: second residual and product buffers (the fused form on tiled levels)
+// d: 2 direction buffers of L.N, or CG_XM for the fused form on tiled levels,
+// which also needs r1, q1 (second residual and product buffers)
 template <typename T>
-static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d0, T* d1, T* q, int refresh, T* r1 = nullptr,
+static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T* d, T* q, int refresh, int nd, T* r1 = nullptr,
                      T* q1 = nullptr) {
     Cg<T> c;
     c.t = t;
@@ -305,11 +314,10 @@ static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T*
     c.x = x;
     c.b = b;
     c.r = r;
-    c.dd[0] = d0;
-    c.dd[1] = d1;
+    for (int i = 0; i < CG_XM; ++i) c.dd[i] = i < nd ? d + (int64_t)i * L.N : nullptr;
     c.q = q;
     c.refresh = refresh;
-    c.fused = r1 && q1 && L.D >= 3 && t->fused_cg;
+    c.fused = r1 && q1 && nd >= CG_XM && L.D >= 3 && t->fused_cg;
     c.rr[0] = r;
     c.rr[1] = r1;
     c.qq[0] = q;
@@ -383,13 +391,13 @@ static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, d
     L.res = 1u << cdepth;
     L.N = nc;
     L.k64 = {stiffness_entry(hc, 0, 0, 0), stiffness_entry(hc, 1, 1, 0), stiffness_entry(hc, 1, 1, 1), epw > 0.0 ? wh3c * 1.005 : 0.0, 0.0};
-    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(2 * nc); t->cdd.ensure(2 * nc); t->cq.ensure(2 * nc);
+    t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(2 * nc); t->cdd.ensure(CG_XM * nc); t->cq.ensure(2 * nc);
     launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
     launch_fill<double>(t->cx.p, 0.0, nc, s);
     gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
     const bool tru = mode == PRX_RESID_TRUE;
     const double bb = tru ? dev_dot<double>(t, t->cb.p, t->cb.p, nc) : 0.0;
-    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cdd.p + nc, t->cq.p, tru ? 0 : 10, t->cr.p + nc,
+    Cg<double> cg = make_cg<double>(t, L, t->cx.p, t->cb.p, t->cr.p, t->cdd.p, t->cq.p, tru ? 0 : 10, CG_XM, t->cr.p + nc,
                                     t->cq.p + nc);
     CgOut c = run_cg<double>(cg, iters, eps, tru ? PRX_RESID_TRUE : PRX_RESID_REFERENCE, bb);
     rep.coarse_iters = c.iters;
@@ -412,7 +420,7 @@ static double true_rel(prx_tree* t) {
 static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solve_report& rep) {
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
-    t->r64.ensure(2 * n); t->d64.ensure(2 * n); t->q64.ensure(2 * n);
+    t->r64.ensure(2 * n); t->d64.ensure(CG_XM * n); t->q64.ensure(2 * n);
     {
         Timer tm(t->stream);
         gs_sweeps<double>(t, L, t->x.p, t->b.p, c.gs);
@@ -421,8 +429,8 @@ static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_so
     Timer tm(t->stream);
     double bb = 0.0;
     if (c.o.residual_mode == PRX_RESID_TRUE) bb = dev_dot<double>(t, t->b.p, t->b.p, n);
-    Cg<double> cg = make_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, c.o.refresh_period,
-                                    t->r64.p + n, t->q64.p + n);
+    Cg<double> cg = make_cg<double>(t, L, t->x.p, t->b.p, t->r64.p, t->d64.p, t->q64.p, c.o.refresh_period, CG_XM, t->r64.p + n,
+                                    t->q64.p + n);
     CgOut o = run_cg<double>(cg, cap, c.eps, c.o.residual_mode, bb);
     rep.ms_fine_cg = tm.ms();
     rep.fine_iters = o.iters;
@@ -445,7 +453,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const LevelView L = t->fine_view();
     const int64_t n = t->N;
     const hipStream_t s = t->stream;
-    t->e32.ensure(n); t->ri32.ensure(2 * n); t->d32.ensure(2 * n); t->q32.ensure(2 * n);
+    t->e32.ensure(n); t->ri32.ensure(2 * n); t->d32.ensure(CG_XM * n); t->q32.ensure(2 * n);
     const double ih = 1.0 / t->h;
     const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.02;
     const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
@@ -468,8 +476,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const double thr32 = thr * ih * ih;  // the same test on the scaled FP32 residual
     uint64_t outer = 0;
     int status = PRX_OK;
-    Cg<float> cg = make_cg<float>(t, L, t->e32.p, nullptr, t->ri32.p, t->d32.p, t->d32.p + n, t->q32.p, 0, t->ri32.p + n,
-                                  t->q32.p + n);
+    Cg<float> cg = make_cg<float>(t, L, t->e32.p, nullptr, t->ri32.p, t->d32.p, t->q32.p, 0, CG_XM, t->ri32.p + n, t->q32.p + n);
     if (rr > thr && rr > 0.0) {
         launch_fill<float>(t->e32.p, 0.0f, n, s);
         cg.start_from_residual(1.0, cap);
@@ -1572,11 +1579,11 @@ int prx_cg(prx_tree* t, int precision, const double* b, double* x, uint64_t max_
     DArr<double> db, dx;
     db.ensure(n);
     dx.ensure(n);
-    t->r64.ensure(2 * n); t->d64.ensure(2 * n); t->q64.ensure(2 * n);
+    t->r64.ensure(2 * n); t->d64.ensure(CG_XM * n); t->q64.ensure(2 * n);
     PRX_HIP(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
     PRX_HIP(hipMemcpyAsync(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
     double bb = residual_mode == PRX_RESID_TRUE ? prx::dev_dot<double>(t, db.p, db.p, n) : 0.0;
-    prx::Cg<double> cg = prx::make_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->d64.p + n, t->q64.p, refresh_period,
+    prx::Cg<double> cg = prx::make_cg<double>(t, L, dx.p, db.p, t->r64.p, t->d64.p, t->q64.p, refresh_period, prx::CG_XM,
                                               t->r64.p + n, t->q64.p + n);
     prx::CgOut o = prx::run_cg<double>(cg, max_iters, eps, residual_mode, bb);
     PRX_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
