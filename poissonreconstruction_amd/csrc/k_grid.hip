@@ -25,7 +25,9 @@ static inline int nblocks_for(int64_t n, int per_thread = 1, int threads = 256, 
 static inline int red_blocks(int64_t n) { return nblocks_for(n, 8, 256, 2048); }
 
 int partial_capacity(int64_t N) {
-    int64_t b = 4 * ((N + 511) / 512);  // four per brick (8^3 or 16^3; fused CG step), one per 512 nodes of k_cg_update
+    // four per brick (8^3 or 16^3; fused CG step) and the first-stage fold
+    // behind them, one per 512 nodes of k_cg_update
+    int64_t b = 4 * ((N + 511) / 512) + 4 * 128;
     int64_t r = red_blocks(N);
     int64_t nv = (N + 255) / 256;   // naive stencil: one partial per block
     int64_t m = b > r ? b : r;
@@ -357,7 +359,51 @@ __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial
     if (threadIdx.x != 0) return;
     fin_step(st, sum, op, slot);
 }
+// FIN_CG first stage: block b folds the four interleaved sums of bricks
+// [b*chunk, min(np, (b+1)*chunk)) in a fixed order (per thread in brick
+// order, then the 64-lane butterfly, then the waves in order) into
+// out[4b .. 4b+3], so the single-block fold reads 4 values per block
+// instead of 4 per brick.
+__global__ __launch_bounds__(256) void k_fold4(const CgState* st, const double* __restrict__ partial, int np, int chunk,
+                                               double* __restrict__ out) {
+    __shared__ double ws[16];
+    if (st->done) return;
+    typedef double nd2 __attribute__((ext_vector_type(2)));
+    const int b0 = blockIdx.x * chunk, b1 = min(np, b0 + chunk);
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = b0 + (int)threadIdx.x; i < b1; i += 256) {
+        const nd2 u = *(const nd2*)(partial + 4 * (int64_t)i);
+        const nd2 v = *(const nd2*)(partial + 4 * (int64_t)i + 2);
+        a[0] += u[0];
+        a[1] += u[1];
+        a[2] += v[0];
+        a[3] += v[1];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_xor(a[j], o, 64);
+        if (lane == 0) ws[j * 4 + wave] = a[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double sum = 0.0;
+        for (int w = 0; w < 4; ++w) sum += ws[threadIdx.x * 4 + w];
+        out[4 * blockIdx.x + threadIdx.x] = sum;
+    }
+}
+
 void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s) {
+    if (op == FIN_CG && np > 256) {  // two stages: up to 128 blocks, then one
+        const int g = std::min(128, (np + 1023) / 1024);
+        const int chunk = (np + g - 1) / g;
+        double* out = const_cast<double*>(partial) + 4 * (int64_t)np;
+        hipLaunchKernelGGL(k_fold4, dim3(g), dim3(256), 0, s, st, partial, np, chunk, out);
+        PRX_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, s, st, out, g, op, slot);
+        PRX_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, s, st, partial, np, op, slot);
     PRX_LAUNCH_CHECK();
 }
