@@ -18,7 +18,7 @@ namespace prx {
 // q it wrote); `xn` counts the terms alpha_j d_j not yet added to x, with
 // coefficients xa[0..xn) (oldest first).  x takes CG_XM of them at once, just
 // before the pass that overwrites the oldest direction of the ring.
-constexpr int CG_XM = 4;
+constexpr int CG_XM = 8;
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
     double bnorm2;
