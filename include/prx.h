@@ -72,7 +72,7 @@ typedef struct prx_params {
     int32_t gs_iters;       /* smoothing sweeps per level (reference: 20) */
     int32_t base_depth;     /* -1: D-2 rule of src/reconstructor.rs:51 */
     int32_t refresh_period; /* CG true-residual refresh period (reference: 10; 0 = never) */
-    double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.02) */
+    double inner_tol;       /* mixed: FP32 residual reduction between FP64 residual replacements (default 0.05) */
     uint64_t max_outer;     /* mixed: cap on FP64 residual replacements (default 64) */
     int32_t precond;        /* PRX_PRECOND_NONE (default: the reference's cascadic solve) or PRX_PRECOND_MG */
 } prx_params;

@@ -227,7 +227,7 @@ int prx_reconstruct(const prx_point* points, size_t n, const prx_params* params,
     o.residual_mode = p.residual_mode;
     o.refresh_period = p.refresh_period;
     o.gs_iters = p.gs_iters;
-    o.inner_tol = p.inner_tol > 0.0 ? p.inner_tol : 0.02;
+    o.inner_tol = p.inner_tol > 0.0 ? p.inner_tol : 0.05;
     o.precond = p.precond;
     o.max_outer = p.max_outer > 0 ? p.max_outer : 64;
     prx_solve_report rep;

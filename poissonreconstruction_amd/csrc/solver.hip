@@ -455,7 +455,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     const hipStream_t s = t->stream;
     t->e32.ensure(n); t->ri32.ensure(2 * n); t->d32.ensure(CG_XM * n); t->q32.ensure(2 * n);
     const double ih = 1.0 / t->h;
-    const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.02;
+    const double theta = c.o.inner_tol > 0.0 && c.o.inner_tol < 1.0 ? c.o.inner_tol : 0.05;
     const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
     // smoothing on the correction: e = GS^k(0; A/h, r/h), x += e
     {
@@ -1109,7 +1109,7 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
         c.o.precision = precision;
         c.o.residual_mode = PRX_RESID_TRUE;
         c.o.refresh_period = 0;
-        c.o.inner_tol = 0.02;
+        c.o.inner_tol = 0.05;
         c.o.max_outer = 200;
         if (t->precond == PRX_PRECOND_MG) {
             c.o.precond = PRX_PRECOND_MG;
@@ -1270,7 +1270,7 @@ void prx_params_default(prx_params* p) {
     p->gs_iters = 20;
     p->base_depth = -1;
     p->refresh_period = 10;
-    p->inner_tol = 0.02;
+    p->inner_tol = 0.05;
     p->max_outer = 64;
 }
 
@@ -1280,7 +1280,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->residual_mode = PRX_RESID_REFERENCE;
     o->refresh_period = 10;
     o->gs_iters = 20;
-    o->inner_tol = 0.02;
+    o->inner_tol = 0.05;
     o->max_outer = 64;
     o->precond = PRX_PRECOND_NONE;
     o->mg_smooth = 2;

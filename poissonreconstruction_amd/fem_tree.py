@@ -165,7 +165,7 @@ class FEMTree:
     # ------------------------------------------------------------------ solve
     def solve_cascadic(self, points, pw: float, base_depth: int, gs: int, cg: int, eps: float,
                        precision: int = L.PRX_PREC_FP64, residual_mode: int = L.PRX_RESID_REFERENCE,
-                       refresh_period: int = 10, inner_tol: float = 0.02, max_outer: int = 64,
+                       refresh_period: int = 10, inner_tol: float = 0.05, max_outer: int = 64,
                        precond: int = L.PRX_PRECOND_NONE, mg_smooth: int = 2, mg_omega: float = 0.0,
                        mg_inner_tol: float = 1e-4, cg_two_pass: bool = False) -> dict:
         """src/fem_tree.rs:458-487; returns the solve report.  precond

@@ -31,7 +31,7 @@ class ReconstructParams:
     gs_iters: int = 20
     base_depth: int = -1
     refresh_period: int = 10
-    inner_tol: float = 0.02
+    inner_tol: float = 0.05
     max_outer: int = 64
     precond: int = L.PRX_PRECOND_NONE
 
