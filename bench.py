@@ -207,6 +207,7 @@ def main():
         "field_error": field or None,
         "multigrid": mg,
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
+        "cg_phase": cg_phase_bandwidth(N, P, rep_mixed, fp64[-1][1] if fp64 else None),
         "roofline": roofline,
         "marching_cubes": mc,
         "adjoint": adjoint,
@@ -399,6 +400,25 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
     out["multigrid"] = mg
     B.close()
     log(f"batch {out}")
+    return out
+
+
+def cg_phase_bandwidth(N, P, rep_mixed, rep_fp64):
+    """Effective bandwidth of the fine CG phase: the iterations' algorithmic
+    bytes over the phase's time.  One fused CG iteration moves 24 bytes per
+    node in the operator pass (r, q, d read; r, d, q written) plus 5 for x
+    (eight directions and x read, x written, every eighth iteration), and the
+    points' positions (12 bytes each) for the screening; twice that in FP64.
+    The phase time also holds the mixed solve's FP64 refinement passes and the
+    folds, so this is a lower bound on what the kernels sustain."""
+    out = {}
+    for name, rep, w in (("mixed", rep_mixed, 4), ("fp64", rep_fp64, 8)):
+        if rep is None or not rep.get("ms_fine_cg"):
+            continue
+        per_iter = (29 * N + 12 * P) * w // 4
+        gbs = rep["fine_iters"] * per_iter / (rep["ms_fine_cg"] * 1e-3) / 1e9
+        out[name] = {"bytes_per_iter": per_iter, "iters": rep["fine_iters"], "ms": round(rep["ms_fine_cg"], 2),
+                     "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     return out
 
 
