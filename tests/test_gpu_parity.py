@@ -245,6 +245,27 @@ def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh):
     assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
 
 
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_fused_cg_step_on_8cubed_tiles(gpu, prec, monkeypatch):
+    # The tile edge is chosen per tree (PRX_TILE=8 at creation): the 8^3-brick form of the fused CG step, the GS pass
+    # and the folds solve the same system as the 16^3 form (depth 6: 512 bricks of 8^3 against 64 of 16^3).
+    D, P = 6, 20000
+    pts = unit_points("torus", P, seed=5)
+    sols, reps = [], []
+    for tile in ("16", "8"):
+        monkeypatch.setenv("PRX_TILE", tile)
+        t = make_tree(D, pts, 4.0)
+        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 40000, 1e-8, precision=prec, residual_mode=pr.PRX_RESID_TRUE,
+                               refresh_period=0)
+        assert rep["status"] == 0 and rep["true_rel"] <= 1e-8
+        sols.append(t.solution)
+        reps.append(rep)
+    slack = 10 if prec == pr.PRX_PREC_MIXED else 50
+    assert abs(int(reps[0]["fine_iters"]) - int(reps[1]["fine_iters"])) <= max(3, int(reps[0]["fine_iters"]) // slack)
+    rng_f = sols[0].max() - sols[0].min()
+    assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
+
+
 def test_evaluate_bit_exact(gpu):
     D = 5
     pts = unit_points("sphere", 3000, seed=6)
