@@ -62,15 +62,17 @@ class Batch:
 
     def solve(self, gs: int = 20, cg_iters: int = 800000, eps: float = 1e-6, residual_mode: int = L.PRX_RESID_TRUE,
               refresh_period: int = 10, autotune: bool = True, precond: int = L.PRX_PRECOND_NONE,
-              precision: int = L.PRX_PREC_FP64):
+              precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False):
         """Returns (batch report dict, list of per-instance report dicts).
         precond PRX_PRECOND_MG solves every instance with multigrid-preconditioned
-        CG (mixed or FP64) instead of the batched cascadic solve (FP64)."""
+        CG (mixed or FP64) instead of the batched cascadic solve (FP64);
+        cg_two_pass selects the two-pass CG iteration instead of the fused one."""
         o = L.default_solve_opts()
         o.precision = int(precision)
         o.residual_mode = int(residual_mode)
         o.refresh_period = int(refresh_period)
         o.precond = int(precond)
+        o.cg_two_pass = 1 if cg_two_pass else 0
         rep = L.BatchReport()
         per = (L.SolveReport * max(len(self._meta), 1))()
         check(L.lib().prx_solve_batched(self._h, int(gs), int(cg_iters), float(eps), ctypes.byref(o), 1 if autotune else 0,

@@ -10,10 +10,12 @@
 // GS and CG (src/solvers.rs:20-69), with the stopping rules of the single
 // tree (reference ||r|| <= eps ||r0|| or the FP64 true residual).
 //
-// One CG iteration of all instances is five launches (screening gather,
-// fused direction update + operator, alpha, vector update, beta); a chunk of
-// iterations is captured once into a hipGraph and replayed until every
-// instance is done, and the GS sweeps are captured the same way.  An
+// One CG iteration of all instances is the fused CG step of the single tree
+// (screening gather, one operator pass, a two-stage fold; x every CG_XM
+// iterations) or, with cg_two_pass, five launches (gather, direction update +
+// operator, alpha, vector update, beta); a chunk of iterations is captured
+// once into a hipGraph and replayed until every instance is done, and the GS
+// sweeps are captured the same way.  An
 // autotuner picks the operator's brick size, the GS block size and the chunk
 // length by timing them on the batch itself.
 #include <algorithm>
@@ -86,6 +88,7 @@ struct BLevel {
 struct GraphEntry {
     const BLevel* L;
     int kind, chunk, refresh, tile, gsb;
+    bool fused;
     hipGraphExec_t exec;
 };
 
@@ -99,10 +102,11 @@ struct prx_batch {
     bool built = false;
     prx::DArr<prx::CgState> st;
     prx::CgState* st_host = nullptr;
-    prx::DArr<double> partial;
+    prx::DArr<double> partial, fold;
     prx::DArr<int32_t> flags, ones;
     std::vector<int32_t> flags_h;
     int tile = 16, chunk = 16, gs_block = 16;  // gs_block: brick edge of the GS passes
+    bool fused = true;                         // the fused CG step (prx_solve_opts.cg_two_pass == 0)
     bool tuned = false;
     double tune_ms = 0.0;
     uint64_t graph_launches = 0;
@@ -171,9 +175,9 @@ static void setup_level(prx_batch* B, BLevel& L, const std::vector<uint32_t>& de
     PRX_HIP(hipMemcpyAsync(L.brick_inst.p, bi.data(), sizeof(uint32_t) * bi.size(), hipMemcpyHostToDevice, s));
     L.x.ensure(L.N);
     L.b.ensure(L.N);
-    L.r.ensure(L.N);
-    L.dd.ensure(2 * L.N);
-    L.q.ensure(L.N);
+    L.r.ensure(2 * L.N);       // fused CG: r and q in ping-pong pairs
+    L.dd.ensure(CG_XM * L.N);  // directions: a ring of CG_XM (the two-pass form uses two)
+    L.q.ensure(2 * L.N);
     PRX_HIP(hipStreamSynchronize(s));  // bi goes out of scope
 }
 
@@ -287,7 +291,8 @@ static void build(prx_batch* B) {
     B->flags_h.assign(K, 0);
     std::vector<int32_t> one(K, 1);
     PRX_HIP(hipMemcpyAsync(B->ones.p, one.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, B->stream));
-    B->partial.ensure((std::max(B->fine.N, B->coarse.N) >> 9) + 64);
+    B->partial.ensure(4 * (std::max(B->fine.N, B->coarse.N) >> 9) + 64);  // four per 8^3 brick (fused CG step)
+    B->fold.ensure((int64_t)K * FOLD_BLOCKS * 4);
     PRX_HIP(hipStreamSynchronize(B->stream));
     B->built = true;
 }
@@ -295,12 +300,13 @@ static void build(prx_batch* B) {
 // ------------------------------------------------------------------ batched building blocks
 static int ulog_tile(int tile) { return tile == 8 ? 9 : 12; }
 
-// r = b - A x for the instances selected by `tab`; partial per brick.
-static void resid_pass(prx_batch* B, BLevel& L, const BatchTab& tab) {
+// r = b - A x (into y, default L.r) for the instances selected by `tab`;
+// partial per brick.
+static void resid_pass(prx_batch* B, BLevel& L, const BatchTab& tab, double* y = nullptr) {
     StencilIO<double> io;
     io.x = L.x.p;
     io.b = L.b.p;
-    io.y = L.r.p;
+    io.y = y ? y : L.r.p;
     io.partial = B->partial.p;
     if (L.has_scr) {
         launch_bscreen_gather<double>(tab, L.pts(), L.x.p, nullptr, nullptr, L.sbuf.p, nullptr, nullptr, B->stream);
@@ -326,7 +332,50 @@ static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
         }
 }
 
-// One CG iteration (index k) of every running instance.
+// One fused CG step (pass index k) of every running instance, as
+// Cg::enqueue_fused: pass k reads r, q from slot k&1 and the direction from
+// ring slot k % CG_XM and writes slot (k+1)&1 and ring slot (k+1) % CG_XM.
+// xn: the x terms pending before the pass (the same for every running
+// instance: they start or restart together at pass 0).
+static void cg_enqueue_fused(prx_batch* B, BLevel& L, uint64_t k, int refresh, int& xn) {
+    const hipStream_t s = B->stream;
+    const BatchTab tab = tab_running(B, L);
+    const int64_t N = L.N;
+    auto slot = [&](uint64_t j) { return L.dd.p + (int64_t)(j % CG_XM) * N; };
+    double* dold = slot(k);
+    double* dnew = slot(k + 1);
+    if (xn == CG_XM) {  // the instances also check their own count
+        XDirs<double> d{};
+        for (int i = 0; i < CG_XM; ++i) d.d[i] = slot(k + 1 + (uint64_t)i);
+        launch_bx_terms<double>(tab, L.x.p, d, L.dd.p, N, 0, s);
+    }
+    StencilIO<double> io;
+    io.r = L.r.p + (int64_t)(k & 1) * N;
+    io.q = L.q.p + (int64_t)(k & 1) * N;
+    io.rout = L.r.p + (int64_t)((k + 1) & 1) * N;
+    io.dold = dold;
+    io.dnew = dnew;
+    io.y = L.q.p + (int64_t)((k + 1) & 1) * N;
+    io.partial = B->partial.p;
+    if (L.has_scr) {
+        launch_bscreen_gather<double>(tab, L.pts(), nullptr, nullptr, nullptr, L.sbuf.p, io.r, dold, s, io.q);
+        io.s = L.sbuf.p;
+    }
+    launch_bstencil<double>(tab, N, L.scr_ptr(), OP_CG, io, B->tile, s);
+    const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
+    launch_bfin_cg(tab, B->partial.p, ulog_tile(B->tile), B->fold.p, fresh ? 1 : 0, s);
+    xn = xn == CG_XM ? 1 : xn + 1;
+    if (fresh) {  // x completed, then r = b - A x replaces the pending update
+        XDirs<double> d{};
+        for (int i = 0; i < xn; ++i) d.d[i] = slot(k + 2 - (uint64_t)xn + (uint64_t)i);
+        launch_bx_terms<double>(tab, L.x.p, d, L.dd.p, N, 2, s);
+        resid_pass(B, L, tab, io.rout);
+        launch_bfin(tab, B->partial.p, ulog_tile(B->tile), FIN_FRESH, 0, s);
+        xn = 0;
+    }
+}
+
+// One CG iteration (index k) of every running instance, two-pass form.
 static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
     const hipStream_t s = B->stream;
     const BatchTab tab = tab_running(B, L);
@@ -359,13 +408,20 @@ static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
 // kind 0: `chunk` CG iterations starting at an even index; kind 1: GS sweeps
 static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int refresh) {
     for (auto& g : B->graphs)
-        if (g.L == &L && g.kind == kind && g.chunk == n && g.refresh == refresh && g.tile == B->tile && g.gsb == B->gs_block)
+        if (g.L == &L && g.kind == kind && g.chunk == n && g.refresh == refresh && g.tile == B->tile && g.gsb == B->gs_block &&
+            g.fused == B->fused)
             return g.exec;
     hipGraph_t graph;
     PRX_HIP(hipStreamSynchronize(B->stream));
     PRX_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
     try {
-        if (kind == 0)
+        if (kind == 0 && B->fused) {
+            // a replay starts with CG_XM terms pending (none at the very first
+            // pass, which the kernels check), or none after a refresh that
+            // ended the previous replay
+            int xn = refresh > 0 ? 0 : CG_XM;
+            for (int k = 0; k < n; ++k) cg_enqueue_fused(B, L, (uint64_t)k, refresh, xn);
+        } else if (kind == 0)
             for (int k = 0; k < n; ++k) cg_enqueue(B, L, (uint64_t)k, refresh);
         else
             gs_enqueue(B, L, n);
@@ -380,7 +436,7 @@ static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int re
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PRX_HIP(e);
-    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, exec});
+    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, B->fused, exec});
     return exec;
 }
 
@@ -424,6 +480,10 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
     int chunk = B->chunk;
     if (refresh > 0) chunk = std::max(refresh, (chunk / refresh) * refresh);
     if (chunk & 1) chunk *= 2;
+    if (B->fused) {  // replays keep pass index = ring position: a multiple of CG_XM (and of the refresh)
+        const int unit = refresh > 0 ? std::lcm(refresh, CG_XM) : CG_XM;
+        chunk = std::max(unit, (chunk + unit / 2) / unit * unit);
+    }
     const hipGraphExec_t g = graph_for(B, L, 0, chunk, refresh);
     std::vector<char> fin(K, 0);
     std::vector<double> last_rt(K, INFINITY);
@@ -439,8 +499,12 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
             read_states(B);
         }
         // complete x: add the pending lazy terms
-        launch_bflush_x<double>(all, L.x.p, L.dd.p, L.dd.p + L.N, L.N, s);
-        for (int k = 0; k < K; ++k) B->st_host[k].xpend = 0;
+        if (B->fused) launch_bx_terms<double>(all, L.x.p, XDirs<double>{}, L.dd.p, L.N, 1, s);
+        else launch_bflush_x<double>(all, L.x.p, L.dd.p, L.dd.p + L.N, L.N, s);
+        for (int k = 0; k < K; ++k) {
+            B->st_host[k].xpend = 0;
+            B->st_host[k].xn = 0;
+        }
         write_states(B);
         if (mode != PRX_RESID_TRUE) break;
         // confirm on the true residual; continue the unconfirmed instances
@@ -475,10 +539,20 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
             h.delta = rt;
             h.done = 0;
             h.status = 0;
-            // the graph restarts at an even iteration index and reads the
-            // direction from dd[0]; move it there if it is in dd[1]
-            if (h.dir_buf == 1) {
-                const int64_t nk = (int64_t)(L.off_h[k + 1] - L.off_h[k]);
+            const int64_t nk = (int64_t)(L.off_h[k + 1] - L.off_h[k]);
+            if (B->fused) {
+                // the graph restarts at pass 0, which reads r from slot 0 (the
+                // true residual just written there, no update pending) and the
+                // direction from ring slot 0: move d_{it-1} there
+                h.rpend = 0;
+                const int sl = (int)(((h.it - (int64_t)h.ring0) % CG_XM + CG_XM) % CG_XM);
+                if (sl != 0)
+                    PRX_HIP(hipMemcpyAsync(L.dd.p + L.off_h[k], L.dd.p + (int64_t)sl * L.N + L.off_h[k], sizeof(double) * nk,
+                                           hipMemcpyDeviceToDevice, s));
+                h.ring0 = (int32_t)(h.it % CG_XM);
+            } else if (h.dir_buf == 1) {
+                // the graph restarts at an even iteration index and reads the
+                // direction from dd[0]; move it there if it is in dd[1]
                 PRX_HIP(hipMemcpyAsync(L.dd.p + L.off_h[k], L.dd.p + L.N + L.off_h[k], sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
             }
         }
@@ -662,6 +736,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     }
     if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched cascadic solve runs in FP64");
     build(B);
+    B->fused = o.cg_two_pass == 0;
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;
     BLevel& C = B->coarse;

@@ -345,16 +345,6 @@ __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) 
 __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
     __shared__ double ws[16];
     if (op != FIN_SLOT && op != FIN_INIT_REF && op != FIN_INIT_TRUE && op != FIN_PINIT && st->done) return;
-    if (op == FIN_CG) {  // four interleaved partials per block
-        double v[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            v[j] = fin_sum<4>(partial + j, np, ws);
-            __syncthreads();  // ws is reused by the next sum
-        }
-        if (threadIdx.x == 0) fin_cg(st, v[0], v[1], v[2], v[3], slot);
-        return;
-    }
     const double sum = fin_sum(partial, np, ws);
     if (threadIdx.x != 0) return;
     fin_step(st, sum, op, slot);
@@ -364,12 +354,15 @@ __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial
 // order, then the 64-lane butterfly, then the waves in order) into
 // out[4b .. 4b+3], so the single-block fold reads 4 values per block
 // instead of 4 per brick.
-__global__ __launch_bounds__(256) void k_fold4(const CgState* st, const double* __restrict__ partial, int np, int chunk,
-                                               double* __restrict__ out) {
-    __shared__ double ws[16];
-    if (st->done) return;
+constexpr int FOLD_SPLIT = 256;  // more bricks than this: two stages (at most FOLD_BLOCKS first-stage blocks)
+__device__ __host__ inline int fold_blocks(int np) {
+    const int g = (np + 1023) / 1024;
+    return np > FOLD_SPLIT ? (g < FOLD_BLOCKS ? g : FOLD_BLOCKS) : 0;
+}
+__device__ inline void fold4_block(const double* __restrict__ partial, int np, int b, double* __restrict__ out, double* ws) {
     typedef double nd2 __attribute__((ext_vector_type(2)));
-    const int b0 = blockIdx.x * chunk, b1 = min(np, b0 + chunk);
+    const int g = fold_blocks(np), chunk = (np + g - 1) / g;
+    const int b0 = b * chunk, b1 = min(np, b0 + chunk);
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = b0 + (int)threadIdx.x; i < b1; i += 256) {
         const nd2 u = *(const nd2*)(partial + 4 * (int64_t)i);
@@ -389,18 +382,40 @@ __global__ __launch_bounds__(256) void k_fold4(const CgState* st, const double* 
     if (threadIdx.x < 4) {
         double sum = 0.0;
         for (int w = 0; w < 4; ++w) sum += ws[threadIdx.x * 4 + w];
-        out[4 * blockIdx.x + threadIdx.x] = sum;
+        out[4 * b + threadIdx.x] = sum;
+    }
+}
+__global__ __launch_bounds__(256) void k_fold4(const CgState* st, const double* __restrict__ partial, int np, double* __restrict__ out) {
+    __shared__ double ws[16];
+    if (st->done) return;
+    fold4_block(partial, np, blockIdx.x, out, ws);
+}
+// the four sums of FIN_CG, second stage (or the only one for small grids)
+__device__ inline void fold4_final(const double* partial, int np, const double* first, double* ws, double v[4]) {
+    const int g = fold_blocks(np);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = g ? fin_sum<4>(first + j, g, ws) : fin_sum<4>(partial + j, np, ws);
+        __syncthreads();  // ws is reused by the next sum
     }
 }
 
+// FIN_CG: the first stage writes behind the partials (partial + 4 np)
+__global__ __launch_bounds__(1024) void k_fin_cg(CgState* st, const double* partial, int np, int slot) {
+    __shared__ double ws[16];
+    if (st->done) return;
+    double v[4];
+    fold4_final(partial, np, partial + 4 * (int64_t)np, ws, v);
+    if (threadIdx.x == 0) fin_cg(st, v[0], v[1], v[2], v[3], slot);
+}
 void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hipStream_t s) {
-    if (op == FIN_CG && np > 256) {  // two stages: up to 128 blocks, then one
-        const int g = std::min(128, (np + 1023) / 1024);
-        const int chunk = (np + g - 1) / g;
-        double* out = const_cast<double*>(partial) + 4 * (int64_t)np;
-        hipLaunchKernelGGL(k_fold4, dim3(g), dim3(256), 0, s, st, partial, np, chunk, out);
-        PRX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_fin, dim3(1), dim3(1024), 0, s, st, out, g, op, slot);
+    if (op == FIN_CG) {
+        const int g = fold_blocks(np);
+        if (g) {
+            hipLaunchKernelGGL(k_fold4, dim3(g), dim3(256), 0, s, st, partial, np, const_cast<double*>(partial) + 4 * (int64_t)np);
+            PRX_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_fin_cg, dim3(1), dim3(1024), 0, s, st, partial, np, slot);
         PRX_LAUNCH_CHECK();
         return;
     }
@@ -424,6 +439,90 @@ void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, i
     hipLaunchKernelGGL(k_bfin, dim3(tab.K), dim3(1024), 0, s, tab, partial, ulog, op, slot);
     PRX_LAUNCH_CHECK();
 }
+
+// FIN_CG per instance, in the single-tree order (fold4_block, fold4_final)
+// over the instance's bricks [off[k] >> ulog, off[k+1] >> ulog)
+__global__ __launch_bounds__(256) void k_bfold4(BatchTab tab, const double* __restrict__ partial, int ulog, double* __restrict__ fold) {
+    __shared__ double ws[16];
+    const uint32_t k = blockIdx.y;
+    if (!b_runs(tab, k)) return;
+    const int64_t p0 = (int64_t)(tab.off[k] >> ulog), np = (int64_t)(tab.off[k + 1] >> ulog) - p0;
+    if ((int)blockIdx.x >= fold_blocks((int)np)) return;
+    fold4_block(partial + 4 * p0, (int)np, blockIdx.x, fold + (int64_t)k * FOLD_BLOCKS * 4, ws);
+}
+__global__ __launch_bounds__(1024) void k_bfin_cg(BatchTab tab, const double* partial, int ulog, const double* fold, int slot) {
+    __shared__ double ws[16];
+    const uint32_t k = blockIdx.x;
+    if (!b_runs(tab, k)) return;
+    const int64_t p0 = (int64_t)(tab.off[k] >> ulog), np = (int64_t)(tab.off[k + 1] >> ulog) - p0;
+    double v[4];
+    fold4_final(partial + 4 * p0, (int)np, fold + (int64_t)k * FOLD_BLOCKS * 4, ws, v);
+    if (threadIdx.x == 0) fin_cg(tab.st + k, v[0], v[1], v[2], v[3], slot);
+}
+void launch_bfin_cg(const BatchTab& tab, const double* partial, int ulog, double* fold, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfold4, dim3(FOLD_BLOCKS, tab.K), dim3(256), 0, s, tab, partial, ulog, fold);
+    PRX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bfin_cg, dim3(tab.K), dim3(1024), 0, s, tab, partial, ulog, fold, slot);
+    PRX_LAUNCH_CHECK();
+}
+
+// Fused CG x terms per instance (x + xa[0] d[0] + ... in FP64, rounded once,
+// as k_x_terms).  A block covers 512 vectors inside one 4096-node brick, so
+// it belongs to one instance.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_bx_terms(BatchTab tab, T* __restrict__ x, XDirs<T> dd, const T* __restrict__ ring,
+                                                  int64_t Ntot) {
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int64_t v0 = (int64_t)blockIdx.x * 512;
+    const uint32_t k = tab.brick_inst[(uint64_t)(v0 * VEC) >> 12];
+    if (!b_runs(tab, k)) return;
+    const CgState* st = tab.st + k;
+    const int xn = st->xn;
+    if (MODE == 0 ? xn != CG_XM : xn == 0) return;
+    const T* dp[CG_XM];
+#pragma unroll
+    for (int j = 0; j < CG_XM; ++j) {
+        if (MODE == 1) {
+            const int64_t sl = ((st->it - (int64_t)st->ring0 - xn + j + 1) % CG_XM + CG_XM) % CG_XM;
+            dp[j] = ring + sl * Ntot;
+        } else {
+            dp[j] = dd.d[j];
+        }
+    }
+    double a[CG_XM];
+#pragma unroll
+    for (int j = 0; j < CG_XM; ++j) a[j] = st->xa[j];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int64_t g = (v0 + u * 256 + threadIdx.x) * VEC;
+        const NV xv = ld_nt(x + g);
+        NV dv[CG_XM];
+#pragma unroll
+        for (int j = 0; j < CG_XM; ++j)
+            if (j < xn && dp[j]) dv[j] = ld_nt(dp[j] + g);
+        NV xo;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            double v = (double)xv[e];
+#pragma unroll
+            for (int j = 0; j < CG_XM; ++j)
+                if (j < xn && dp[j]) v += a[j] * (double)dv[j][e];
+            xo[e] = (T)v;
+        }
+        st_nt(x + g, xo);
+    }
+}
+template <typename T>
+void launch_bx_terms(const BatchTab& tab, T* x, const XDirs<T>& d, const T* ring, int64_t Ntot, int mode, hipStream_t s) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const unsigned nb = (unsigned)(Ntot / (512 * VEC));  // instances are multiples of 4096 nodes
+    if (mode == 0) hipLaunchKernelGGL((k_bx_terms<T, 0>), dim3(nb), dim3(256), 0, s, tab, x, d, ring, Ntot);
+    else if (mode == 1) hipLaunchKernelGGL((k_bx_terms<T, 1>), dim3(nb), dim3(256), 0, s, tab, x, d, ring, Ntot);
+    else hipLaunchKernelGGL((k_bx_terms<T, 2>), dim3(nb), dim3(256), 0, s, tab, x, d, ring, Ntot);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bx_terms<double>(const BatchTab&, double*, const XDirs<double>&, const double*, int64_t, int, hipStream_t);
 
 // r -= alpha_k q ; partial = r.r per block ; x as in k_cg_update (eager, or
 // lazy in pairs).  A block covers 1024 vectors (4096 float or 2048 double
@@ -706,14 +805,14 @@ __global__ __launch_bounds__(256) void k_x_terms(T* __restrict__ x, XDirs<T> dd,
             NV dv[CG_XM];
 #pragma unroll
             for (int j = 0; j < CG_XM; ++j)
-                if (j < xn) dv[j] = ld_nt(dd.d[j] + g);
+                if (j < xn && dd.d[j]) dv[j] = ld_nt(dd.d[j] + g);
             NV xo;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
                 double v = (double)xv[e];
 #pragma unroll
                 for (int j = 0; j < CG_XM; ++j)
-                    if (j < xn) v += a[j] * (double)dv[j][e];
+                    if (j < xn && dd.d[j]) v += a[j] * (double)dv[j][e];
                 xo[e] = (T)v;
             }
             st_nt(x + g, xo);

@@ -495,14 +495,17 @@ template void launch_screen_gather<float>(const LevelView&, const float*, const 
 template <typename T, int MODE>
 __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __restrict__ cell, const T* __restrict__ tt,
                                  const uint32_t* __restrict__ pinst, const T* __restrict__ x, const double* __restrict__ x64,
-                                 const float* __restrict__ e32, T* s, const T* __restrict__ r, const T* __restrict__ dd) {
+                                 const float* __restrict__ e32, T* s, const T* __restrict__ r, const T* __restrict__ dd,
+                                 const T* __restrict__ qq) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const uint32_t k = pinst[i];
     if (tab.run[(int64_t)k * tab.run_stride] != tab.run_when) return;
     const BCoef cf = tab.coef[k];
     const uint64_t ib = tab.off[k];
-    const double bt = MODE == 2 ? tab.st[k].beta : 0.0;
+    const double bt = MODE >= 2 ? tab.st[k].beta : 0.0;
+    const double al = MODE == 3 ? tab.st[k].alpha : 0.0;
+    const int rp = MODE == 3 ? tab.st[k].rpend : 0;
     const uint32_t res = cf.res;
     const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
     const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
@@ -517,7 +520,8 @@ __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __rest
         T v;
         if (MODE == 0) v = x[m];
         else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else v = dir_form(bt, dd[m], r[m]);
+        else if (MODE == 2) v = dir_form(bt, dd[m], r[m]);
+        else v = cg_form(bt, al, rp, dd[m], r[m], rp ? qq[m] : T(0));
         acc += bx * by * bz * v;
     }
     const T wh3 = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;
@@ -525,26 +529,29 @@ __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __rest
 }
 template <typename T>
 void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
-                           const T* r, const T* d, hipStream_t s) {
+                           const T* r, const T* d, hipStream_t s, const T* q) {
     if (pts.P <= 0) return;
     const T* tt;
     if constexpr (sizeof(T) == 8) tt = (const T*)pts.t64;
     else tt = (const T*)pts.t32;
-    if (r)
+    if (r && q)
+        hipLaunchKernelGGL((k_bscreen_gather<T, 3>), dim3(nblk(pts.P)), dim3(256), 0, s, tab, pts.P, pts.cell, tt, pts.inst, x, x64,
+                           e32, sbuf, r, d, q);
+    else if (r)
         hipLaunchKernelGGL((k_bscreen_gather<T, 2>), dim3(nblk(pts.P)), dim3(256), 0, s, tab, pts.P, pts.cell, tt, pts.inst, x, x64,
-                           e32, sbuf, r, d);
+                           e32, sbuf, r, d, q);
     else if (x64)
         hipLaunchKernelGGL((k_bscreen_gather<T, 1>), dim3(nblk(pts.P)), dim3(256), 0, s, tab, pts.P, pts.cell, tt, pts.inst, x, x64,
-                           e32, sbuf, r, d);
+                           e32, sbuf, r, d, q);
     else
         hipLaunchKernelGGL((k_bscreen_gather<T, 0>), dim3(nblk(pts.P)), dim3(256), 0, s, tab, pts.P, pts.cell, tt, pts.inst, x, x64,
-                           e32, sbuf, r, d);
+                           e32, sbuf, r, d, q);
     PRX_LAUNCH_CHECK();
 }
 template void launch_bscreen_gather<double>(const BatchTab&, const BPoints&, const double*, const double*, const float*, double*,
-                                            const double*, const double*, hipStream_t);
+                                            const double*, const double*, hipStream_t, const double*);
 template void launch_bscreen_gather<float>(const BatchTab&, const BPoints&, const float*, const double*, const float*, float*,
-                                           const float*, const float*, hipStream_t);
+                                           const float*, const float*, hipStream_t, const float*);
 
 // ------------------------------------------------------------------ verification CSR
 // The CSR of assemble_system (src/fem_tree.rs:180-249) as the reference

@@ -261,6 +261,11 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         ib = tab.off[inst];
         lb = bid - (uint32_t)(ib / B3);
         if constexpr (OP == OP_DIR_APPLY) bt = tab.st[inst].beta;
+        if constexpr (OP == OP_CG) {
+            bt = tab.st[inst].beta;
+            al = tab.st[inst].alpha;
+            rp = tab.st[inst].rpend;
+        }
     } else {
         if constexpr (OP == OP_DIR_APPLY) bt = *io.beta;
         if constexpr (OP == OP_CG) {
@@ -947,12 +952,14 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
         else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(16, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(16, OP_RESID);
         else if (op == OP_GS) PRX_BTILE_LAUNCH(16, OP_GS);
+        else if (op == OP_CG) PRX_BTILE_LAUNCH(16, OP_CG);
         else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     } else {
         if (op == OP_APPLY) PRX_BTILE_LAUNCH(8, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(8, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(8, OP_RESID);
         else if (op == OP_GS) PRX_BTILE_LAUNCH(8, OP_GS);
+        else if (op == OP_CG) PRX_BTILE_LAUNCH(8, OP_CG);
         else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     }
 #undef PRX_BTILE_LAUNCH

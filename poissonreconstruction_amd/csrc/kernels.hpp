@@ -17,7 +17,9 @@ namespace prx {
 // the last iteration as still to be formed (by the next pass, from the r and
 // q it wrote); `xn` counts the terms alpha_j d_j not yet added to x, with
 // coefficients xa[0..xn) (oldest first).  x takes CG_XM of them at once, just
-// before the pass that overwrites the oldest direction of the ring.
+// before the pass that overwrites the oldest direction of the ring.  ring0:
+// the iteration count at which the solve last (re)started on ring slot 0, so
+// direction j sits in slot (j - ring0 + 1) mod CG_XM.
 constexpr int CG_XM = 8;
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
@@ -28,7 +30,7 @@ struct CgState {
     double alpha_pend;
     int32_t xpend, dir_buf;
     double xa[CG_XM];
-    int32_t rpend, xn;
+    int32_t rpend, xn, ring0;
 };
 
 // FIN_BETA_FULL: as FIN_BETA after an iteration that left x fully updated.
@@ -230,12 +232,25 @@ struct BPoints {
 template <typename T>
 void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int op, const StencilIO<T>& io, int tile,
                      hipStream_t s);
+// With q (fused CG step): the operand as the OP_CG pass forms it.
 template <typename T>
 void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
-                           const T* r, const T* d, hipStream_t s);
+                           const T* r, const T* d, hipStream_t s, const T* q = nullptr);
 // Per instance: fold its partials [off[k] >> ulog, off[k+1] >> ulog) in the
 // order k_fin uses and apply the same scalar step to st[k].
 void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, int slot, hipStream_t s);
+// FIN_CG per instance, folded in the single-tree order: instances with more
+// than 256 bricks first fold their four sums per brick in up to FOLD_BLOCKS
+// blocks into fold[(k * FOLD_BLOCKS + b) * 4 + j], then one block per
+// instance folds those.
+constexpr int FOLD_BLOCKS = 128;
+void launch_bfin_cg(const BatchTab& tab, const double* partial, int ulog, double* fold, int slot, hipStream_t s);
+// fused CG x terms per instance.  mode 0: all CG_XM terms before a pass
+// (directions d, oldest first); mode 2: whatever is pending, while running
+// (directions d); mode 1: whatever is pending, each instance finding its
+// directions in the ring (slot i at ring + i * Ntot) from its it, ring0, xn.
+template <typename T>
+void launch_bx_terms(const BatchTab& tab, T* x, const XDirs<T>& d, const T* ring, int64_t Ntot, int mode, hipStream_t s);
 // x += alpha_k d ; r -= alpha_k q ; partial per (256 * 16 / sizeof(T)) nodes
 template <typename T>
 int launch_bcg_update(const BatchTab& tab, T* x, T* r, const T* dprev, const T* d, const T* q, int64_t Ntot, double* partial,

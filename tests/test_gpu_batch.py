@@ -24,26 +24,28 @@ def clouds():
     return [(depth_for(n), unit(synthetic.superquadric(n, seed=sd)), pw) for n, pw, sd in CLOUDS]
 
 
-def single(D, pts, pw, eps, mode, refresh):
-    # the batched cascadic solve runs the two-pass CG iteration; the single-tree solve with the same form is the
-    # reference for round-off identity (the fused form is the same iteration with other rounding)
+def single(D, pts, pw, eps, mode, refresh, two_pass=False):
+    # the single-tree solve with the same CG form is the reference for round-off identity (the fused and two-pass
+    # forms are the same iteration with other rounding)
     t = pr.FEMTree(D)
     t.splat_normal_field(pts)
     rep = t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=pr.PRX_PREC_FP64, residual_mode=mode,
-                           refresh_period=refresh, cg_two_pass=True)
+                           refresh_period=refresh, cg_two_pass=two_pass)
     return t.solution, rep, t
 
 
+@pytest.mark.parametrize("two_pass", [False, True])
 @pytest.mark.parametrize("mode,eps,refresh", [(pr.PRX_RESID_TRUE, 1e-10, 0), (pr.PRX_RESID_REFERENCE, 1e-6, 10)])
-def test_batch_matches_single_tree_solves(gpu, mode, eps, refresh):
+def test_batch_matches_single_tree_solves(gpu, mode, eps, refresh, two_pass):
     cs = clouds()
     B = Batch()
     for D, pts, pw in cs:
         B.add(D, pts, pw)
-    rep, per = B.solve(gs=20, cg_iters=800000, eps=eps, residual_mode=mode, refresh_period=refresh, autotune=False)
+    rep, per = B.solve(gs=20, cg_iters=800000, eps=eps, residual_mode=mode, refresh_period=refresh, autotune=False,
+                       cg_two_pass=two_pass)
     assert rep["instances"] == len(cs) and rep["converged"] == len(cs)
     for k, (D, pts, pw) in enumerate(cs):
-        xs, rs, _ = single(D, pts, pw, eps, mode, refresh)
+        xs, rs, _ = single(D, pts, pw, eps, mode, refresh, two_pass)
         xb = B.tree(k).solution
         scale = np.abs(xs).max()
         assert np.abs(xb - xs).max() <= 1e-9 * scale, k
