@@ -175,10 +175,20 @@ static void setup_level(prx_batch* B, BLevel& L, const std::vector<uint32_t>& de
     PRX_HIP(hipMemcpyAsync(L.brick_inst.p, bi.data(), sizeof(uint32_t) * bi.size(), hipMemcpyHostToDevice, s));
     L.x.ensure(L.N);
     L.b.ensure(L.N);
+    PRX_HIP(hipStreamSynchronize(s));  // bi goes out of scope
+}
+
+// The cascadic CG's vectors of a level (allocated by the batched cascadic
+// solve, released by the multigrid one, which solves through the trees)
+static void ensure_cg(BLevel& L) {
     L.r.ensure(2 * L.N);       // fused CG: r and q in ping-pong pairs
     L.dd.ensure(CG_XM * L.N);  // directions: a ring of CG_XM (the two-pass form uses two)
     L.q.ensure(2 * L.N);
-    PRX_HIP(hipStreamSynchronize(s));  // bi goes out of scope
+}
+static void release_cg(BLevel& L) {
+    L.r.release();
+    L.dd.release();
+    L.q.release();
 }
 
 // Concatenate the per-instance screening lists and point data (global node
@@ -660,6 +670,11 @@ static void solve_batched_mg(prx_batch* B, uint64_t cg_iters, double eps, const 
     const int S = std::min(K, 4);
     for (prx_tree* t : B->inst)
         if (!t->assembled) throw Error(PRX_ERR_STATE, "batch instance not assembled");
+    // the trees carry these solves: the batched levels' CG vectors (and the
+    // graphs that hold their addresses) are not needed meanwhile
+    B->drop_graphs();
+    release_cg(B->fine);
+    release_cg(B->coarse);
     std::vector<int> order(K);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->inst[a]->N > B->inst[b]->N; });
@@ -736,6 +751,9 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     }
     if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched cascadic solve runs in FP64");
     build(B);
+    if (!B->fine.r.p || !B->coarse.r.p) B->drop_graphs();  // captured graphs hold the old vector addresses
+    ensure_cg(B->fine);
+    ensure_cg(B->coarse);
     B->fused = o.cg_two_pass == 0;
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;
