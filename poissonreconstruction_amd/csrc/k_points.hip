@@ -444,21 +444,30 @@ __global__ void k_screen_gather(uint32_t D, int64_t P, const uint32_t* __restric
     const uint32_t res = 1u << D;
     const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
     const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
-    T acc = 0;
+    // all corners' loads first (a corner outside the grid reads node 0 with
+    // weight 0, which adds an exact zero), then the sum in corner order
+    uint32_t m[8];
+    T w[8], v[8];
+#pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t ox = c[0] + (k & 1), oy = c[1] + ((k >> 1) & 1), oz = c[2] + (k >> 2);
-        if (ox >= res || oy >= res || oz >= res) continue;
+        const bool in = ox < res && oy < res && oz < res;
         const T bx = (k & 1) ? t[0] : T(1) - t[0];
         const T by = ((k >> 1) & 1) ? t[1] : T(1) - t[1];
         const T bz = (k >> 2) ? t[2] : T(1) - t[2];
-        const uint32_t m = morton3(ox, oy, oz);
-        T v;
-        if (MODE == 0) v = x[m];
-        else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else if (MODE == 2) v = dir_form(bt, dd[m], r[m]);
-        else v = cg_form(bt, al, rp, dd[m], r[m], rp ? qq[m] : T(0));
-        acc += bx * by * bz * v;
+        w[k] = in ? bx * by * bz : T(0);
+        m[k] = in ? morton3(ox, oy, oz) : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (MODE == 0) v[k] = x[m[k]];
+        else if (MODE == 1) v[k] = (T)(x64[m[k]] + (double)e32[m[k]]);
+        else if (MODE == 2) v[k] = dir_form(bt, dd[m[k]], r[m[k]]);
+        else v[k] = cg_form(bt, al, rp, dd[m[k]], r[m[k]], qq[m[k]]);  // q is read always and used when rp
+    }
+    T acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += w[k] * v[k];
     s[i] = wh3 * acc;
 }
 template <typename T>
@@ -509,21 +518,30 @@ __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __rest
     const uint32_t res = cf.res;
     const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
     const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
-    T acc = 0;
+    // as k_screen_gather: all loads first (out-of-grid corners read the
+    // instance's node 0 with weight 0), then the sum in corner order
+    uint64_t m[8];
+    T w[8], v[8];
+#pragma unroll
     for (int q = 0; q < 8; ++q) {
         const uint32_t ox = c[0] + (q & 1), oy = c[1] + ((q >> 1) & 1), oz = c[2] + (q >> 2);
-        if (ox >= res || oy >= res || oz >= res) continue;
+        const bool in = ox < res && oy < res && oz < res;
         const T bx = (q & 1) ? t[0] : T(1) - t[0];
         const T by = ((q >> 1) & 1) ? t[1] : T(1) - t[1];
         const T bz = (q >> 2) ? t[2] : T(1) - t[2];
-        const uint64_t m = ib + morton3(ox, oy, oz);
-        T v;
-        if (MODE == 0) v = x[m];
-        else if (MODE == 1) v = (T)(x64[m] + (double)e32[m]);
-        else if (MODE == 2) v = dir_form(bt, dd[m], r[m]);
-        else v = cg_form(bt, al, rp, dd[m], r[m], rp ? qq[m] : T(0));
-        acc += bx * by * bz * v;
+        w[q] = in ? bx * by * bz : T(0);
+        m[q] = ib + (in ? morton3(ox, oy, oz) : 0u);
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (MODE == 0) v[q] = x[m[q]];
+        else if (MODE == 1) v[q] = (T)(x64[m[q]] + (double)e32[m[q]]);
+        else if (MODE == 2) v[q] = dir_form(bt, dd[m[q]], r[m[q]]);
+        else v[q] = cg_form(bt, al, rp, dd[m[q]], r[m[q]], qq[m[q]]);  // q is read always and used when rp
+    }
+    T acc = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += w[q] * v[q];
     const T wh3 = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;
     s[i] = wh3 * acc;
 }
