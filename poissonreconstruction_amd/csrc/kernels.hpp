@@ -18,8 +18,8 @@ namespace prx {
 // q it wrote); `xn` counts the terms alpha_j d_j not yet added to x, with
 // coefficients xa[0..xn) (oldest first).  x takes CG_XM of them at once, just
 // before the pass that overwrites the oldest direction of the ring.  ring0:
-// the iteration count at which the solve last (re)started on ring slot 0, so
-// direction j sits in slot (j - ring0 + 1) mod CG_XM.
+// the iteration count (mod CG_XM) at which the solve last (re)started on ring
+// slot 0, so direction j sits in slot (j - ring0 + 1) mod CG_XM.
 constexpr int CG_XM = 8;
 struct CgState {
     double delta, delta0, dq, alpha, beta, thr, eps, delta_prev;
@@ -95,6 +95,7 @@ template <typename T> void launch_cg_update(T* x, T* r, const T* dprev, const T*
 template <typename T> void launch_axpy_alpha(T* x, const T* dprev, const T* d, const CgState* st, int64_t n, hipStream_t s);
 // x += alpha_pend * d{dir_buf} when a lazy term is pending (the caller then clears xpend)
 template <typename T> void launch_flush_x(T* x, const T* d0, const T* d1, const CgState* st, int64_t n, hipStream_t s);
+// xpend = xn = 0, after the pending x terms have been added
 void launch_clear_xpend(CgState* st, hipStream_t s);
 // fused CG: x += sum_j xa[j] * d[j] over the pending terms (d[j] the
 // direction of term j, oldest first).  launch_x_terms_full adds them only
