@@ -245,6 +245,26 @@ def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh):
     assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
 
 
+@pytest.mark.parametrize("cap,refresh", [(37, 0), (37, 10), (45, 10), (8, 0)])
+def test_fused_cg_stops_at_the_cap_with_x_terms_pending(gpu, cap, refresh):
+    # Stopped by the iteration cap (the reference's cg_iters / 4 on the fine level, src/fem_tree.rs:458-487) after
+    # `cap` fused steps, up to seven x terms are still pending and must be added exactly once: the solution then equals
+    # the two-pass form's after the same number of iterations, to round-off.
+    D, P = 5, 8000
+    pts = unit_points("sphere", P, seed=8)
+    sols, reps = [], []
+    for two_pass in (True, False):
+        t = make_tree(D, pts, 4.0)
+        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 4 * cap, 1e-14, precision=pr.PRX_PREC_FP64,
+                               residual_mode=pr.PRX_RESID_REFERENCE, refresh_period=refresh, cg_two_pass=two_pass)
+        assert rep["fine_iters"] == cap
+        sols.append(t.solution)
+        reps.append(rep)
+    assert reps[0]["coarse_iters"] == reps[1]["coarse_iters"]
+    scale = np.abs(sols[0]).max()
+    assert np.abs(sols[0] - sols[1]).max() <= 1e-9 * scale
+
+
 @pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
 def test_fused_cg_step_on_8cubed_tiles(gpu, prec, monkeypatch):
     # The tile edge is chosen per tree (PRX_TILE=8 at creation): the 8^3-brick form of the fused CG step, the GS pass
