@@ -216,11 +216,13 @@ def test_cascadic_c1_field_matches_oracle(gpu, prec):
     assert np.abs(x - xo).max() <= 1e-5 * rng_f
 
 
-@pytest.mark.parametrize("prec,mode,refresh", [(pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 10),
-                                               (pr.PRX_PREC_FP64, pr.PRX_RESID_REFERENCE, 10),
-                                               (pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 0),
-                                               (pr.PRX_PREC_MIXED, pr.PRX_RESID_TRUE, 0)])
-def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh):
+@pytest.mark.parametrize("prec,mode,refresh,pw", [(pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 10, 4.0),
+                                                  (pr.PRX_PREC_FP64, pr.PRX_RESID_REFERENCE, 10, 4.0),
+                                                  (pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 0, 4.0),
+                                                  (pr.PRX_PREC_MIXED, pr.PRX_RESID_TRUE, 0, 4.0),
+                                                  (pr.PRX_PREC_FP64, pr.PRX_RESID_TRUE, 0, 0.0),
+                                                  (pr.PRX_PREC_MIXED, pr.PRX_RESID_TRUE, 0, 0.0)])
+def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh, pw):
     # The fused CG iteration (the update r -= alpha q joined to the next operator pass, |r|^2 expanded from d.q, r.q
     # and q.q, r and q in ping-pong pairs, x in pairs) is the two-pass iteration of src/solvers.rs:36-64 with other
     # rounding: iteration counts within 2 % (10 % for the mixed solve, whose FP64 refinement points follow the FP32
@@ -230,8 +232,8 @@ def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh):
     pts = unit_points("torus", P, seed=3)
     sols, reps = [], []
     for two_pass in (True, False):
-        t = make_tree(D, pts, 4.0)
-        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 40000, 1e-8, precision=prec, residual_mode=mode,
+        t = make_tree(D, pts, pw)
+        rep = t.solve_cascadic(pts, pw, D - 2, 20, 40000, 1e-8, precision=prec, residual_mode=mode,
                                refresh_period=refresh, cg_two_pass=two_pass)
         assert rep["status"] == 0
         sols.append(t.solution)
