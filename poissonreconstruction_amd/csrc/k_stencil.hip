@@ -24,6 +24,13 @@
 // is coalesced too.  Screening: points were pre-reduced to s_p =
 // wh3*sum(phi*x) (k_screen_gather); the brick's touched nodes add
 // sum(phi*s_p) in a fixed order, so the result is deterministic.
+//
+// The operand is formed per node as the pass needs it: x (apply, residual,
+// smoothing), x64 + e32 (the mixed solve's FP64 refinement), r + beta d_old
+// (the two-pass CG direction) or, for the fused CG step, (r - alpha q) +
+// beta d_old, with the same expression for interior, halo and gathered
+// values so that every brick sees one value per node.  Workgroups go to the
+// XCDs in chunks of consecutive bricks (xcd_chunk).
 #include "kernels.hpp"
 
 namespace prx {
