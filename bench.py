@@ -469,6 +469,17 @@ def operator_roofline(L, tree, N, P, reps):
     return roofline
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(raw, D, pw, rep_fp64):
     """The oracle's CSR path (reference algorithm, FP64, OpenMP where the
     reference uses rayon) on a bounded sample of the same cloud: assembly at
@@ -494,6 +505,15 @@ def cpu_baseline(raw, D, pw, rep_fp64):
             t = time.perf_counter()
             A.cg(b, x, its, 1e-30)
             t_it = (time.perf_counter() - t) / its
+            # the same iterations on one thread (the reference with RAYON_NUM_THREADS=1): SpMV and dots are
+            # the only parallel loops of the path, so only the CG term changes
+            oracle.set_threads(1)
+            its1 = 10
+            x1 = np.zeros(8 ** d)
+            t = time.perf_counter()
+            A.cg(b, x1, its1, 1e-30)
+            t_it1 = (time.perf_counter() - t) / its1
+            oracle.set_threads(threads)
             gs_t = time.perf_counter()
             A.gs_sweep(b, x)
             t_gs = time.perf_counter() - gs_t
@@ -505,16 +525,23 @@ def cpu_baseline(raw, D, pw, rep_fp64):
         asm_D = t_asm[7] + a * (nD - n7)
     else:
         asm_D = t_asm[7] * nD / n7
-    cg_D = t_it * nD / n7 * rep_fp64["fine_iters"]
     gs_D = t_gs * nD / n7 * 20
-    coarse_D = (t_it * (nD / 64) / n7 * rep_fp64["coarse_iters"] + t_gs * (nD / 64) / n7 * 20)
-    total_ms = (asm_D + cg_D + gs_D + coarse_D) * 1e3
+
+    def solve_ms(ti):
+        cg_D = ti * nD / n7 * rep_fp64["fine_iters"]
+        coarse_D = (ti * (nD / 64) / n7 * rep_fp64["coarse_iters"] + t_gs * (nD / 64) / n7 * 20)
+        return (asm_D + cg_D + gs_D + coarse_D) * 1e3
+
+    total_ms, total1_ms = solve_ms(t_it), solve_ms(t_it1)
     log(f"cpu baseline sample took {time.time() - t_all:.1f}s: asm6={t_asm[6]:.2f}s asm7={t_asm[7]:.2f}s "
-        f"cg_iter7={t_it * 1e3:.1f}ms gs7={t_gs * 1e3:.1f}ms -> {total_ms / 1e3:.1f}s per depth-{D} solve")
+        f"cg_iter7={t_it * 1e3:.1f}ms (1 thread {t_it1 * 1e3:.1f}ms) gs7={t_gs * 1e3:.1f}ms -> "
+        f"{total_ms / 1e3:.1f}s per depth-{D} solve ({total1_ms / 1e3:.1f}s on one thread)")
     return {"value": round(total_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "value_1thread": round(total1_ms, 1), "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": (f"oracle CSR restatement on the same 1M-point torus: assembly timed at depths 6 and 7 "
                        f"({t_asm[6]:.2f}s, {t_asm[7]:.2f}s), "
-                       f"40 CG iterations and one Gauss-Seidel sweep timed at depth 7, extrapolated linearly in "
+                       f"40 CG iterations (10 more on one thread for value_1thread) and one Gauss-Seidel sweep "
+                       f"timed at depth 7, extrapolated linearly in "
                        f"nodes to depth {D} with the FP64 GPU iteration counts (fine {rep_fp64['fine_iters']}, "
                        f"coarse {rep_fp64['coarse_iters']}); measured, not run at depth {D} (about 140 GB of CSR)")}
 
