@@ -446,11 +446,15 @@ def operator_roofline(L, tree, N, P, reps):
     copy_gbs = copy_bytes / (cms.value * 1e-3) / 1e9
     cg_bytes, dir_bytes, apply_bytes = 24 * N + 12 * P, 16 * N + 12 * P, 8 * N + 12 * P
     achieved = cg_bytes / (kc * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "stencil_cg_step_depth9_torus1M_traffic.json")
-    if N == 8 ** 9 and os.path.exists(tpath):  # measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    def measured_traffic(stem):
+        # HBM-side bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at depth 9
+        tpath = os.path.join(ROOT, "profiles", f"{stem}_depth9_torus1M_traffic.json")
+        if N != 8 ** 9 or not os.path.exists(tpath):
+            return None
         with open(tpath) as fh:
-            traffic = int(json.load(fh)["traffic_bytes_per_launch"])
+            return int(json.load(fh)["traffic_bytes_per_launch"])
+
+    traffic = measured_traffic("stencil_cg_step")
     gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
     sub = lambda kern, b, ms, g: {"kernel": kern, "bytes": b, "ms": round(ms, 4), "GBs": gbs(b, ms),
                                   "frac": round(gbs(b, ms) / HBM_PEAK_GBS, 4), "gather_ms": round(g, 4)}
@@ -459,7 +463,8 @@ def operator_roofline(L, tree, N, P, reps):
                 "kernel": "k_stencil_tile<float,16,OP_CG>", "bytes_per_launch": cg_bytes,
                 "ms_per_launch": round(kc, 4), "gather_ms": round(gc, 4),
                 "frac_of_copy": round(achieved / copy_gbs, 4),
-                "cg_step_fp64": sub("k_stencil_tile<double,16,OP_CG>", 48 * N + 24 * P, kc64, gc64),
+                "cg_step_fp64": dict(sub("k_stencil_tile<double,16,OP_CG>", 48 * N + 24 * P, kc64, gc64),
+                                     traffic=measured_traffic("stencil_cg_step_fp64")),
                 "dir_apply_fp32": sub("k_stencil_tile<float,16,OP_DIR_APPLY>", dir_bytes, kd, gd),
                 "dir_apply_fp64": sub("k_stencil_tile<double,16,OP_DIR_APPLY>", 32 * N + 24 * P, kd64, gd64),
                 "apply_fp32": sub("k_stencil_tile<float,16,OP_APPLY>", apply_bytes, ka, ga),
