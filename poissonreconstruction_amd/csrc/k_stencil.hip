@@ -830,7 +830,8 @@ __global__ __launch_bounds__(256) void k_stencil_node(LevelArgs<T> L, StencilIO<
 // before by some XCD and is still in the last-level cache.  Measured on the
 // depth-9 grid (32768 16^3 bricks), FP32 direction pass: chunk 1 0.557 ms,
 // 8 0.462, 32 0.440, 64 0.435, 128 0.440, 4096 (one contiguous octant per
-// XCD) 0.465.
+// XCD) 0.465; FP32 fused CG step: chunk 16 0.656 ms, 32 0.645, 64 0.637,
+// 128 0.643.
 #ifndef PRX_XCD_CHUNK
 #define PRX_XCD_CHUNK 64
 #endif
