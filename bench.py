@@ -173,6 +173,12 @@ def main():
                  "mixed_err_points": float(np.abs(f_mix - f_ref).max() / rng_f)}
         if x_fp64 is not None:
             field["fp64_err_nodes"] = float(np.abs(x_fp64 - x_ref).max() / rng_f)
+            tree.solution = x_fp64
+            f_64 = tree.evaluate_many(q)
+            field["fp64_err_points"] = float(np.abs(f_64 - f_ref).max() / rng_f)
+            # the mixed solve against the FP64 GPU solve of the same algorithm stopped at the same residual
+            field["mixed_vs_fp64_nodes"] = float(np.abs(x_mixed - x_fp64).max() / rng_f)
+            field["mixed_vs_fp64_points"] = float(np.abs(f_mix - f_64).max() / rng_f)
         for name, xs in x_mg.items():
             tree.solution = xs
             field[f"mg_{name}_err_nodes"] = float(np.abs(xs - x_ref).max() / rng_f)
