@@ -17,10 +17,18 @@ Parity pins (test file that holds each check):
   CSR assembly            symmetry, 21 nonzeros, zero faces, screening sum identity,
                           rank-1 one-point block, dense brute force at D<=2
                                                                 tests/test_oracle.py
-  splat / RHS             constant-normal and closed-form invariants, brute force
-                                                                tests/test_oracle.py
-  coarse system/prolong   brute-force numpy restatement of the closed forms
-                                                                tests/test_oracle.py
+  splat / RHS             one-point weight orientation (node c gets (1-t)^3 products),
+                          two-point negated weighted average, total weight as a
+                          product of in-range 1-D weights; constant-normal and
+                          closed-form RHS invariants             tests/test_oracle.py
+  coarse system/prolong   impulse placement at the row-major coarse cell of
+                          f >> (D-d); prolongation impulse response and exact
+                          reproduction of a linear field with distinct slopes
+                          per axis; screening sum identities    tests/test_oracle.py
+  marching-cubes tables   SHA-256 digests of the reference's TRI_TABLE,
+                          EDGE_TABLE and corner/edge topology (written from
+                          src/tri_table.inl and src/fem_tree.rs:13-30,587-588
+                          by tests/golden/make_mc_digest.py)    tests/test_mc_oracle.py
   GS                      hand-computed sweep; fixed point = direct solve
                                                                 tests/test_oracle.py
   CG                      KAT src/solvers.rs:234-268; scipy direct solve

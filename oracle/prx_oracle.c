@@ -19,9 +19,11 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
-
-#include "../poissonreconstruction_amd/csrc/mc_cases.h"
 #endif
+
+/* The case table is pinned to the reference's TRI_TABLE / EDGE_TABLE by a
+ * digest fixture (tests/golden/mc_tables_digest.json). */
+#include "../poissonreconstruction_amd/csrc/mc_cases.h"
 
 typedef struct { double px, py, pz, nx, ny, nz; } orc_point;
 

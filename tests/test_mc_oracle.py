@@ -101,3 +101,69 @@ def test_vertices_lie_on_cut_edges_at_the_linear_zero():
     inner = np.all((V >= 1 / 16 - 1e-12) & (V <= 15 / 16 + 1e-12), axis=1)
     assert inner.sum() > 50
     assert np.abs(V[inner] @ a - 0.2).max() < 1e-12
+
+
+# ----------------------------------------------------------------- tables vs the reference's
+# tests/golden/mc_tables_digest.json holds SHA-256 digests of the reference's
+# TRI_TABLE (src/tri_table.inl:1-258), EDGE_TABLE (src/fem_tree.rs:13-30) and
+# corner/edge topology (src/fem_tree.rs:587-588), written by
+# tests/golden/make_mc_digest.py from the reference sources.  The canonical
+# strings below are recomputed from csrc/mc_cases.h (tables) and from the
+# cube topology (cut-edge masks, which the product derives instead of
+# tabulating), so one altered triangle, edge id or corner fails here.
+def _header_topology():
+    src = open(os.path.join(HERE, "..", "poissonreconstruction_amd", "csrc", "mc_cases.h")).read()
+
+    def block(name):
+        s = src.index(name)
+        s = src.index("= {", s) + 3
+        return src[s:src.index("};", s)]
+    nums = lambda txt: [int(v) for v in txt.replace("{", " ").replace("}", " ").replace(",", " ").split()]
+    c = nums(block("kMcCorner[8][3]"))
+    e = nums(block("kMcEdge[12][2]"))
+    return [tuple(c[3 * k:3 * k + 3]) for k in range(8)], [tuple(e[2 * k:2 * k + 2]) for k in range(12)]
+
+
+def _digest(s):
+    import hashlib
+    return hashlib.sha256(s.encode()).hexdigest()
+
+
+def _golden():
+    import json
+    return json.load(open(os.path.join(HERE, "golden", "mc_tables_digest.json")))
+
+
+def test_case_table_matches_reference_tri_table_digest():
+    T = case_table()
+    canon = ";".join(",".join(str(e) for e in c) for c in T)
+    g = _golden()
+    assert sum(len(c) // 3 for c in T) == g["n_triangles"]
+    assert _digest(canon) == g["tri_table_sha256"]
+
+
+def test_derived_cut_edges_match_reference_edge_table_digest():
+    corners, edges = _header_topology()
+    masks = []
+    for c in range(256):
+        m = 0
+        for e, (a, b) in enumerate(edges):
+            if ((c >> a) & 1) != ((c >> b) & 1):
+                m |= 1 << e
+        masks.append(m)
+    assert _digest(",".join(str(m) for m in masks)) == _golden()["edge_table_sha256"]
+
+
+def test_corner_and_edge_topology_match_reference_digest():
+    corners, edges = _header_topology()
+    canon = "|".join(",".join(str(v) for v in c) for c in corners) + "#" + "|".join(f"{a},{b}" for a, b in edges)
+    assert _digest(canon) == _golden()["topology_sha256"]
+    assert corners == CORNER and edges == EDGE
+
+
+def test_digest_detects_one_altered_case():
+    # the pin is sensitive: swapping two edge ids of one triangle changes it
+    T = [list(c) for c in case_table()]
+    T[37][0], T[37][1] = T[37][1], T[37][0]
+    canon = ";".join(",".join(str(e) for e in c) for c in T)
+    assert _digest(canon) != _golden()["tri_table_sha256"]

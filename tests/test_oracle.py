@@ -192,7 +192,11 @@ def test_splat_constant_normal_and_partition_of_unity():
     touched = w > 0
     assert np.allclose(nf[touched], [-0.6, 0.0, -0.8], atol=1e-15)
     inside = np.all([(pts[c] >= 0) & (pts[c] < 1 - 1.0 / 16) for c in ("px", "py", "pz")], axis=0)
-    assert w.sum() == pytest.approx(inside.sum() + np.sum(w[:0]), abs=len(pts) - inside.sum() + 1e-9) or True
+    # total weight = sum over points of the product over axes of the in-range
+    # 1-D weights (floor cells, Q4): 1 for a point with all 8 nodes in range,
+    # a fraction for one near the high faces, 0 for one outside the cube
+    assert w.sum() == pytest.approx(_weight_sum(pts, D, cell=np.floor), rel=1e-12)
+    assert w.sum() < len(pts) and inside.sum() < len(pts)
     # partition of unity for points whose 8 corners are all in range
     sub = pts[inside]
     _, w2 = oracle.splat(D, g2i, sub)
@@ -237,13 +241,13 @@ def test_rhs_closed_forms():
     assert abs((A1 - A0).sum() - 0.0) > 0
 
 
-def _weight_sum(pts, D):
+def _weight_sum(pts, D, cell=np.trunc):
     h = 1.0 / (1 << D)
     res = 1 << D
     tot = 0.0
     for p in pts:
         s = np.array([p["px"], p["py"], p["pz"]]) / h
-        c = np.trunc(s)
+        c = cell(s)
         t = s - c
         w = 1.0
         for a in range(3):
@@ -283,6 +287,101 @@ def test_prolongation_closed_form_q11():
         return 1.0 if f < res // 2 - 1 else (0.5 if f == res // 2 - 1 else 0.0)
     expect = np.array([g(x) * g(y) * g(z) for x, y, z in i2o])
     assert np.array_equal(fine, expect)
+
+
+def test_coarse_rhs_row_major_cell_placement():
+    # build_system_at_depth (src/fem_tree.rs:387-397): fine node (fx,fy,fz)
+    # adds v.x+v.y+v.z to the coarse cell (f >> (D-d)) at row-major
+    # (cz*r+cy)*r+cx.  Distinct coordinates per axis, so an x-major index or a
+    # wrong shift moves the value to another cell.
+    D, d = 5, 3
+    g2i, i2o = oracle.tree(D)
+    res, r = 1 << D, 1 << d
+    nf = np.zeros((8 ** D, 3))
+    for (fx, fy, fz), v in (((5, 9, 22), (0.125, 0.25, 0.5)), ((30, 1, 12), (0.0625, 0.0, 0.0))):
+        nf[g2i[(fz * res + fy) * res + fx]] = v
+    _, cb = oracle.coarse_system(D, d, i2o, nf, 0.0)
+    expect = np.zeros(8 ** d)
+    expect[(5 * r + 2) * r + 1] = 0.875   # cell (1, 2, 5)
+    expect[(3 * r + 0) * r + 7] = 0.0625  # cell (7, 0, 3)
+    assert np.array_equal(cb, expect)
+
+
+def test_prolongation_impulse_row_major():
+    # prolongate (src/fem_tree.rs:310-343): coarse row-major (cz*r+cy)*r+cx
+    # -> fine 2c+delta-1 with weights [0.5,1,0.5] per axis (Q11)
+    D, d = 5, 3
+    g2i, _ = oracle.tree(D)
+    res, r = 1 << D, 1 << d
+    cx, cy, cz = 1, 2, 3
+    coarse = np.zeros(8 ** d)
+    coarse[(cz * r + cy) * r + cx] = 1.0
+    fine = oracle.prolongate(D, d, g2i, coarse)
+    bw = (0.5, 1.0, 0.5)
+    expect = np.zeros(8 ** D)
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                fx, fy, fz = 2 * cx + dx - 1, 2 * cy + dy - 1, 2 * cz + dz - 1
+                expect[g2i[(fz * res + fy) * res + fx]] = bw[dx] * bw[dy] * bw[dz]
+    assert np.array_equal(fine, expect)
+
+
+def test_prolongation_reproduces_a_linear_field():
+    # trilinear prolongation is exact on linear fields: with coarse
+    # v(c) = 1 + 2cx + 5cy + 11cz, every fine node whose three offsets are
+    # <= 2(r-1) (full support under Q11) holds v(f/2) = 1 + fx + 2.5fy + 5.5fz,
+    # exactly (all weights are powers of two).  Distinct slopes per axis pin
+    # the coarse index order.
+    D, d = 5, 3
+    g2i, i2o = oracle.tree(D)
+    r = 1 << d
+    c = np.arange(8 ** d)
+    cx, cy, cz = c % r, (c // r) % r, c // (r * r)
+    fine = oracle.prolongate(D, d, g2i, 1.0 + 2 * cx + 5 * cy + 11 * cz)
+    f = i2o.astype(np.float64)
+    full = np.all(i2o <= 2 * (r - 1), axis=1)
+    assert full.sum() == (2 * r - 1) ** 3
+    assert np.array_equal(fine[full], 1.0 + f[full, 0] + 2.5 * f[full, 1] + 5.5 * f[full, 2])
+    beyond = np.any(i2o > 2 * r - 1, axis=1)
+    assert np.all(fine[beyond] == 0.0)
+
+
+def test_splat_weight_orientation_single_point():
+    # splat_normal_field (src/fem_tree.rs:137-158): c = floor(s), t = s - c;
+    # node c + (dx,dy,dz) gets (dx ? t : 1-t) per axis (node o peaks at s = o, Q4)
+    D = 3
+    g2i, _ = oracle.tree(D)
+    res, h = 1 << D, 1.0 / (1 << D)
+    s = np.array([2.3, 3.6, 4.8])
+    p = oracle.as_points([s * h], [[0.0, 0.0, 1.0]])
+    _, w = oracle.splat(D, g2i, p)
+    t = s - np.floor(s)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                e = ((t[0] if dx else 1 - t[0]) * (t[1] if dy else 1 - t[1]) * (t[2] if dz else 1 - t[2]))
+                node = g2i[((4 + dz) * res + 3 + dy) * res + 2 + dx]
+                assert w[node] == pytest.approx(e, rel=1e-12)
+    assert w[g2i[(4 * res + 3) * res + 2]] == pytest.approx(0.7 * 0.4 * 0.2, rel=1e-12)
+    assert np.count_nonzero(w) == 8
+
+
+def test_splat_normal_is_negated_weighted_average():
+    # two points with distinct normals share node (3,4,5): nf = -(n1 w1 + n2 w2) / (w1 + w2)
+    D = 3
+    g2i, _ = oracle.tree(D)
+    res, h = 1 << D, 1.0 / (1 << D)
+    s1, s2 = np.array([2.3, 3.6, 4.8]), np.array([3.5, 4.25, 5.5])
+    p = oracle.as_points([s1 * h, s2 * h], [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    nf, w = oracle.splat(D, g2i, p)
+    node = g2i[(5 * res + 4) * res + 3]
+    w1 = 0.3 * 0.6 * 0.8
+    w2 = 0.5 * 0.75 * 0.5
+    assert w[node] == pytest.approx(w1 + w2, rel=1e-12)
+    assert nf[node] == pytest.approx([-w1 / (w1 + w2), -w2 / (w1 + w2), 0.0], rel=1e-12, abs=1e-15)
+    # node (2,3,4) is reached by the first point only
+    assert nf[g2i[(4 * res + 3) * res + 2]] == pytest.approx([-1.0, 0.0, 0.0], abs=1e-15)
 
 
 # ----------------------------------------------------------------- GS / CG / SpMV (a13-a16)
