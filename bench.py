@@ -115,6 +115,10 @@ def main():
             log(f"{name} step {i}: {ms:.1f} ms  fine_iters={rep['fine_iters']} outer={rep['outer_iters']} "
                 f"coarse_iters={rep['coarse_iters']} true_rel={rep['true_rel']:.3e} status={rep['status']}")
             if i >= args.warmup:
+                # a timed solve that did not reach the tolerance invalidates the figure
+                if rep["status"] != 0 or not rep["true_rel"] <= args.tol:
+                    raise SystemExit(f"bench.py: {name} step {i} did not converge (status {rep['status']}, "
+                                     f"true_rel {rep['true_rel']:.3e} > {args.tol:.1e})")
                 reps.append((ms, rep))
         return reps
 

@@ -310,7 +310,11 @@ __device__ inline void fin_cg(CgState* st, double dq, double rq, double qq, doub
     st->delta_prev = st->delta;
     st->delta = g;
     st->it += 1;
-    if (!refresh && g <= st->thr) {
+    // a refresh of r follows: FIN_FRESH tests the refreshed residual first,
+    // then forms beta and applies the iteration cap, as the two-pass form does
+    // (src/solvers.rs:49-63)
+    if (refresh) return;
+    if (g <= st->thr) {
         st->done = 1;
         st->iters = st->it;
         return;

@@ -103,12 +103,12 @@ def test_vjp_differentiation_workload(gpu):
     L(theta +- step) - L(theta) is formed without cancellation: the perturbed
     solution is x0 + dx with A' dx = b' - A' x0 solved by FP64 CG to the
     attainable accuracy, and L' - L = sum dd (2 (F - mean F) + dd) with dd the
-    centred change of F.  Coordinates whose +-step crosses a cell face (a
-    kink of the piecewise-linear weights) are reported, not asserted.  Where
-    the loss is strongly curved (a node whose total weight is tiny), the
-    O(step^2) and O(step^4) errors of the central difference D are removed by
-    a two-level Romberg table over D(step), D(step/2), D(step/4), and that
-    coordinate is reported too."""
+    centred change of F.  Every coordinate's plain central-difference error is
+    printed; those whose +-step crosses a cell face (a kink of the
+    piecewise-linear weights) are reported, not asserted.  Where the plain
+    difference misses 1e-4, a two-level Romberg table over D(step), D(step/2),
+    D(step/4) is printed beside it as a diagnostic of the difference's own
+    truncation error."""
     import math
     D, pw, step = 8, 4.0, 1e-4
     pts = unit_points(synthetic.ellipsoids(200000, seed=7))
@@ -141,26 +141,35 @@ def test_vjp_differentiation_workload(gpu):
 
     rng = np.random.default_rng(64)
     picks = rng.choice(6 * len(pts), size=64, replace=False)
-    worst, crossing, curved = 0.0, [], []
+    rows = []  # (point, field, plain central-difference rel. error, crosses a face, Romberg rel. error or None)
     for c in picks:
         p, f = int(c // 6), FIELDS[int(c % 6)]
         g = gpos if f[0] == "p" else gnrm
         an = g[p, "xyz".index(f[1])]
         den = lambda v: max(abs(v), 1e-3 * np.abs(g).max())
         s = pts[f][p] / h
-        if f[0] == "p" and abs(s - np.round(s)) * h < step:
-            crossing.append((p, f))
-            continue
+        crosses = bool(f[0] == "p" and abs(s - np.round(s)) * h < step)
         fd = central(p, f, step)
         rel = abs(fd - an) / den(fd)
-        if rel > 1e-4:
+        rich_rel = None
+        if rel > 1e-4 and not crosses:  # diagnostic: the central difference's own truncation error
             d2, d4 = central(p, f, step / 2), central(p, f, step / 4)
             r1a, r1b = (4.0 * d2 - fd) / 3.0, (4.0 * d4 - d2) / 3.0
             rich = (16.0 * r1b - r1a) / 15.0
-            rel = abs(rich - an) / den(rich)
-            curved.append((p, f, round(abs(fd - an) / den(fd), 6), round(rel, 8)))
-        worst = max(worst, rel)
-    print(f"\n[differentiation workload] 64 coordinates: worst rel {worst:.2e}; "
-          f"{len(crossing)} cross a cell face {crossing}; Romberg used for {curved}")
-    assert worst <= 1e-4
-    assert len(curved) <= 4
+            rich_rel = abs(rich - an) / den(rich)
+        rows.append((p, f, rel, crosses, rich_rel))
+    plain = [r for r in rows if not r[3]]
+    n_ok = sum(r[2] <= 1e-4 for r in plain)
+    print(f"\n[differentiation workload] central differences, step {step}, 64 coordinates: "
+          f"{len(plain)} inside a cell, {n_ok} of them within 1e-4 (worst {max(r[2] for r in plain):.2e}); "
+          f"{len(rows) - len(plain)} cross a cell face (worst {max([r[2] for r in rows if r[3]] or [0.0]):.2e})",
+          flush=True)
+    for r in rows:
+        print(f"  point {r[0]} {r[1]}: central rel {r[2]:.3e}{' (crosses a face)' if r[3] else ''}"
+              f"{'' if r[4] is None else f'; Romberg rel {r[4]:.3e}'}", flush=True)
+    # the plain central difference at the stated step: all but at most one of
+    # the in-cell coordinates within 1e-4 (60 of 61 for this seed); the outlier
+    # sits where a node's total splat weight is tiny and the loss is strongly
+    # curved, and its Romberg extrapolation (a diagnostic) still agrees
+    assert n_ok >= len(plain) - 1 and len(plain) >= 58
+    assert all(r[4] <= 1e-4 for r in plain if r[4] is not None)

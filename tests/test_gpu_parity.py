@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 import poissonreconstruction_amd as pr
+from poissonreconstruction_amd import _lib as L
 from poissonreconstruction_amd import synthetic
 
 pytestmark = pytest.mark.gpu
@@ -247,7 +248,7 @@ def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh, pw):
     assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
 
 
-@pytest.mark.parametrize("cap,refresh", [(37, 0), (37, 10), (45, 10), (8, 0)])
+@pytest.mark.parametrize("cap,refresh", [(37, 0), (37, 10), (45, 10), (8, 0), (40, 10)])
 def test_fused_cg_stops_at_the_cap_with_x_terms_pending(gpu, cap, refresh):
     # Stopped by the iteration cap (the reference's cg_iters / 4 on the fine level, src/fem_tree.rs:458-487) after
     # `cap` fused steps, up to seven x terms are still pending and must be added exactly once: the solution then equals
@@ -263,6 +264,10 @@ def test_fused_cg_stops_at_the_cap_with_x_terms_pending(gpu, cap, refresh):
         sols.append(t.solution)
         reps.append(rep)
     assert reps[0]["coarse_iters"] == reps[1]["coarse_iters"]
+    assert reps[0]["status"] == reps[1]["status"] == L.PRX_ERR_NOT_CONVERGED
+    if refresh and cap % refresh == 0:
+        # the cap falls on a refresh iteration: both forms report the refreshed residual
+        assert reps[1]["fine_rel"] == pytest.approx(reps[0]["fine_rel"], rel=1e-6)
     scale = np.abs(sols[0]).max()
     assert np.abs(sols[0] - sols[1]).max() <= 1e-9 * scale
 

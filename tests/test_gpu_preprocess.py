@@ -1,11 +1,12 @@
 """Point preprocessing on the device (bounding box, unit-cube map, normal
 normalisation; src/reconstructor.rs:31-40, src/geometry.rs:256-269,349-402)
-against the host restatement, bit for bit."""
+against the oracle's restatement and the library's host path, bit for bit."""
 import ctypes
 
 import numpy as np
 import pytest
 
+import oracle
 import poissonreconstruction_amd as pr
 from poissonreconstruction_amd import _lib as L
 from poissonreconstruction_amd import synthetic
@@ -34,6 +35,13 @@ def test_device_transform_is_bit_identical(gpu, cloud):
         raw[:, 2] = 0.25
         raw[:, 3:] *= 3.0
     (oh, mh, mih), (od, md, mid) = both(raw)
+    # the oracle (unit_cube_transform, Mat4::inverse, transform_point/vector, normalize)
+    mo, mio = oracle.unit_transform(raw, 1.1)
+    po = oracle.apply_transform(raw, mo)
+    assert np.array_equal(np.asarray(mo).view(np.uint64), md.view(np.uint64))
+    assert np.array_equal(np.asarray(mio).view(np.uint64), mid.view(np.uint64))
+    for f in ("px", "py", "pz", "nx", "ny", "nz"):
+        assert np.array_equal(np.ascontiguousarray(po[f]).view(np.uint64), np.ascontiguousarray(od[f]).view(np.uint64)), f
     assert np.array_equal(mh.view(np.uint64), md.view(np.uint64))
     assert np.array_equal(mih.view(np.uint64), mid.view(np.uint64))
     assert np.array_equal(oh.view(np.uint64), od.view(np.uint64))
