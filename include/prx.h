@@ -324,6 +324,22 @@ int prx_csr_spmv(int64_t n, const int64_t* off, const int64_t* col, const double
 int prx_csr_cg(int64_t n, const int64_t* off, const int64_t* col, const double* val, const double* b, double* x,
                uint64_t max_iters, double eps, uint64_t* iters);
 
+/* ------------------------------------------------------------ no allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * solve_cg as prx_csr_cg, with off[n+1], col, val, b and x (in/out) already in
+ * device memory on the current device, launched on `stream` (a hipStream_t;
+ * NULL = the null stream).  The solver's scratch (r, d, q, partial sums, CG
+ * scalars) lives in one fixed array of the library per device (about 6.3 MB,
+ * held until the process ends); the call makes no device allocation.  n above
+ * PRX_CSR_CG_NOALLOC_MAX_ROWS returns PRX_ERR_INVALID.  Every call on a device
+ * shares that array: calls that overlap on one device race.  The host reads
+ * the stopping flag every 16 iterations, so the call cannot be graph-captured. */
+#define PRX_CSR_CG_NOALLOC_MAX_ROWS 262144
+int prx_csr_cg_noalloc_device(int64_t n, const int64_t* off_device, const int64_t* col_device, const double* val_device,
+                              const double* b_device, double* x_device, uint64_t max_iters, double eps, uint64_t* iters,
+                              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

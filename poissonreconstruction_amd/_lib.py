@@ -21,6 +21,7 @@ PRX_ERR_BREAKDOWN = 4
 PRX_ERR_DEVICE = 5
 PRX_ERR_OOM = 6
 PRX_ERR_STATE = 7
+PRX_CSR_CG_NOALLOC_MAX_ROWS = 262144  # include/prx.h
 PRX_PREC_FP64 = 0
 PRX_PREC_MIXED = 1
 PRX_RESID_REFERENCE = 0
@@ -151,6 +152,7 @@ SIGNATURES = {
     "prx_solve_batched": (_i32, [_vp, _u32, _u64, _f64, _vp, _i32, _vp, _vp]),
     "prx_csr_spmv": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "prx_csr_cg": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _u64, _f64, _vp]),
+    "prx_csr_cg_noalloc_device": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _u64, _f64, _vp, _vp]),
 }
 
 _lib = None
