@@ -750,6 +750,13 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         return;
     }
     if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched cascadic solve runs in FP64");
+    // the instance trees' own solver workspace (left by multigrid or single
+    // solves) is not used here; return it before the concatenated levels grow
+    PRX_HIP(hipStreamSynchronize(B->stream));
+    for (prx_tree* t : B->inst) {
+        PRX_HIP(hipStreamSynchronize(t->stream));
+        t->release_workspace();
+    }
     build(B);
     if (!B->fine.r.p || !B->coarse.r.p) B->drop_graphs();  // captured graphs hold the old vector addresses
     ensure_cg(B->fine);

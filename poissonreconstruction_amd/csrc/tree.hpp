@@ -139,6 +139,16 @@ struct prx_tree {
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
+    // The solver workspace (CG, mixed-precision and multigrid vectors, the
+    // screening gathers): every solve allocates what it needs on first use,
+    // so this only returns memory, e.g. before a batched solve that needs its
+    // own concatenated vectors.  x, b and the point lists stay.
+    void release_workspace() {
+        r64.release(); d64.release(); q64.release(); s64.release(); tmp64.release();
+        e32.release(); r32.release(); ri32.release(); d32.release(); q32.release(); s32.release();
+        mg64.release(); mg32.release();
+    }
+
     LevelView fine_view() const {
         LevelView v;
         v.D = D;
