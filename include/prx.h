@@ -259,6 +259,13 @@ int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, d
  * each with its screening gather. */
 int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
+/* Measurement: the phases of the last prx_splat_normal_field, ms[5] =
+ * {host packing and upload of the points, spatial sort of the points, the
+ * splat's per-node lists, the screening (truncated-cell) per-node lists, the
+ * splat kernel}.  The screening lists stand in for the per-point blocks the
+ * reference pushes in assemble_system (src/fem_tree.rs:199-227); they are
+ * built once per point set and reused by every assembly and solve. */
+int prx_splat_times(prx_tree* t, double* ms);
 /* Measurement: evaluate the current solution at every marching-cubes corner
  * (as prx_evaluate_corners, src/fem_tree.rs:596-605) into a device buffer
  * `reps` times; *ms = average ms per evaluation, *sumsq = sum of the squared

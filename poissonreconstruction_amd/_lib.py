@@ -140,6 +140,7 @@ SIGNATURES = {
     "prx_time_operator": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "prx_time_stencil_op": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp]),
     "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
+    "prx_splat_times": (_i32, [_vp, _vp]),
     "prx_time_evaluate_corners": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "prx_extract_surface": (_i32, [_vp, _f64, _vp]),
     "prx_unit_transform_device": (_i32, [_vp, _sz, _f64, _i32, _vp, _vp, _vp]),

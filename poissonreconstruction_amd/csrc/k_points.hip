@@ -328,13 +328,27 @@ __global__ void k_rhs_laplacian(uint32_t D, const double* __restrict__ nf, doubl
             }
     rhs[i] = sum;
 }
-// The same sums with the operands staged in LDS: one workgroup per aligned
-// 8^3 brick (a contiguous Morton range) loads the brick's three normal-field
-// components and the one-node halo, then each thread forms two nodes with
-// exactly the per-node expression and neighbour order above, so the result is
-// bit-identical.
+// The same sums, bit for bit, from operands staged in LDS.  One workgroup
+// per aligned 8^3 brick (a contiguous Morton range) loads the brick's three
+// normal-field components and the one-node halo (zeros outside the grid).
+//
+// Exact restatement.  A term of the reference's sum is h*h*(a + b + c) with
+// a = ((vx*G(dx))*M(dy))*M(dz), b = ((vy*M(dx))*G(dy))*M(dz),
+// c = ((vz*M(dx))*M(dy))*G(dz).  G(d) is 0 or -+1/2 and h*h = 2^-2D, and
+// scaling by a power of two commutes with rounding (no subnormal operands:
+// the field components are averages of unit normals), so
+//   a = -+1/2 fl(fl(vx*M(dy))*M(dz)),  b = -+1/2 fl(fl(vy*M(dx))*M(dz)),
+//   c = -+1/2 fl(fl(vz*M(dx))*M(dy)),
+// a part with G(0) = 0 is an exact zero that leaves the sum a + b + c
+// unchanged, and so are the terms of out-of-grid neighbours (zero halo; the
+// reference skips them; a running sum is never -0, so adding a zero changes
+// nothing).  The node's sum is then 2^(-2D-1) times the sequential sum, in
+// the same dx, dy, dz order, of the terms' nonzero parts: 2 multiplies per
+// part instead of 4, no bounds tests, and, since the neighbour order is a
+// sweep along z, each thread forms two z-adjacent nodes of one (x, y) column
+// from the 4 planes they share (21 LDS reads per node instead of 81).
 __global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const double* __restrict__ nf, double* rhs) {
-    constexpr int E = 10, E3 = E * E * E;
+    constexpr int E = 10, E2 = E * E, E3 = E * E * E;
     __shared__ double v[3][E3];
     const uint32_t res = 1u << D;
     const uint64_t N = 1ull << (3 * D);
@@ -345,46 +359,102 @@ __global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const do
     demorton3(bb, bx, by, bz);
     const int ox = (int)bx * 8, oy = (int)by * 8, oz = (int)bz * 8;
     typedef double d2 __attribute__((ext_vector_type(2)));
-    // interior: thread t holds local Morton nodes 2t and 2t+1 (same x pair)
+    // interior: thread t loads local Morton nodes 2t and 2t+1 (an x pair)
     const uint32_t m0 = 2u * (uint32_t)tid;
-    uint32_t lx, ly, lz;
-    demorton3(m0, lx, ly, lz);
-    const int li = ((int)(lz + 1) * E + (int)ly + 1) * E + (int)lx + 1;
+    {
+        uint32_t lx, ly, lz;
+        demorton3(m0, lx, ly, lz);
+        const int li = ((int)(lz + 1) * E + (int)ly + 1) * E + (int)lx + 1;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const d2 q = *(const d2*)(nf + (uint64_t)c * N + base + m0);
-        v[c][li] = q[0];
-        v[c][li + 1] = q[1];
+        for (int c = 0; c < 3; ++c) {
+            const d2 q = *(const d2*)(nf + (uint64_t)c * N + base + m0);
+            v[c][li] = q[0];
+            v[c][li + 1] = q[1];
+        }
     }
-    // halo shell (10^3 - 8^3 = 488 nodes); values outside the grid are never read
-    for (int i = tid; i < E3; i += 256) {
-        const int hx = i % E - 1, hy = (i / E) % E - 1, hz = i / (E * E) - 1;
-        if (hx >= 0 && hx < 8 && hy >= 0 && hy < 8 && hz >= 0 && hz < 8) continue;
+    // halo shell (10^3 - 8^3 = 488 nodes), zero outside the grid: every
+    // thread issues all of its loads before it writes any of them
+    constexpr int NH = (E3 + 255) / 256;
+    double hv[NH][3];
+    int hl[NH];
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int i = tid + 256 * k;
+        const int hx = i % E - 1, hy = (i / E) % E - 1, hz = i / E2 - 1;
+        const bool shell = i < E3 && !(hx >= 0 && hx < 8 && hy >= 0 && hy < 8 && hz >= 0 && hz < 8);
         const int gx = ox + hx, gy = oy + hy, gz = oz + hz;
-        if (gx < 0 || gy < 0 || gz < 0 || gx >= (int)res || gy >= (int)res || gz >= (int)res) continue;
-        const uint32_t j = morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz);
+        const bool in = shell && gx >= 0 && gy >= 0 && gz >= 0 && gx < (int)res && gy < (int)res && gz < (int)res;
+        const uint32_t j = in ? morton3((uint32_t)gx, (uint32_t)gy, (uint32_t)gz) : 0u;
+        hl[k] = shell ? i : -1;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c][i] = nf[(uint64_t)c * N + j];
+        for (int c = 0; c < 3; ++c) hv[k][c] = in ? nf[(uint64_t)c * N + j] : 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < NH; ++k)
+        if (hl[k] >= 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c][hl[k]] = hv[k][c];
+        }
     __syncthreads();
-    const double h = 1.0 / (double)res;
-    d2 out;
+    // thread -> column (cx, cy), nodes z0 and z0 + 1
+    const int cx = tid & 7, cy = (tid >> 3) & 7, z0 = (tid >> 6) * 2;
+    constexpr double M0 = 2. / 3., M1 = 1. / 6.;  // mass_1d(0), mass_1d(+-1)
+    double S[2] = {0.0, 0.0};
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const int x = (int)lx + e, gxc = ox + x, gyc = oy + (int)ly, gzc = oz + (int)lz;
-        double sum = 0.0;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            // planes z0-1 .. z0+2 of the neighbour column (tile z index z0 .. z0+3)
+            const int col = (cy + 1 + dy) * E + cx + 1 + dx;
+            double X[4], Y[4], Z[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const int k = (z0 + p) * E2 + col;
+                X[p] = dx != 0 ? v[0][k] : 0.0;
+                Y[p] = dy != 0 ? v[1][k] : 0.0;
+                Z[p] = v[2][k];
+            }
+            const double mx = dx != 0 ? M1 : M0, my = dy != 0 ? M1 : M0;
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+#pragma unroll
                 for (int dz = -1; dz <= 1; ++dz) {
-                    const int jx = gxc + dx, jy = gyc + dy, jz = gzc + dz;
-                    if (jx < 0 || jx >= (int)res || jy < 0 || jy >= (int)res || jz < 0 || jz >= (int)res) continue;
-                    const int k = (((int)lz + 1 + dz) * E + (int)ly + 1 + dy) * E + x + 1 + dx;
-                    const double vx = v[0][k], vy = v[1][k], vz = v[2][k];
-                    sum += h * h * (vx * deriv_mass_1d(dx) * mass_1d(dy) * mass_1d(dz) + vy * mass_1d(dx) * deriv_mass_1d(dy) * mass_1d(dz) +
-                                    vz * mass_1d(dx) * mass_1d(dy) * deriv_mass_1d(dz));
+                    const int p = o + 1 + dz;
+                    const double mz = dz != 0 ? M1 : M0;
+                    // parts in a, b, c order; G(-1) = +1/2, G(+1) = -1/2
+                    double t = 0.0;
+                    bool have = false;
+                    if (dx != 0) {
+                        const double a = (X[p] * my) * mz;
+                        t = dx < 0 ? a : -a;
+                        have = true;
+                    }
+                    if (dy != 0) {
+                        const double b = (Y[p] * mx) * mz;
+                        const double bs = dy < 0 ? b : -b;
+                        t = have ? t + bs : bs;
+                        have = true;
+                    }
+                    if (dz != 0) {
+                        const double c = (Z[p] * mx) * my;
+                        const double cs = dz < 0 ? c : -c;
+                        t = have ? t + cs : cs;
+                        have = true;
+                    }
+                    if (have) S[o] = S[o] + t;
                 }
-        out[e] = sum;
+            }
+        }
     }
+    const double scale = ldexp(1.0, -2 * (int)D - 1);
+    __syncthreads();  // the tile is read; stage the outputs in v[0] for a coalesced store
+    const uint32_t mc = spread3((uint32_t)cx) | (spread3((uint32_t)cy) << 1);
+    v[0][mc | (spread3((uint32_t)z0) << 2)] = S[0] * scale;
+    v[0][mc | (spread3((uint32_t)z0 + 1u) << 2)] = S[1] * scale;
+    __syncthreads();
+    d2 out;
+    out[0] = v[0][m0];
+    out[1] = v[0][m0 + 1];
     *(d2*)(rhs + base + m0) = out;
 }
 

@@ -948,6 +948,7 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
 
 // ------------------------------------------------------------------ splat and assembly
 void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
+    const auto h0 = std::chrono::steady_clock::now();
     const int64_t P = (int64_t)n;
     if (P >= ((int64_t)1 << 29)) throw Error(PRX_ERR_INVALID, "at most 2^29 points per tree");
     std::vector<double> soa(6 * (size_t)std::max<int64_t>(P, 1));
@@ -963,6 +964,7 @@ void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
         PRX_HIP(hipMemcpyAsync(t->nrm.p, soa.data() + 3 * P, sizeof(double) * 3 * P, hipMemcpyHostToDevice, t->stream));
     }
     PRX_HIP(hipStreamSynchronize(t->stream));
+    t->setup_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
 }
 
 void splat(prx_tree* t) {
@@ -984,7 +986,9 @@ void splat(prx_tree* t) {
     t->pts.t32 = t->t32.p;
     t->nf.ensure(3 * t->N);
     t->w.ensure(t->N);
+    for (int i = 1; i < 5; ++i) t->setup_ms[i] = 0.0;
     if (P > 0) {
+        Timer tsort(s);
         DArr<uint32_t> kf, kt, val, kf2, kt2, val2, val3, ck, cv, ck2, inv;
         kf.ensure(M8); kt.ensure(M8); val.ensure(M8); kf2.ensure(M8); kt2.ensure(M8); val2.ensure(M8); val3.ensure(M8);
         ck.ensure(P); cv.ensure(P); ck2.ensure(P); inv.ensure(P);
@@ -1001,16 +1005,23 @@ void splat(prx_tree* t) {
         PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, ck.p, ck2.p, cv.p, t->perm.p, (int)P, 0, kbits, s));
         launch_invert_perm(t->perm.p, P, inv.p, s);
         launch_point_sorted_data(t->D, t->pos.p, P, t->perm.p, t->cell.p, t->t64.p, t->t32.p, s);
+        t->setup_ms[1] = tsort.ms();
         // floor list (splat), trunc list (screening); radix sort is stable, so
         // entries of one node stay in input point order
+        Timer tfloor(s);
         PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, kf.p, kf2.p, val.p, val2.p, (int)M8, 0, kbits, s));
         build_node_list(t->floor_list, t->D, kf2.p, val2.p, M8, t->pts, inv.p, scratch.p, sb, s);
+        t->setup_ms[2] = tfloor.ms();
+        // the screening lists: the reference builds the per-point blocks in
+        // assemble_system (src/fem_tree.rs:199-227)
+        Timer ttrunc(s);
         PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, kt.p, kt2.p, val.p, val3.p, (int)M8, 0, kbits, s));
         build_node_list(t->trunc_list, t->D, kt2.p, val3.p, M8, t->pts, inv.p, scratch.p, sb, s);
-        PRX_HIP(hipStreamSynchronize(s));
+        t->setup_ms[3] = ttrunc.ms();
     }
+    Timer tsplat(s);
     launch_splat(t->D, t->floor_list, t->pts, t->nf.p, t->w.p, s);
-    PRX_HIP(hipStreamSynchronize(s));
+    t->setup_ms[4] = tsplat.ms();
     t->splatted = true;
     t->assembled = false;
 }
@@ -1338,6 +1349,14 @@ int prx_splat_normal_field(prx_tree* t, const prx_point* pts, size_t n) {
     set_device(t);
     prx::upload_points(t, pts, n);
     prx::splat(t);
+    PRX_API_END
+}
+
+int prx_splat_times(prx_tree* t, double* ms) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && ms, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->splatted, PRX_ERR_STATE, "splat_normal_field not called");
+    for (int i = 0; i < 5; ++i) ms[i] = t->setup_ms[i];
     PRX_API_END
 }
 

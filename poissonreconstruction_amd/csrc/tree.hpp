@@ -127,6 +127,9 @@ struct prx_tree {
     DArr<double> mg64, mg_ainv;
     DArr<float> mg32;
     int precond = 0;  // solver of the last solve (the adjoint solve uses the same)
+    // last prx_splat_normal_field, ms: host packing + upload, spatial point
+    // sort, splat lists, screening lists, splat kernel
+    double setup_ms[5] = {0, 0, 0, 0, 0};
     int fused_cg = 1;  // CG iteration form of the last solve (prx_solve_opts.cg_two_pass == 0)
 
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }

@@ -116,6 +116,12 @@ class FEMTree:
         self._points = pts
         check(L.lib().prx_splat_normal_field(self._h, ptr(pts), len(pts)))
 
+    def splat_times(self) -> dict:
+        """ms of the phases of the last splat_normal_field (measurement)."""
+        ms = np.zeros(5)
+        check(L.lib().prx_splat_times(self._h, ptr(ms)))
+        return dict(zip(("upload", "point_sort", "splat_lists", "screening_lists", "splat"), ms.tolist()))
+
     @property
     def normal_field(self) -> np.ndarray:
         nf = np.empty((self.fem_node_count, 3))
