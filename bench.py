@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--large-fp64", action="store_true", help="with --large: also time the FP64 solve")
     ap.add_argument("--op-only", action="store_true",
                     help="only time the operator kernels and the copy kernel (for counter runs)")
+    ap.add_argument("--mg-only", action="store_true",
+                    help="only run the multigrid-preconditioned solves (mixed, then FP64; warmup + steps each) on the "
+                         "precision workload and print their line (for kernel traces)")
     ap.add_argument("--cpu-configs", action="store_true",
                     help="only run the oracle's full solves of the plain (depth 6) and differentiation (depth 8) "
                          "workloads on the host beside the GPU solves, and print their line")
@@ -132,6 +135,17 @@ def main():
                                      f"true_rel {rep['true_rel']:.3e} > {args.tol:.1e})")
                 reps.append((ms, rep))
         return reps
+
+    if args.mg_only:
+        res = {}
+        for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):
+            runs = run(prec, f"mg {name}", fn=solve_mg)
+            res[name] = {"ms": round(float(np.mean([m for m, _ in runs])), 2),
+                         "fine_iters": runs[-1][1]["fine_iters"], "outer_iters": runs[-1][1]["outer_iters"],
+                         "true_rel": runs[-1][1]["true_rel"]}
+        res["mixed_speedup_vs_fp64"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
+        print(json.dumps({"mg_only": True, "multigrid": res}), flush=True)
+        return
 
     if args.op_only:
         L.check(L.lib().prx_assemble(tree._h, pw))

@@ -14,20 +14,22 @@
 namespace prx {
 
 template <typename T>
-__global__ void k_mg_scale(T* __restrict__ x, const T* __restrict__ b, double w, int64_t n) {
+__global__ void k_mg_scale(T* __restrict__ x, const T* __restrict__ b, double w, int64_t n, const int32_t* done) {
+    if (done && *done) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         x[i] = (T)(w * (double)b[i]);
 }
 template <typename T>
-void launch_mg_scale(T* x, const T* b, double w, int64_t n, hipStream_t s) {
+void launch_mg_scale(T* x, const T* b, double w, int64_t n, hipStream_t s, const int32_t* done) {
     const int64_t nb = std::min<int64_t>(65536, std::max<int64_t>(1, (n + 1023) / 1024));
-    hipLaunchKernelGGL(k_mg_scale<T>, dim3((unsigned)nb), dim3(256), 0, s, x, b, w, n);
+    hipLaunchKernelGGL(k_mg_scale<T>, dim3((unsigned)nb), dim3(256), 0, s, x, b, w, n, done);
     PRX_LAUNCH_CHECK();
 }
 
 // b_c[I] = sum_{delta in {-1,0,1}^3} 2^-(number of nonzero delta) r_f[2I + delta]
 template <typename T>
-__global__ void k_mg_restrict(uint32_t dc, const T* __restrict__ rf, T* __restrict__ bc) {
+__global__ void k_mg_restrict(uint32_t dc, const T* __restrict__ rf, T* __restrict__ bc, const int32_t* done) {
+    if (done && *done) return;
     const uint32_t nc = 1u << (3 * dc);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
@@ -57,7 +59,9 @@ __global__ void k_mg_restrict(uint32_t dc, const T* __restrict__ rf, T* __restri
 // (a contiguous Morton range) plus the one-node layer below it in x, y and z
 // into LDS, and writes the aligned 8^3 coarse brick (also contiguous).
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_restrict_tile(uint32_t df, const T* __restrict__ rf, T* __restrict__ bc) {
+__global__ __launch_bounds__(256) void k_mg_restrict_tile(uint32_t df, const T* __restrict__ rf, T* __restrict__ bc,
+                                                          const int32_t* done) {
+    if (done && *done) return;
     constexpr int E = 17;  // local fine coordinates -1..15
     __shared__ T f[E * E * E];
     using NV = typename NV16<T>::type;
@@ -118,19 +122,20 @@ __global__ __launch_bounds__(256) void k_mg_restrict_tile(uint32_t df, const T* 
 }
 
 template <typename T>
-void launch_mg_restrict(uint32_t dc, const T* rf, T* bc, hipStream_t s) {
+void launch_mg_restrict(uint32_t dc, const T* rf, T* bc, hipStream_t s, const int32_t* done) {
     const int64_t nc = (int64_t)1 << (3 * dc);
     if (dc >= 3) {  // fine grid of at least 16^3
-        hipLaunchKernelGGL(k_mg_restrict_tile<T>, dim3((unsigned)(nc >> 9)), dim3(256), 0, s, dc + 1, rf, bc);
+        hipLaunchKernelGGL(k_mg_restrict_tile<T>, dim3((unsigned)(nc >> 9)), dim3(256), 0, s, dc + 1, rf, bc, done);
     } else {
-        hipLaunchKernelGGL(k_mg_restrict<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, dc, rf, bc);
+        hipLaunchKernelGGL(k_mg_restrict<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, dc, rf, bc, done);
     }
     PRX_LAUNCH_CHECK();
 }
 
 // x_f[i] += sum_I P_iI x_c[I]
 template <typename T>
-__global__ void k_mg_prolong_add(uint32_t df, const T* __restrict__ xc, T* __restrict__ xf) {
+__global__ void k_mg_prolong_add(uint32_t df, const T* __restrict__ xc, T* __restrict__ xf, const int32_t* done) {
+    if (done && *done) return;
     const uint32_t nf = 1u << (3 * df);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
@@ -165,7 +170,9 @@ __global__ void k_mg_prolong_add(uint32_t df, const T* __restrict__ xc, T* __res
 // brick plus the coarse layer above it (coordinate 8; absent at the grid's
 // upper end) into LDS and updates the fine brick with 16-byte accesses.
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_prolong_tile(uint32_t df, const T* __restrict__ xc, T* __restrict__ xf) {
+__global__ __launch_bounds__(256) void k_mg_prolong_tile(uint32_t df, const T* __restrict__ xc, T* __restrict__ xf,
+                                                         const int32_t* done) {
+    if (done && *done) return;
     constexpr int E = 9;  // local coarse coordinates 0..8
     __shared__ T c[E * E * E];
     using NV = typename NV16<T>::type;
@@ -210,20 +217,22 @@ __global__ __launch_bounds__(256) void k_mg_prolong_tile(uint32_t df, const T* _
 }
 
 template <typename T>
-void launch_mg_prolong_add(uint32_t df, const T* xc, T* xf, hipStream_t s) {
+void launch_mg_prolong_add(uint32_t df, const T* xc, T* xf, hipStream_t s, const int32_t* done) {
     const int64_t nf = (int64_t)1 << (3 * df);
     if (df >= 4) {
-        hipLaunchKernelGGL(k_mg_prolong_tile<T>, dim3((unsigned)(nf >> 12)), dim3(256), 0, s, df, xc, xf);
+        hipLaunchKernelGGL(k_mg_prolong_tile<T>, dim3((unsigned)(nf >> 12)), dim3(256), 0, s, df, xc, xf, done);
     } else {
-        hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, df, xc, xf);
+        hipLaunchKernelGGL(k_mg_prolong_add<T>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, df, xc, xf, done);
     }
     PRX_LAUNCH_CHECK();
 }
 
 // x = Ainv b, one workgroup; b staged in LDS, FP64 products
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_dense(int n, const double* __restrict__ ainv, const T* __restrict__ b, T* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_mg_dense(int n, const double* __restrict__ ainv, const T* __restrict__ b, T* __restrict__ x,
+                                                  const int32_t* done) {
     __shared__ double sb[512];
+    if (done && *done) return;
     for (int j = threadIdx.x; j < n; j += blockDim.x) sb[j] = (double)b[j];
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -234,19 +243,19 @@ __global__ __launch_bounds__(256) void k_mg_dense(int n, const double* __restric
     }
 }
 template <typename T>
-void launch_mg_dense(int n, const double* ainv, const T* b, T* x, hipStream_t s) {
+void launch_mg_dense(int n, const double* ainv, const T* b, T* x, hipStream_t s, const int32_t* done) {
     if (n > 512) throw Error(PRX_ERR_INVALID, "dense coarse solve limited to 512 nodes");
-    hipLaunchKernelGGL(k_mg_dense<T>, dim3(1), dim3(256), 0, s, n, ainv, b, x);
+    hipLaunchKernelGGL(k_mg_dense<T>, dim3(1), dim3(256), 0, s, n, ainv, b, x, done);
     PRX_LAUNCH_CHECK();
 }
 
-template void launch_mg_scale<double>(double*, const double*, double, int64_t, hipStream_t);
-template void launch_mg_scale<float>(float*, const float*, double, int64_t, hipStream_t);
-template void launch_mg_restrict<double>(uint32_t, const double*, double*, hipStream_t);
-template void launch_mg_restrict<float>(uint32_t, const float*, float*, hipStream_t);
-template void launch_mg_prolong_add<double>(uint32_t, const double*, double*, hipStream_t);
-template void launch_mg_prolong_add<float>(uint32_t, const float*, float*, hipStream_t);
-template void launch_mg_dense<double>(int, const double*, const double*, double*, hipStream_t);
-template void launch_mg_dense<float>(int, const double*, const float*, float*, hipStream_t);
+template void launch_mg_scale<double>(double*, const double*, double, int64_t, hipStream_t, const int32_t*);
+template void launch_mg_scale<float>(float*, const float*, double, int64_t, hipStream_t, const int32_t*);
+template void launch_mg_restrict<double>(uint32_t, const double*, double*, hipStream_t, const int32_t*);
+template void launch_mg_restrict<float>(uint32_t, const float*, float*, hipStream_t, const int32_t*);
+template void launch_mg_prolong_add<double>(uint32_t, const double*, double*, hipStream_t, const int32_t*);
+template void launch_mg_prolong_add<float>(uint32_t, const float*, float*, hipStream_t, const int32_t*);
+template void launch_mg_dense<double>(int, const double*, const double*, double*, hipStream_t, const int32_t*);
+template void launch_mg_dense<float>(int, const double*, const float*, float*, hipStream_t, const int32_t*);
 
 }  // namespace prx

@@ -111,13 +111,13 @@ template <typename T> void launch_x_flush(T* x, const XDirs<T>& d, const CgState
 
 // ---- k_mg.hip: geometric multigrid on the dense Morton grids
 // x = w * b
-template <typename T> void launch_mg_scale(T* x, const T* b, double w, int64_t n, hipStream_t s);
+template <typename T> void launch_mg_scale(T* x, const T* b, double w, int64_t n, hipStream_t s, const int32_t* done = nullptr);
 // b_c = P^T r_f from depth dc + 1 to dc (trilinear hats, nodes outside the grid absent)
-template <typename T> void launch_mg_restrict(uint32_t dc, const T* rf, T* bc, hipStream_t s);
+template <typename T> void launch_mg_restrict(uint32_t dc, const T* rf, T* bc, hipStream_t s, const int32_t* done = nullptr);
 // x_f += P x_c from depth df - 1 to df
-template <typename T> void launch_mg_prolong_add(uint32_t df, const T* xc, T* xf, hipStream_t s);
+template <typename T> void launch_mg_prolong_add(uint32_t df, const T* xc, T* xf, hipStream_t s, const int32_t* done = nullptr);
 // x = Ainv b on a grid of n <= 512 nodes (Ainv row-major, FP64)
-template <typename T> void launch_mg_dense(int n, const double* ainv, const T* b, T* x, hipStream_t s);
+template <typename T> void launch_mg_dense(int n, const double* ainv, const T* b, T* x, hipStream_t s, const int32_t* done = nullptr);
 template <typename T> void launch_xpby_beta(T* d, const T* r, const CgState* st, int64_t n, hipStream_t s);
 template <typename T> void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s);
 void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s);

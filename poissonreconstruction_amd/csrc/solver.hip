@@ -681,8 +681,10 @@ static Mg<T> mg_build(prx_tree* t, int nu, double omega) {
 // first two pre-smoothing steps from x = 0 are one pass over b (OP_JACOBI2);
 // the last post-smoothing pass of the finest level also leaves the partial
 // sums of b.z in t->partial.  Returns their count (0: none were formed).
+// done (optional): every kernel of the cycle returns at once when *done is
+// set, so the PCG loop can enqueue a cycle before it knows whether it is needed.
 template <typename T>
-static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
+static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine, const int32_t* done = nullptr) {
     const hipStream_t s = t->stream;
     const int top = (int)m.lv.size() - 1;
     std::vector<T*> cur(top + 1, nullptr), oth(top + 1, nullptr);
@@ -694,6 +696,7 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
         io.y = o;
         io.jw = w;
         io.partial = rz ? t->partial.p : nullptr;
+        io.done = done;
         int nb = 0;
         launch_stencil<T>(m.lv[l], OP_JACOBI, io, &nb, t->tile, s);
         if (rz) nb_rz = nb;
@@ -714,32 +717,34 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
             io.y = c;
             io.jw = m.jw[l][0];
             io.jw2 = m.jw[l][1];
+            io.done = done;
             int nb = 0;
             launch_stencil<T>(L, OP_JACOBI2, io, &nb, t->tile, s);
             for (int i = 2; i < m.nu; ++i) jacobi(l, b, c, o, m.jw[l][i], false);
         } else {
-            launch_mg_scale<T>(c, b, m.jw[l][0], L.N, s);
+            launch_mg_scale<T>(c, b, m.jw[l][0], L.N, s, done);
         }
         StencilIO<T> io;
         io.x = c;
         io.b = b;
         io.y = m.rv[l];
+        io.done = done;
         int nb = 0;
         launch_stencil<T>(L, OP_RESID, io, &nb, t->tile, s);
-        launch_mg_restrict<T>(L.D - 1, m.rv[l], m.bv[l - 1], s);
+        launch_mg_restrict<T>(L.D - 1, m.rv[l], m.bv[l - 1], s, done);
         cur[l] = c;
         oth[l] = o;
     }
-    launch_mg_dense<T>(m.nc, t->mg_ainv.p, top == 0 ? b_fine : m.bv[0], top == 0 ? z_fine : m.xa[0], s);
+    launch_mg_dense<T>(m.nc, t->mg_ainv.p, top == 0 ? b_fine : m.bv[0], top == 0 ? z_fine : m.xa[0], s, done);
     cur[0] = top == 0 ? z_fine : m.xa[0];
     for (int l = 1; l <= top; ++l) {
         const T* b = l == top ? b_fine : m.bv[l];
-        launch_mg_prolong_add<T>(m.lv[l].D, cur[l - 1], cur[l], s);
+        launch_mg_prolong_add<T>(m.lv[l].D, cur[l - 1], cur[l], s, done);
         // post-smoothing: the same steps in reverse order (M stays symmetric)
         for (int i = 0; i < m.nu; ++i) jacobi(l, b, cur[l], oth[l], m.jw[l][m.nu - 1 - i], l == top && i == m.nu - 1);
     }
     if (top > 0 && cur[top] != z_fine) throw Error(PRX_ERR_STATE, "multigrid buffer parity");
-    m.cycles++;
+    if (!done) m.cycles++;  // conditional cycles are counted by the caller once it knows which ran
     return nb_rz;
 }
 
@@ -747,8 +752,12 @@ static int mg_vcycle(prx_tree* t, Mg<T>& m, const T* b_fine, T* z_fine) {
 // screening; x is updated in place.  Stops when |r|^2 <= bnorm2 eps^2 (r the
 // recursive residual) or after max_iters iterations.  The update
 // d = z + beta d is fused into the operator pass as in the plain CG, and x
-// is updated lazily in pairs.  One host round trip per iteration decides
-// whether the next V-cycle is needed.
+// is updated lazily in pairs.  The host does not wait for an iteration
+// before enqueueing the next: each iteration's state is copied to pinned
+// memory behind its last fold, and the host reads iteration k-1's copy only
+// after it has enqueued iteration k, so the device never waits on the host.
+// The one iteration enqueued past convergence (or the cap) costs only its
+// launches: every kernel returns at once when the device flag `done` is set.
 template <typename T>
 static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b, bool x_zero, T* r, T* z, T* dd, T* q,
                      double eps, double bnorm2, uint64_t max_iters) {
@@ -770,6 +779,13 @@ static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b
     launch_fin(t->st, t->partial.p, nb, FIN_PINIT, 0, s);
     launch_fill<T>(dd, T(0), n, s);
     read_state(t);
+    CgState* snap = t->state_snapshots();
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto& e : ev) PRX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() { for (int i = 0; i < 2; ++i) if (e[i]) (void)hipEventDestroy(e[i]); }
+    } evg{ev};
     uint64_t k = 0;
     while (!t->st_host->done) {
         T* dold = dd + (k & 1) * n;
@@ -793,12 +809,19 @@ static CgOut run_pcg(prx_tree* t, Mg<T>& m, const LevelView& L, T* x, const T* b
         launch_cg_update<T>(x, r, dold, dnew, q, t->st, n, t->partial.p, &nb, s);
         launch_fin(t->st, t->partial.p, nb, FIN_PRES, (int)((k + 1) & 1), s);
         ++k;
-        read_state(t);
-        if (t->st_host->done) break;
-        nb = mg_vcycle<T>(t, m, r, z);
+        PRX_HIP(hipMemcpyAsync(snap + (k & 1), t->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        PRX_HIP(hipEventRecord(ev[k & 1], s));
+        if (k >= 2) {  // iteration k-1 finished the solve: iteration k's kernels returned at once
+            PRX_HIP(hipEventSynchronize(ev[(k - 1) & 1]));
+            if (snap[(k - 1) & 1].done) break;
+        }
+        nb = mg_vcycle<T>(t, m, r, z, &t->st->done);
         if (nb == 0) launch_dot<T>(r, z, n, t->partial.p, &nb, s);
         launch_fin(t->st, t->partial.p, nb, FIN_PBETA, 0, s);
     }
+    read_state(t);
+    // the conditional cycles that ran: one after each iteration but the last
+    if (t->st_host->it > 1) m.cycles += (uint64_t)(t->st_host->it - 1);
     if (t->st_host->xpend) {
         launch_flush_x<T>(x, dd, dd + n, t->st, n, s);
         launch_clear_xpend(t->st, s);
@@ -855,6 +878,8 @@ static void mg_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_solv
     rep.outer_iters = 1;
     rep.coarse_iters = m.cycles;
     rep.fine_rel = bb > 0.0 ? std::sqrt(true2 / bb) : 0.0;
+    // true2 is ||b - A x||^2 of the returned x unless PCG broke down before its confirmation
+    if (status != PRX_ERR_BREAKDOWN) rep.true_rel = rep.fine_rel;
     rep.status = status;
 }
 
@@ -876,12 +901,10 @@ static void mg_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_sol
     launch_fill<double>(t->x.p, 0.0, n, s);
     const double bb = dev_dot<double>(t, t->b.p, t->b.p, n);
     const double thr = (bb * c.eps) * c.eps;
-    launch_fill<float>(t->e32.p, 0.0f, n, s);
-    int nb = op_resid_update(t, L, t->e32.p, t->b.p, t->ri32.p, ih);
-    rep.applies++;
-    launch_fin(t->st, t->partial.p, nb, FIN_SLOT, 0, s);
-    read_state(t);
-    double rr = t->st_host->slot[0];
+    // x = 0: the first residual is b itself (scaled into the FP32 system)
+    launch_cast_scale_d2f(t->ri32.p, t->b.p, ih, n, s);
+    double rr = bb;
+    int nb = 0;
     uint64_t iters = 0, outer = 0;
     int status = PRX_OK;
     while (rr > thr) {
@@ -908,15 +931,18 @@ static void mg_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_sol
     rep.outer_iters = outer;
     rep.coarse_iters = m.cycles;
     rep.fine_rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    // rr is ||b - A x||^2 of the returned x, formed in FP64 by the last update pass
+    if (status != PRX_ERR_BREAKDOWN) rep.true_rel = rep.fine_rel;
     rep.status = status;
 }
 
 static void solve_mg(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {
     Timer total(t->stream);
+    rep.true_rel = -1.0;  // set by the solver when its last pass formed ||b - A x|| of the returned x
     if (c.o.precision == PRX_PREC_MIXED) mg_solve_mixed(t, c, c.cg_iters, rep);
     else mg_solve_fp64(t, c, c.cg_iters, rep);
     rep.ms_total = total.ms();
-    rep.true_rel = true_rel(t);
+    if (rep.true_rel < 0.0) rep.true_rel = true_rel(t);
 }
 
 static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep) {

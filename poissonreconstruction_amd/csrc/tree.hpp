@@ -117,6 +117,11 @@ struct prx_tree {
     DArr<float> e32, r32, ri32, d32, q32, s32;
     CgState* st = nullptr;
     CgState* st_host = nullptr;
+    CgState* st_snap = nullptr;  // [2] pinned: per-iteration state copies (PCG without host waits)
+    CgState* state_snapshots() {
+        if (!st_snap) PRX_HIP(hipHostMalloc((void**)&st_snap, 2 * sizeof(CgState), hipHostMallocDefault));
+        return st_snap;
+    }
 
     // coarse level (Morton order inside)
     uint32_t cd = 0;
@@ -139,6 +144,7 @@ struct prx_tree {
         free_list(eval_list);
         if (st) (void)hipFree(st);
         if (st_host) (void)hipHostFree(st_host);
+        if (st_snap) (void)hipHostFree(st_snap);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
