@@ -402,6 +402,17 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
             f"gs_block {rep['gs_block']}")
         runs.append((rep, wall))
     rep, wall = runs[-1]
+    # the same batched cascadic solve in mixed precision (FP32 correction CG, FP64 residual replacements)
+    mixed_runs = []
+    for i in range(2):
+        ts = time.perf_counter()
+        mrep, mper = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True,
+                             precision=L.PRX_PREC_MIXED)
+        mwall = (time.perf_counter() - ts) * 1e3
+        log(f"batch mixed solve {i}: device {mrep['ms_total']:.1f} ms wall {mwall:.1f} ms converged {mrep['converged']}/"
+            f"{mrep['instances']} max_true_rel {mrep['max_true_rel']:.3e} graph launches {mrep['graph_launches']}")
+        mixed_runs.append((mrep, mwall, mper))
+    mrep, mwall, mper = mixed_runs[-1]
     ts = time.perf_counter()
     seq_iters = 0
     for k in range(len(sizes)):
@@ -425,7 +436,15 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
            "ms_fine_cg": round(rep["ms_fine_cg"], 2),
            "autotuned": {"tile": rep["tile"], "chunk": rep["chunk"], "gs_block": rep["gs_block"],
                          "ms_tune": round(rep["ms_tune"], 1)},
-           "graph_launches": rep["graph_launches"]}
+           "graph_launches": rep["graph_launches"],
+           "mixed": {"ms_batched": round(mrep["ms_total"], 2), "ms_batched_wall": round(mwall, 2),
+                     "speedup_vs_fp64_batched": round(rep["ms_total"] / mrep["ms_total"], 3),
+                     "speedup_vs_sequential_fp64": round(seq_ms / mwall, 3),
+                     "converged": mrep["converged"], "max_true_rel": mrep["max_true_rel"],
+                     "max_fine_iters": mrep["max_fine_iters"], "sum_fine_iters": mrep["sum_fine_iters"],
+                     "ms_fine_gs": round(mrep["ms_fine_gs"], 2), "ms_fine_cg": round(mrep["ms_fine_cg"], 2),
+                     "max_outer": int(max(p["outer_iters"] for p in mper)),
+                     "graph_launches_cumulative": mrep["graph_launches"]}}
     # every instance by multigrid-preconditioned CG, over four streams
     mg = {}
     for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):

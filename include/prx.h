@@ -299,7 +299,11 @@ int prx_vjp_points(prx_tree* t, const double* dL_dF, double* dL_dpos, double* dL
  * instance; prx_solve_batched runs FEMTree::solve_cascadic
  * (src/fem_tree.rs:458-487, base depth D-2) for every instance over one
  * concatenated node array, with the CG iterations and GS sweeps replayed as
- * captured HIP graphs.  FP64 only.  With autotune != 0 the first solve times
+ * captured HIP graphs, in FP64 or (opts->precision = PRX_PREC_MIXED, fused
+ * CG step only) in mixed precision: FP32 smoothing and CG on each instance's
+ * correction with FP64 scalars, and batched FP64 passes that fold the
+ * correction into x and replace each instance's residual by its FP64 true
+ * residual, as the single-tree mixed solve does.  With autotune != 0 the first solve times
  * the operator brick size, the GS block size and the CG chunk length on the
  * batch and keeps the fastest (reported).  With opts->precond =
  * PRX_PRECOND_MG every instance is instead solved by multigrid-preconditioned

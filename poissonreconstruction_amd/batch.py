@@ -64,9 +64,10 @@ class Batch:
               refresh_period: int = 10, autotune: bool = True, precond: int = L.PRX_PRECOND_NONE,
               precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False):
         """Returns (batch report dict, list of per-instance report dicts).
-        precond PRX_PRECOND_MG solves every instance with multigrid-preconditioned
-        CG (mixed or FP64) instead of the batched cascadic solve (FP64);
-        cg_two_pass selects the two-pass CG iteration instead of the fused one."""
+        The batched cascadic solve runs in `precision` (FP64, or mixed with the
+        fused CG step); precond PRX_PRECOND_MG solves every instance with
+        multigrid-preconditioned CG (mixed or FP64) instead; cg_two_pass
+        selects the two-pass CG iteration instead of the fused one (FP64)."""
         o = L.default_solve_opts()
         o.precision = int(precision)
         o.residual_mode = int(residual_mode)

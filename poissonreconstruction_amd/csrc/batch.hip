@@ -65,6 +65,10 @@ struct BLevel {
     DArr<uint32_t> brick_inst;
     DArr<BCoef> coef;
     DArr<double> x, b, r, dd, q, sbuf;
+    // the mixed-precision fine solve: FP32 correction e, its CG vectors and
+    // screening values, and x2 (x + e formed by the FP64 residual update)
+    DArr<float> e32, r32, dd32, q32, sbuf32;
+    DArr<double> x2;
     NodeList scr;  // global-id screening list (fine level)
     bool has_scr = false;
     int64_t P = 0;
@@ -87,10 +91,21 @@ struct BLevel {
 
 struct GraphEntry {
     const BLevel* L;
-    int kind, chunk, refresh, tile, gsb;
+    int kind, chunk, refresh, tile, gsb, prec;
     bool fused;
     hipGraphExec_t exec;
 };
+
+// The vectors one precision's CG and smoothing run on over a level: FP64 on
+// the solution itself, FP32 on the correction e of the mixed solve.
+template <typename T>
+struct CgVecs {
+    T *x, *r, *q, *dd, *sbuf;
+};
+static CgVecs<double> vecs_of(BLevel& L, double*) { return {L.x.p, L.r.p, L.q.p, L.dd.p, L.sbuf.p}; }
+static CgVecs<float> vecs_of(BLevel& L, float*) { return {L.e32.p, L.r32.p, L.q32.p, L.dd32.p, L.sbuf32.p}; }
+template <typename T>
+static CgVecs<T> vecs(BLevel& L) { return vecs_of(L, (T*)nullptr); }
 
 }  // namespace prx
 
@@ -189,6 +204,23 @@ static void release_cg(BLevel& L) {
     L.r.release();
     L.dd.release();
     L.q.release();
+}
+// the mixed fine solve's vectors (the FP64 CG vectors of the level are not needed then)
+static void ensure_cg32(BLevel& L) {
+    L.e32.ensure(L.N);
+    L.r32.ensure(2 * L.N);
+    L.dd32.ensure(CG_XM * L.N);
+    L.q32.ensure(2 * L.N);
+    L.x2.ensure(L.N);
+    L.sbuf32.ensure(std::max<int64_t>(L.P, 1));
+}
+static void release_cg32(BLevel& L) {
+    L.e32.release();
+    L.r32.release();
+    L.dd32.release();
+    L.q32.release();
+    L.x2.release();
+    L.sbuf32.release();
 }
 
 // Concatenate the per-instance screening lists and point data (global node
@@ -327,18 +359,23 @@ static void resid_pass(prx_batch* B, BLevel& L, const BatchTab& tab, double* y =
 
 // Multicolour GS sweeps of every instance: one brick-tiled pass per colour,
 // the same pass (and arithmetic) as the single-tree sweep.
+// FP64: on x with right-hand side b; FP32: on the correction e with the
+// scaled residual in r slot 0.
+template <typename T>
 static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
     const BatchTab tab = tab_all(B, L);
-    StencilIO<double> io;
-    io.x = L.x.p;
-    io.y = L.x.p;
-    io.b = L.b.p;
-    io.s = L.sbuf.p;
+    const CgVecs<T> V = vecs<T>(L);
+    StencilIO<T> io;
+    io.x = V.x;
+    io.y = V.x;
+    if constexpr (sizeof(T) == 8) io.b = L.b.p;
+    else io.b = V.r;
+    io.s = V.sbuf;
     for (int k = 0; k < sweeps; ++k)
         for (int c = 0; c < 8; ++c) {
-            if (L.has_scr) launch_bscreen_gather<double>(tab, L.pts(), L.x.p, nullptr, nullptr, L.sbuf.p, nullptr, nullptr, B->stream);
+            if (L.has_scr) launch_bscreen_gather<T>(tab, L.pts(), V.x, nullptr, nullptr, V.sbuf, nullptr, nullptr, B->stream);
             io.color = c;
-            launch_bstencil<double>(tab, L.N, L.scr_ptr(), OP_GS, io, B->gs_block, B->stream);
+            launch_bstencil<T>(tab, L.N, L.scr_ptr(), OP_GS, io, B->gs_block, B->stream);
         }
 }
 
@@ -347,41 +384,47 @@ static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
 // ring slot k % CG_XM and writes slot (k+1)&1 and ring slot (k+1) % CG_XM.
 // xn: the x terms pending before the pass (the same for every running
 // instance: they start or restart together at pass 0).
+template <typename T>
 static void cg_enqueue_fused(prx_batch* B, BLevel& L, uint64_t k, int refresh, int& xn) {
     const hipStream_t s = B->stream;
     const BatchTab tab = tab_running(B, L);
     const int64_t N = L.N;
-    auto slot = [&](uint64_t j) { return L.dd.p + (int64_t)(j % CG_XM) * N; };
-    double* dold = slot(k);
-    double* dnew = slot(k + 1);
+    const CgVecs<T> V = vecs<T>(L);
+    auto slot = [&](uint64_t j) { return V.dd + (int64_t)(j % CG_XM) * N; };
+    T* dold = slot(k);
+    T* dnew = slot(k + 1);
     if (xn == CG_XM) {  // the instances also check their own count
-        XDirs<double> d{};
+        XDirs<T> d{};
         for (int i = 0; i < CG_XM; ++i) d.d[i] = slot(k + 1 + (uint64_t)i);
-        launch_bx_terms<double>(tab, L.x.p, d, L.dd.p, N, 0, s);
+        launch_bx_terms<T>(tab, V.x, d, V.dd, N, 0, s);
     }
-    StencilIO<double> io;
-    io.r = L.r.p + (int64_t)(k & 1) * N;
-    io.q = L.q.p + (int64_t)(k & 1) * N;
-    io.rout = L.r.p + (int64_t)((k + 1) & 1) * N;
+    StencilIO<T> io;
+    io.r = V.r + (int64_t)(k & 1) * N;
+    io.q = V.q + (int64_t)(k & 1) * N;
+    io.rout = V.r + (int64_t)((k + 1) & 1) * N;
     io.dold = dold;
     io.dnew = dnew;
-    io.y = L.q.p + (int64_t)((k + 1) & 1) * N;
+    io.y = V.q + (int64_t)((k + 1) & 1) * N;
     io.partial = B->partial.p;
     if (L.has_scr) {
-        launch_bscreen_gather<double>(tab, L.pts(), nullptr, nullptr, nullptr, L.sbuf.p, io.r, dold, s, io.q);
-        io.s = L.sbuf.p;
+        launch_bscreen_gather<T>(tab, L.pts(), nullptr, nullptr, nullptr, V.sbuf, io.r, dold, s, io.q);
+        io.s = V.sbuf;
     }
-    launch_bstencil<double>(tab, N, L.scr_ptr(), OP_CG, io, B->tile, s);
+    launch_bstencil<T>(tab, N, L.scr_ptr(), OP_CG, io, B->tile, s);
     const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
     launch_bfin_cg(tab, B->partial.p, ulog_tile(B->tile), B->fold.p, fresh ? 1 : 0, s);
     xn = xn == CG_XM ? 1 : xn + 1;
     if (fresh) {  // x completed, then r = b - A x replaces the pending update
-        XDirs<double> d{};
-        for (int i = 0; i < xn; ++i) d.d[i] = slot(k + 2 - (uint64_t)xn + (uint64_t)i);
-        launch_bx_terms<double>(tab, L.x.p, d, L.dd.p, N, 2, s);
-        resid_pass(B, L, tab, io.rout);
-        launch_bfin(tab, B->partial.p, ulog_tile(B->tile), FIN_FRESH, 0, s);
-        xn = 0;
+        if constexpr (sizeof(T) == 8) {
+            XDirs<double> d{};
+            for (int i = 0; i < xn; ++i) d.d[i] = slot(k + 2 - (uint64_t)xn + (uint64_t)i);
+            launch_bx_terms<double>(tab, L.x.p, d, L.dd.p, N, 2, s);
+            resid_pass(B, L, tab, io.rout);
+            launch_bfin(tab, B->partial.p, ulog_tile(B->tile), FIN_FRESH, 0, s);
+            xn = 0;
+        } else {
+            throw Error(PRX_ERR_INVALID, "the mixed solve replaces its residual by the FP64 one, not by a refresh");
+        }
     }
 }
 
@@ -415,26 +458,33 @@ static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
     launch_bfin(tab, B->partial.p, ulog, fop, (int)((k + 1) & 1), s);
 }
 
-// kind 0: `chunk` CG iterations starting at an even index; kind 1: GS sweeps
-static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int refresh) {
+// kind 0: `chunk` CG iterations starting at an even index; kind 1: GS
+// sweeps.  prec 1: the FP32 correction system of the mixed fine solve.
+static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int refresh, int prec = 0) {
     for (auto& g : B->graphs)
         if (g.L == &L && g.kind == kind && g.chunk == n && g.refresh == refresh && g.tile == B->tile && g.gsb == B->gs_block &&
-            g.fused == B->fused)
+            g.fused == B->fused && g.prec == prec)
             return g.exec;
     hipGraph_t graph;
     PRX_HIP(hipStreamSynchronize(B->stream));
     PRX_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
     try {
-        if (kind == 0 && B->fused) {
+        if (kind == 0 && prec == 1) {
+            int xn = CG_XM;
+            for (int k = 0; k < n; ++k) cg_enqueue_fused<float>(B, L, (uint64_t)k, 0, xn);
+        } else if (kind == 0 && B->fused) {
             // a replay starts with CG_XM terms pending (none at the very first
             // pass, which the kernels check), or none after a refresh that
             // ended the previous replay
             int xn = refresh > 0 ? 0 : CG_XM;
-            for (int k = 0; k < n; ++k) cg_enqueue_fused(B, L, (uint64_t)k, refresh, xn);
-        } else if (kind == 0)
+            for (int k = 0; k < n; ++k) cg_enqueue_fused<double>(B, L, (uint64_t)k, refresh, xn);
+        } else if (kind == 0) {
             for (int k = 0; k < n; ++k) cg_enqueue(B, L, (uint64_t)k, refresh);
-        else
-            gs_enqueue(B, L, n);
+        } else if (prec == 1) {
+            gs_enqueue<float>(B, L, n);
+        } else {
+            gs_enqueue<double>(B, L, n);
+        }
     } catch (...) {
         hipGraph_t g2;
         (void)hipStreamEndCapture(B->stream, &g2);
@@ -446,7 +496,7 @@ static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int re
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PRX_HIP(e);
-    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, B->fused, exec});
+    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, prec, B->fused, exec});
     return exec;
 }
 
@@ -586,6 +636,194 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
     return o;
 }
 
+// x2 = x + e, r32 = (b - A x2) / h (the instance's 1/h) in r slot 0, partial
+// |b - A x2|^2 per brick in FP64, for the instances selected by `tab`: the
+// single-tree residual update pass (OP_RESID_UPD) over the batch.
+static void resid_update_pass(prx_batch* B, BLevel& F, const BatchTab& tab) {
+    StencilIO<double> io;
+    io.x64 = F.x.p;
+    io.e32 = F.e32.p;
+    io.x_out = F.x2.p;
+    io.b = F.b.p;
+    io.r32 = F.r32.p;
+    io.partial = B->partial.p;
+    if (F.has_scr) {
+        launch_bscreen_gather<double>(tab, F.pts(), nullptr, F.x.p, F.e32.p, F.sbuf.p, nullptr, nullptr, B->stream);
+        io.s = F.sbuf.p;
+    }
+    launch_bstencil<double>(tab, F.N, F.scr_ptr(), OP_RESID_UPD, io, B->tile, B->stream);
+}
+
+struct MixedOut {
+    BOut cg;
+    std::vector<uint64_t> outer;
+    std::vector<double> true_rel;
+    double ms_gs = 0.0;
+};
+
+// The fine level of every instance in mixed precision, as the single-tree
+// mixed solve runs it (the fine half of FEMTree::solve_cascadic,
+// src/fem_tree.rs:485-486): Gauss-Seidel and CG on the FP32 correction e of
+// the system scaled by 1/h, with FP64 scalars and reductions.  Whenever an
+// instance's FP32 residual has fallen by theta since its last replacement
+// (or its CG stops), one batched FP64 pass folds x += e and forms the true
+// residual b - A x, which replaces the FP32 one while CG continues along its
+// direction (a reliable update); an instance stops on its FP64 true residual
+// ||b - A x|| <= eps ||b||.
+static MixedOut batch_fine_mixed(prx_batch* B, BLevel& F, int gs, uint64_t max_iters, double eps, int mode, double theta,
+                                 uint64_t max_outer) {
+    const hipStream_t s = B->stream;
+    const int K = F.K;
+    const int64_t N = F.N;
+    const BatchTab all = tab_all(B, F);
+    MixedOut out;
+    out.outer.assign(K, 0);
+    out.true_rel.assign(K, 0.0);
+    // ||b||^2 per instance
+    for (int k = 0; k < K; ++k) B->st_host[k] = CgState{};
+    write_states(B);
+    int ul = launch_bdot<double>(all, F.b.p, F.b.p, N, B->partial.p, s);
+    launch_bfin(all, B->partial.p, ul, FIN_SLOT, 0, s);
+    read_states(B);
+    std::vector<double> bb(K), thr(K), thr32(K), rr(K), r0(K);
+    for (int k = 0; k < K; ++k) {
+        const double ih = (double)F.coef_h[k].res;
+        bb[k] = B->st_host[k].slot[0];
+        thr[k] = (bb[k] * eps) * eps;
+        thr32[k] = thr[k] * ih * ih;
+    }
+    // smoothing on the correction: e = GS^gs(0; A/h, (b - A x)/h), then x += e
+    {
+        Timer tg(s);
+        launch_fill<float>(F.e32.p, 0.0f, N, s);
+        resid_update_pass(B, F, all);
+        launch_bcommit(all, F.x.p, F.x2.p, F.e32.p, N, s);
+        if (gs > 0) launch_graph(B, graph_for(B, F, 1, gs, 0, 1));
+        out.ms_gs = tg.ms();
+    }
+    resid_update_pass(B, F, all);
+    launch_bfin(all, B->partial.p, ulog_tile(B->tile), FIN_SLOT, 1, s);
+    launch_bcommit(all, F.x.p, F.x2.p, F.e32.p, N, s);
+    read_states(B);
+    for (int k = 0; k < K; ++k) {
+        rr[k] = r0[k] = B->st_host[k].slot[1];
+        if (mode != PRX_RESID_TRUE) {  // the reference's test: relative to the residual after smoothing
+            const double ih = (double)F.coef_h[k].res;
+            thr[k] = (rr[k] * eps) * eps;
+            thr32[k] = thr[k] * ih * ih;
+        }
+    }
+    // each instance's CG starts from its scaled residual (ring slot 0 = d_-1 = 0)
+    for (int k = 0; k < K; ++k) {
+        CgState h{};
+        h.eps = 1.0;
+        h.max_iters = (int64_t)max_iters;
+        h.lazy = 1;
+        B->st_host[k] = h;
+    }
+    write_states(B);
+    ul = launch_bdot<float>(all, F.r32.p, F.r32.p, N, B->partial.p, s);
+    launch_bfin(all, B->partial.p, ul, FIN_INIT_REF, 0, s);
+    launch_fill<float>(F.dd32.p, 0.0f, N, s);
+    read_states(B);
+    std::vector<char> fin(K, 0);
+    std::vector<int> status(K, PRX_OK);
+    for (int k = 0; k < K; ++k) {
+        CgState& h = B->st_host[k];
+        if (!(rr[k] > thr[k]) || rr[k] == 0.0) {
+            fin[k] = 1;
+            h.done = 1;
+            continue;
+        }
+        h.beta = 0.0;
+        h.thr = std::max(theta * theta * h.delta, thr32[k]);
+        h.done = 0;
+        h.status = 0;
+        h.rpend = 0;
+        h.xn = 0;
+    }
+    write_states(B);
+    int chunk = B->chunk;
+    chunk = std::max(CG_XM, (chunk + CG_XM / 2) / CG_XM * CG_XM);  // replays keep pass index = ring position
+    const hipGraphExec_t g = graph_for(B, F, 0, chunk, 0, 1);
+    for (;;) {
+        for (;;) {
+            bool all_done = true;
+            for (int k = 0; k < K; ++k)
+                if (!B->st_host[k].done) { all_done = false; break; }
+            if (all_done) break;
+            launch_graph(B, g);
+            read_states(B);
+        }
+        // complete e: add the pending terms
+        launch_bx_terms<float>(all, F.e32.p, XDirs<float>{}, F.dd32.p, N, 1, s);
+        bool any = false;
+        for (int k = 0; k < K; ++k) {
+            B->st_host[k].xpend = 0;
+            B->st_host[k].xn = 0;
+            B->flags_h[k] = fin[k] ? 0 : 1;
+            any = any || !fin[k];
+        }
+        write_states(B);
+        if (!any) break;
+        // fold e into x and replace each residual by the FP64 true one
+        write_flags(B);
+        const BatchTab fl = tab_flagged(B, F);
+        resid_update_pass(B, F, fl);
+        launch_bfin(fl, B->partial.p, ulog_tile(B->tile), FIN_SLOT, 1, s);
+        launch_bcommit(fl, F.x.p, F.x2.p, F.e32.p, N, s);
+        ul = launch_bdot<float>(fl, F.r32.p, F.r32.p, N, B->partial.p, s);
+        launch_bfin(fl, B->partial.p, ul, FIN_SLOT, 2, s);
+        read_states(B);
+        for (int k = 0; k < K; ++k) {
+            if (!B->flags_h[k]) continue;
+            CgState& h = B->st_host[k];
+            const double rt = h.slot[1], dt = h.slot[2];
+            rr[k] = rt;
+            out.outer[k]++;
+            if (rt <= thr[k]) { fin[k] = 1; h.done = 1; status[k] = PRX_OK; continue; }
+            if (h.status == PRX_ERR_BREAKDOWN) { fin[k] = 1; h.done = 1; status[k] = PRX_ERR_BREAKDOWN; continue; }
+            if (h.status == PRX_ERR_NOT_CONVERGED || (uint64_t)h.it >= max_iters || out.outer[k] >= max_outer) {
+                fin[k] = 1;
+                h.done = 1;
+                status[k] = PRX_ERR_NOT_CONVERGED;
+                continue;
+            }
+            const bool drifted = !(dt <= 4.0 * h.delta);
+            h.beta = (!drifted && h.delta_prev > 0.0) ? dt / h.delta_prev : 0.0;
+            h.delta = dt;
+            h.thr = std::max(theta * theta * dt, thr32[k]);
+            h.done = 0;
+            h.status = 0;
+            h.rpend = 0;  // the replacement is the residual of the next pass
+            // the graph restarts at pass 0, which reads r from slot 0 (just
+            // written) and the direction from ring slot 0: move d_{it-1} there
+            const int sl = (int)(((h.it - (int64_t)h.ring0) % CG_XM + CG_XM) % CG_XM);
+            const int64_t nk = (int64_t)(F.off_h[k + 1] - F.off_h[k]);
+            if (sl != 0)
+                PRX_HIP(hipMemcpyAsync(F.dd32.p + F.off_h[k], F.dd32.p + (int64_t)sl * N + F.off_h[k], sizeof(float) * nk,
+                                       hipMemcpyDeviceToDevice, s));
+            h.ring0 = (int32_t)(h.it % CG_XM);
+        }
+        write_states(B);
+        bool running = false;
+        for (int k = 0; k < K; ++k) running = running || !fin[k];
+        if (!running) break;
+    }
+    out.cg.iters.resize(K);
+    out.cg.rel.resize(K);
+    out.cg.status.resize(K);
+    for (int k = 0; k < K; ++k) {
+        out.cg.iters[k] = (uint64_t)B->st_host[k].it;
+        out.cg.status[k] = status[k];
+        out.true_rel[k] = bb[k] > 0.0 ? std::sqrt(rr[k] / bb[k]) : 0.0;
+        const double den = mode == PRX_RESID_TRUE ? bb[k] : r0[k];
+        out.cg.rel[k] = den > 0.0 ? std::sqrt(rr[k] / den) : 0.0;
+    }
+    PRX_HIP(hipStreamSynchronize(s));
+    return out;
+}
+
 // ------------------------------------------------------------------ autotuner
 static double time_ms(hipStream_t s, int reps, const std::function<void()>& f) {
     hipEvent_t a, b;
@@ -675,6 +913,7 @@ static void solve_batched_mg(prx_batch* B, uint64_t cg_iters, double eps, const 
     B->drop_graphs();
     release_cg(B->fine);
     release_cg(B->coarse);
+    release_cg32(B->fine);
     std::vector<int> order(K);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return B->inst[a]->N > B->inst[b]->N; });
@@ -749,7 +988,9 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         solve_batched_mg(B, cg_iters, eps, o, rep, per);
         return;
     }
-    if (o.precision != PRX_PREC_FP64) throw Error(PRX_ERR_INVALID, "the batched cascadic solve runs in FP64");
+    const bool mixed = o.precision == PRX_PREC_MIXED;
+    if (o.precision != PRX_PREC_FP64 && !mixed) throw Error(PRX_ERR_INVALID, "precision must be PRX_PREC_FP64 or PRX_PREC_MIXED");
+    if (mixed && o.cg_two_pass) throw Error(PRX_ERR_INVALID, "the batched mixed solve runs the fused CG step (cg_two_pass = 0)");
     // the instance trees' own solver workspace (left by multigrid or single
     // solves) is not used here; return it before the concatenated levels grow
     PRX_HIP(hipStreamSynchronize(B->stream));
@@ -758,14 +999,26 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         t->release_workspace();
     }
     build(B);
-    if (!B->fine.r.p || !B->coarse.r.p) B->drop_graphs();  // captured graphs hold the old vector addresses
-    ensure_cg(B->fine);
-    ensure_cg(B->coarse);
     B->fused = o.cg_two_pass == 0;
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;
     BLevel& C = B->coarse;
     const int K = F.K;
+    // the coarse level runs the FP64 CG; the fine level the FP64 CG, or the
+    // FP32 correction vectors of the mixed solve (the autotuner times FP64
+    // passes, so a first mixed solve with tuning holds both for a moment).
+    // Captured graphs hold vector addresses: any change drops them.
+    auto fine_vectors = [&](bool want64, bool want32) {
+        bool changed = false;
+        if (want64 != (F.r.p != nullptr)) { want64 ? ensure_cg(F) : release_cg(F); changed = true; }
+        if (want32 != (F.e32.p != nullptr)) { want32 ? ensure_cg32(F) : release_cg32(F); changed = true; }
+        if (changed) B->drop_graphs();
+    };
+    if (!C.r.p) {
+        B->drop_graphs();
+        ensure_cg(C);
+    }
+    fine_vectors(!mixed || (tune && !B->tuned), false);
     // right-hand sides: fine b from each tree, coarse RHS from its normal field
     for (int k = 0; k < K; ++k) {
         prx_tree* t = B->inst[k];
@@ -781,6 +1034,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         B->gs_block = 16;
         B->chunk = 16;
     }
+    if (mixed) fine_vectors(false, true);
     const int mode = o.residual_mode == PRX_RESID_TRUE ? PRX_RESID_TRUE : PRX_RESID_REFERENCE;
     const int refresh = mode == PRX_RESID_TRUE ? 0 : (o.refresh_period > 0 ? o.refresh_period : 0);
     Timer total(s);
@@ -806,12 +1060,21 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         }
         ms_coarse = tm.ms();
     }
-    {
+    std::vector<uint64_t> outer(K, 1);
+    if (mixed) {
         Timer tm(s);
-        if (gs > 0) launch_graph(B, graph_for(B, F, 1, (int)gs, 0));
-        ms_gs = tm.ms();
-    }
-    {
+        const double theta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.05;
+        MixedOut mo = batch_fine_mixed(B, F, (int)gs, cg_iters / 4, eps, mode, theta, o.max_outer > 0 ? o.max_outer : 64);
+        ms_gs = mo.ms_gs;
+        ms_cg = tm.ms() - mo.ms_gs;
+        fo = mo.cg;
+        outer = mo.outer;
+    } else {
+        {
+            Timer tm(s);
+            if (gs > 0) launch_graph(B, graph_for(B, F, 1, (int)gs, 0));
+            ms_gs = tm.ms();
+        }
         Timer tm(s);
         fo = batch_cg(B, F, cg_iters / 4, eps, mode, refresh);
         ms_cg = tm.ms();
@@ -821,7 +1084,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     const BatchTab all = tab_all(B, F);
     int ul = launch_bdot<double>(all, F.b.p, F.b.p, F.N, B->partial.p, s);
     launch_bfin(all, B->partial.p, ul, FIN_SLOT, 2, s);
-    resid_pass(B, F, all);
+    resid_pass(B, F, all, mixed ? F.x2.p : nullptr);
     launch_bfin(all, B->partial.p, ulog_tile(B->tile), FIN_SLOT, 3, s);
     read_states(B);
     for (int k = 0; k < K; ++k) {
@@ -859,7 +1122,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
             p.coarse_rel = co.rel[k];
             p.fine_iters = fo.iters[k];
             p.fine_rel = fo.rel[k];
-            p.outer_iters = 1;
+            p.outer_iters = outer[k];
             p.true_rel = tr;
             p.status = fo.status[k];
             p.ms_total = ms_total;

@@ -524,6 +524,26 @@ void launch_bx_terms(const BatchTab& tab, T* x, const XDirs<T>& d, const T* ring
     PRX_LAUNCH_CHECK();
 }
 template void launch_bx_terms<double>(const BatchTab&, double*, const XDirs<double>&, const double*, int64_t, int, hipStream_t);
+template void launch_bx_terms<float>(const BatchTab&, float*, const XDirs<float>&, const float*, int64_t, int, hipStream_t);
+
+// x = x2 and e = 0 on the nodes of the running instances: the fold of a
+// batched residual-update pass, which wrote x + e into x2 (a block covers
+// 1024 nodes inside one 4096-node brick, so it belongs to one instance).
+__global__ __launch_bounds__(256) void k_bcommit(BatchTab tab, double* __restrict__ x, const double* __restrict__ x2, float* __restrict__ e) {
+    typedef double nd2 __attribute__((ext_vector_type(2)));
+    typedef float nf4 __attribute__((ext_vector_type(4)));
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    const uint32_t k = tab.brick_inst[((uint64_t)blockIdx.x * 1024) >> 12];
+    if (!b_runs(tab, k)) return;
+    const nd2 a = *(const nd2*)(x2 + j), b = *(const nd2*)(x2 + j + 2);
+    __builtin_nontemporal_store(a, (nd2*)(x + j));
+    __builtin_nontemporal_store(b, (nd2*)(x + j + 2));
+    __builtin_nontemporal_store(nf4{0.f, 0.f, 0.f, 0.f}, (nf4*)(e + j));
+}
+void launch_bcommit(const BatchTab& tab, double* x, const double* x2, float* e, int64_t Ntot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bcommit, dim3((unsigned)(Ntot / 1024)), dim3(256), 0, s, tab, x, x2, e);
+    PRX_LAUNCH_CHECK();
+}
 
 // r -= alpha_k q ; partial = r.r per block ; x as in k_cg_update (eager, or
 // lazy in pairs).  A block covers 1024 vectors (4096 float or 2048 double

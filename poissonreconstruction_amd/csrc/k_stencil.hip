@@ -267,6 +267,7 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         res = c.res;
         if constexpr (sizeof(T) == 8) { L.s = (T)c.s64; L.wg = (T)c.wg64; L.wh3 = (T)c.wh3_64; }
         else { L.s = (T)c.s32; L.wg = (T)c.wg32; L.wh3 = (T)c.wh3_32; }
+        if constexpr (OP == OP_RESID_UPD) io.rscale = (double)c.res;  // 1/h of the instance (FP32 system scale)
         ib = tab.off[inst];
         lb = bid - (uint32_t)(ib / B3);
         if constexpr (OP == OP_DIR_APPLY) bt = tab.st[inst].beta;
@@ -963,14 +964,20 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(16, OP_RESID);
         else if (op == OP_GS) PRX_BTILE_LAUNCH(16, OP_GS);
         else if (op == OP_CG) PRX_BTILE_LAUNCH(16, OP_CG);
-        else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
+        else if (op == OP_RESID_UPD) {
+            if constexpr (sizeof(T) == 8) PRX_BTILE_LAUNCH(16, OP_RESID_UPD);
+            else throw Error(PRX_ERR_INVALID, "the residual update pass is FP64");
+        } else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     } else {
         if (op == OP_APPLY) PRX_BTILE_LAUNCH(8, OP_APPLY);
         else if (op == OP_DIR_APPLY) PRX_BTILE_LAUNCH(8, OP_DIR_APPLY);
         else if (op == OP_RESID) PRX_BTILE_LAUNCH(8, OP_RESID);
         else if (op == OP_GS) PRX_BTILE_LAUNCH(8, OP_GS);
         else if (op == OP_CG) PRX_BTILE_LAUNCH(8, OP_CG);
-        else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
+        else if (op == OP_RESID_UPD) {
+            if constexpr (sizeof(T) == 8) PRX_BTILE_LAUNCH(8, OP_RESID_UPD);
+            else throw Error(PRX_ERR_INVALID, "the residual update pass is FP64");
+        } else throw Error(PRX_ERR_INVALID, "batched operator pass not available");
     }
 #undef PRX_BTILE_LAUNCH
     PRX_LAUNCH_CHECK();

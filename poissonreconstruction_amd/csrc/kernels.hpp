@@ -261,6 +261,8 @@ void launch_baxpy_alpha(const BatchTab& tab, T* x, const T* dprev, const T* d, i
 // per instance: x += alpha_pend * d{dir_buf} where a lazy term is pending
 template <typename T>
 void launch_bflush_x(const BatchTab& tab, T* x, const T* d0, const T* d1, int64_t Ntot, hipStream_t s);
+// x = x2, e = 0 on the running instances (after a batched OP_RESID_UPD pass into x2)
+void launch_bcommit(const BatchTab& tab, double* x, const double* x2, float* e, int64_t Ntot, hipStream_t s);
 // partial = a.b per (256 * 16 / sizeof(T)) nodes; returns log2 of that unit
 template <typename T>
 int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s);

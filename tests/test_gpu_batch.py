@@ -95,8 +95,48 @@ def test_batch_rejects_what_it_does_not_do(gpu):
     from poissonreconstruction_amd import _lib as L
     o = L.default_solve_opts()
     o.precision = L.PRX_PREC_MIXED
+    o.cg_two_pass = 1  # the batched mixed solve runs the fused CG step only
     rc = L.lib().prx_solve_batched(B._h, 20, 1000, 1e-6, ctypes.byref(o), 0, None, None)
     assert rc == L.PRX_ERR_INVALID
+
+
+def single_mixed(D, pts, pw, eps):
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    rep = t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=pr.PRX_PREC_MIXED, residual_mode=pr.PRX_RESID_TRUE,
+                           refresh_period=0)
+    return t.solution, rep
+
+
+@pytest.mark.parametrize("autotune", [False, True])
+def test_batch_mixed_matches_single_tree_mixed_solves(gpu, autotune):
+    # the batched mixed cascadic solve (FP32 correction CG with FP64 residual
+    # replacements per instance) against each cloud solved alone by the
+    # single-tree mixed solve: both reach the FP64 true residual, and the
+    # solutions agree to the accuracy that residual fixes
+    cs = clouds()
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    eps = 1e-9
+    for rnd in range(2):  # the second solve replays the captured graphs
+        rep, per = B.solve(eps=eps, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=autotune,
+                           precision=pr.PRX_PREC_MIXED)
+        assert rep["instances"] == len(cs) and rep["converged"] == len(cs) and rep["status"] == pr.PRX_OK
+        assert rep["max_true_rel"] <= eps * 1.0001
+        for k, (D, pts, pw) in enumerate(cs):
+            xs, rs = single_mixed(D, pts, pw, eps)
+            xb = B.tree(k).solution
+            rng_f = xs.max() - xs.min()
+            print(f"  round {rnd} instance {k} depth {D}: batched {per[k]['fine_iters']} iterations "
+                  f"({per[k]['outer_iters']} replacements), alone {rs['fine_iters']} ({rs['outer_iters']}); "
+                  f"max diff {np.abs(xb - xs).max() / rng_f:.2e} of range", flush=True)
+            assert per[k]["true_rel"] <= eps * 1.0001 and per[k]["status"] == pr.PRX_OK
+            assert np.abs(xb - xs).max() <= 1e-6 * rng_f, (rnd, k)
+            assert abs(per[k]["fine_iters"] - rs["fine_iters"]) <= max(4, 0.1 * rs["fine_iters"]), (rnd, k)
+    # and the FP64 batched solve after the mixed one (vectors swapped back)
+    rep, per = B.solve(eps=1e-10, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False)
+    assert rep["converged"] == len(cs) and rep["max_true_rel"] <= 1e-10 * 1.0001
 
 
 @pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])

@@ -96,6 +96,12 @@ def test_batching_workload_256_superquadrics(gpu):
         print(f"  multigrid precision {prec}: converged {rep['converged']}/{rep['instances']} "
               f"max_true_rel {rep['max_true_rel']:.3e} {rep['ms_total']:.1f} ms", flush=True)
         assert rep["instances"] == 256 and rep["converged"] == 256 and rep["max_true_rel"] <= 1e-6
+    mrep, mper = B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=True,
+                         precision=pr.PRX_PREC_MIXED)
+    print(f"  cascadic mixed batched: converged {mrep['converged']}/{mrep['instances']} max_true_rel "
+          f"{mrep['max_true_rel']:.3e} {mrep['ms_total']:.1f} ms", flush=True)
+    assert mrep["converged"] == 256 and mrep["max_true_rel"] <= 1e-6
+    mixed_x = {d: B.tree(depths.index(d)).solution for d in (6, 7, 8)}
     rep, per = B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=True)
     print(f"  cascadic FP64 batched: converged {rep['converged']}/{rep['instances']} max_true_rel "
           f"{rep['max_true_rel']:.3e} {rep['ms_total']:.1f} ms, {rep['graph_launches']} graph launches", flush=True)
@@ -110,6 +116,8 @@ def test_batching_workload_256_superquadrics(gpu):
         xs, xb = t.solution, B.tree(k).solution
         assert np.abs(xb - xs).max() <= 1e-9 * np.abs(xs).max(), (d, k)
         assert abs(per[k]["fine_iters"] - rs["fine_iters"]) <= 2
+        # the mixed batched instance solved the same system to the same residual
+        assert np.abs(mixed_x[d] - xs).max() <= 1e-4 * (xs.max() - xs.min()), (d, k)
         t.close()
     B.close()
 
