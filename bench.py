@@ -505,6 +505,13 @@ def operator_roofline(L, tree, N, P, reps):
     copy_bytes = 2 * 4 * N
     L.check(L.lib().prx_copy_bandwidth(copy_bytes, reps, 0, ctypes.byref(cms)))
     copy_gbs = copy_bytes / (cms.value * 1e-3) / 1e9
+    # the fused step's access pattern without arithmetic: 3 vectors in, 3 out, one brick per workgroup
+    pattern = {}
+    for prec, name, esz in ((L.PRX_PREC_MIXED, "fp32", 4), (L.PRX_PREC_FP64, "fp64", 8)):
+        for streams in (1, 3):
+            bms = ctypes.c_double(0)
+            L.check(L.lib().prx_brick_copy_bandwidth(N // 4096, streams, prec, reps, 0, ctypes.byref(bms)))
+            pattern[f"{name}_{streams}x{streams}_GBs"] = round(2 * streams * N * esz / (bms.value * 1e-3) / 1e9, 1)
     cg_bytes, dir_bytes, apply_bytes = 24 * N + 12 * P, 16 * N + 12 * P, 8 * N + 12 * P
     achieved = cg_bytes / (kc * 1e-3) / 1e9
     def measured_traffic(stem):
@@ -524,8 +531,12 @@ def operator_roofline(L, tree, N, P, reps):
                 "kernel": "k_stencil_tile<float,16,OP_CG>", "bytes_per_launch": cg_bytes,
                 "ms_per_launch": round(kc, 4), "gather_ms": round(gc, 4),
                 "frac_of_copy": round(achieved / copy_gbs, 4),
+                "brick_copy": pattern,
+                "frac_of_pattern_ceiling": round(achieved / pattern["fp32_3x3_GBs"], 4),
                 "cg_step_fp64": dict(sub("k_stencil_tile<double,16,OP_CG>", 48 * N + 24 * P, kc64, gc64),
-                                     traffic=measured_traffic("stencil_cg_step_fp64")),
+                                     traffic=measured_traffic("stencil_cg_step_fp64"),
+                                     frac_of_pattern_ceiling=round((48 * N + 24 * P) / (kc64 * 1e-3) / 1e9
+                                                                   / pattern["fp64_3x3_GBs"], 4)),
                 "dir_apply_fp32": sub("k_stencil_tile<float,16,OP_DIR_APPLY>", dir_bytes, kd, gd),
                 "dir_apply_fp64": sub("k_stencil_tile<double,16,OP_DIR_APPLY>", 32 * N + 24 * P, kd64, gd64),
                 "apply_fp32": sub("k_stencil_tile<float,16,OP_APPLY>", apply_bytes, ka, ga),

@@ -170,6 +170,10 @@ int prx_tree_export_nodes(prx_tree* t, uint32_t* idx_to_offset, int64_t* grid_to
  * (src/fem_tree.rs:130-178).  Builds the per-node point lists used by
  * assembly and the operator. */
 int prx_splat_normal_field(prx_tree* t, const prx_point* unit_points, size_t n);
+/* The same with the points already in device memory of the tree's device:
+ * pos_device and nrm_device are [n][3] (x, y, z per point), as a torch
+ * tensor of shape (n, 3) holds them.  Used by the torch path. */
+int prx_splat_normal_field_device(prx_tree* t, const double* pos_device, const double* nrm_device, size_t n);
 /* normal_field [3N, AoS x,y,z] and node_weight [N] (src/fem_tree.rs:60-61). */
 int prx_export_normal_field(prx_tree* t, double* normal_field, double* node_weight);
 
@@ -220,6 +224,8 @@ int prx_set_solution(prx_tree* t, const double* x);
 
 /* FEMTree::evaluate (src/fem_tree.rs:490-513) at m query points q[3m]. */
 int prx_evaluate(prx_tree* t, const double* q, size_t m, double* out);
+/* The same with q [m][3] and out [m] in device memory (the torch path). */
+int prx_evaluate_device(prx_tree* t, const double* q_device, size_t m, double* out_device);
 /* evaluate at every marching-cubes corner k/res, k in [0,res]^3
  * (src/fem_tree.rs:596-605), x fastest: out[(kz*(res+1)+ky)*(res+1)+kx]. */
 int prx_evaluate_corners(prx_tree* t, double* out);
@@ -259,6 +265,14 @@ int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, d
  * each with its screening gather. */
 int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
+/* Measurement: the tile kernels' memory access pattern without arithmetic.
+ * Each 256-thread workgroup copies one 4096-node brick (FP32 or FP64 by
+ * `precision`) from each of `streams` (1..3) input vectors to as many
+ * output vectors, workgroups mapped to the XCDs as the tile kernels map them;
+ * *ms = average ms per launch over nbricks bricks (a multiple of 8).  The
+ * fused CG step reads and writes three vectors this way, so the 3-stream
+ * rate is the ceiling its access pattern allows. */
+int prx_brick_copy_bandwidth(int64_t nbricks, int streams, int precision, int reps, int device, double* ms);
 /* Measurement: the phases of the last prx_splat_normal_field, ms[5] =
  * {host packing and upload of the points, spatial sort of the points, the
  * splat's per-node lists, the screening (truncated-cell) per-node lists, the
@@ -290,6 +304,10 @@ typedef struct prx_vjp_report {
 } prx_vjp_report;
 int prx_vjp_points(prx_tree* t, const double* dL_dF, double* dL_dpos, double* dL_dnrm, int precision, double eps,
                    prx_vjp_report* report);
+/* The same with dL_dF [P], dL_dpos [3P] and dL_dnrm [3P] in device memory
+ * (the torch path). */
+int prx_vjp_points_device(prx_tree* t, const double* dL_dF_device, double* dL_dpos_device, double* dL_dnrm_device,
+                          int precision, double eps, prx_vjp_report* report);
 
 /* ------------------------------------------------------------ batches
  * No counterpart in the reference (it reconstructs one cloud per call,

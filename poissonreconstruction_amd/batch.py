@@ -25,6 +25,7 @@ class _BatchTree(FEMTree):
         self.max_depth = int(depth)
         self._h = handle
         self._points = points
+        self._n_points = len(points)
         self._pw = float(pw)
 
     def close(self):

@@ -993,6 +993,32 @@ void upload_points(prx_tree* t, const prx_point* pts, size_t n) {
     t->setup_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
 }
 
+// positions and normals already on the device, each [n][3] (x, y, z per
+// point): transposed into the tree's SoA arrays on the device
+__global__ void k_aos3_to_soa(const double* __restrict__ a, int64_t n, double* __restrict__ soa) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        soa[i] = a[3 * i];
+        soa[n + i] = a[3 * i + 1];
+        soa[2 * n + i] = a[3 * i + 2];
+    }
+}
+void upload_points_device(prx_tree* t, const double* pos, const double* nrm, size_t n) {
+    const auto h0 = std::chrono::steady_clock::now();
+    const int64_t P = (int64_t)n;
+    if (P >= ((int64_t)1 << 29)) throw Error(PRX_ERR_INVALID, "at most 2^29 points per tree");
+    t->P = P;
+    t->pos.ensure(3 * std::max<int64_t>(P, 1));
+    t->nrm.ensure(3 * std::max<int64_t>(P, 1));
+    if (P > 0) {
+        const unsigned nb = (unsigned)std::min<int64_t>((P + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_aos3_to_soa, dim3(nb), dim3(256), 0, t->stream, pos, P, t->pos.p);
+        hipLaunchKernelGGL(k_aos3_to_soa, dim3(nb), dim3(256), 0, t->stream, nrm, P, t->nrm.p);
+        PRX_LAUNCH_CHECK();
+    }
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    t->setup_ms[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+}
+
 void splat(prx_tree* t) {
     const hipStream_t s = t->stream;
     const int64_t P = t->P, M8 = 8 * P;
@@ -1120,7 +1146,7 @@ static void build_eval_list(prx_tree* t) {
 // vjp of the field at the points (see include/prx.h): one adjoint solve with
 // the forward operator (A is symmetric), then point-local terms.
 static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, double* gnrm_host, int precision, double eps,
-                       prx_vjp_report* rep) {
+                       prx_vjp_report* rep, bool on_device = false) {
     if (!t->assembled) throw Error(PRX_ERR_STATE, "vjp before assemble/solve");
     const hipStream_t s = t->stream;
     const int64_t N = t->N, P = t->P;
@@ -1129,7 +1155,8 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
     build_eval_list(t);
     DArr<double> gF;
     gF.ensure(std::max<int64_t>(P, 1));
-    if (P > 0) PRX_HIP(hipMemcpyAsync(gF.p, gF_host, sizeof(double) * P, hipMemcpyHostToDevice, s));
+    if (P > 0)
+        PRX_HIP(hipMemcpyAsync(gF.p, gF_host, sizeof(double) * P, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     t->adj_g.ensure(N);
     t->adj_lam.ensure(N);
     launch_fill<double>(t->adj_g.p, 0.0, N, s);
@@ -1175,8 +1202,9 @@ static void vjp_points(prx_tree* t, const double* gF_host, double* gpos_host, do
     launch_adj_points(t->D, t->pts, t->floor_list, gF.p, t->x.p, t->adj_lam.p, t->nf.p, t->w.p, t->adj_gnf.p,
                       t->pw > 0.0 ? t->wh3 : 0.0, t->adj_out.p, t->adj_out.p + 3 * P, s);
     if (P > 0) {
-        PRX_HIP(hipMemcpyAsync(gpos_host, t->adj_out.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, s));
-        PRX_HIP(hipMemcpyAsync(gnrm_host, t->adj_out.p + 3 * P, sizeof(double) * 3 * P, hipMemcpyDeviceToHost, s));
+        const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        PRX_HIP(hipMemcpyAsync(gpos_host, t->adj_out.p, sizeof(double) * 3 * P, k, s));
+        PRX_HIP(hipMemcpyAsync(gnrm_host, t->adj_out.p + 3 * P, sizeof(double) * 3 * P, k, s));
     }
     PRX_HIP(hipStreamSynchronize(s));
     if (rep) {
@@ -1383,6 +1411,16 @@ int prx_splat_times(prx_tree* t, double* ms) {
     PRX_REQUIRE(t && ms, PRX_ERR_INVALID, "null pointer");
     PRX_REQUIRE(t->splatted, PRX_ERR_STATE, "splat_normal_field not called");
     for (int i = 0; i < 5; ++i) ms[i] = t->setup_ms[i];
+    PRX_API_END
+}
+
+int prx_splat_normal_field_device(prx_tree* t, const double* pos_device, const double* nrm_device, size_t n) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    PRX_REQUIRE((pos_device && nrm_device) || n == 0, PRX_ERR_INVALID, "null points");
+    set_device(t);
+    prx::upload_points_device(t, pos_device, nrm_device, n);
+    prx::splat(t);
     PRX_API_END
 }
 
@@ -1697,6 +1735,17 @@ int prx_evaluate(prx_tree* t, const double* q, size_t m, double* out) {
     PRX_API_END
 }
 
+int prx_evaluate_device(prx_tree* t, const double* q_device, size_t m, double* out_device) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && (q_device || m == 0) && (out_device || m == 0), PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->x.p, PRX_ERR_STATE, "no solution");
+    set_device(t);
+    if (m == 0) return PRX_OK;
+    launch_evaluate(t->D, t->x.p, q_device, (int64_t)m, out_device, t->stream);
+    PRX_HIP(hipStreamSynchronize(t->stream));
+    PRX_API_END
+}
+
 int prx_evaluate_corners(prx_tree* t, double* out) {
     PRX_API_BEGIN
     PRX_REQUIRE(t && out, PRX_ERR_INVALID, "null pointer");
@@ -1767,6 +1816,16 @@ int prx_vjp_points(prx_tree* t, const double* gF, double* gpos, double* gnrm, in
     PRX_REQUIRE(eps > 0.0 && eps < 1.0, PRX_ERR_INVALID, "eps must be in (0,1)");
     set_device(t);
     prx::vjp_points(t, gF, gpos, gnrm, precision, eps, report);
+    PRX_API_END
+}
+
+int prx_vjp_points_device(prx_tree* t, const double* gF, double* gpos, double* gnrm, int precision, double eps,
+                          prx_vjp_report* report) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && (gF || t->P == 0) && (gpos || t->P == 0) && (gnrm || t->P == 0), PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(eps > 0.0 && eps < 1.0, PRX_ERR_INVALID, "eps must be in (0,1)");
+    set_device(t);
+    prx::vjp_points(t, gF, gpos, gnrm, precision, eps, report, true);
     PRX_API_END
 }
 

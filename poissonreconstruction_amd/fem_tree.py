@@ -72,6 +72,7 @@ class FEMTree:
         check(L.lib().prx_tree_create(self.max_depth, int(device), ctypes.byref(h)))
         self._h = h
         self._points = None
+        self._n_points = 0
         self._pw = None
 
     def close(self):
@@ -114,7 +115,19 @@ class FEMTree:
         """src/fem_tree.rs:130-178; `points` are in the unit cube."""
         pts = L.as_points(points)
         self._points = pts
+        self._n_points = len(pts)
+        self._pw = None
         check(L.lib().prx_splat_normal_field(self._h, ptr(pts), len(pts)))
+
+    def splat_normal_field_device(self, pos_ptr: int, nrm_ptr: int, n: int) -> None:
+        """splat_normal_field with positions and normals already in device
+        memory, each [n][3] float64 (e.g. torch tensors' data_ptr()); the
+        caller's writes to them must be complete (stream synchronized)."""
+        self._points = None
+        self._n_points = int(n)
+        self._pw = None
+        check(L.lib().prx_splat_normal_field_device(self._h, ctypes.c_void_p(pos_ptr), ctypes.c_void_p(nrm_ptr),
+                                                    int(n)))
 
     def splat_times(self) -> dict:
         """ms of the phases of the last splat_normal_field (measurement)."""
@@ -237,6 +250,11 @@ class FEMTree:
         check(L.lib().prx_evaluate(self._h, ptr(q), q.shape[0], ptr(out)))
         return out
 
+    def evaluate_device(self, q_ptr: int, m: int, out_ptr: int) -> None:
+        """evaluate_many with q [m][3] float64 and out [m] float64 in device
+        memory; returns once out is written."""
+        check(L.lib().prx_evaluate_device(self._h, ctypes.c_void_p(q_ptr), int(m), ctypes.c_void_p(out_ptr)))
+
     def evaluate_corners(self) -> np.ndarray:
         """Field at every marching-cubes corner, shape (res+1,)*3 indexed [z,y,x]."""
         r1 = (1 << self.max_depth) + 1
@@ -249,7 +267,7 @@ class FEMTree:
         current solution must be converged): returns (dL/dpositions [P,3],
         dL/dnormals [P,3], report) in unit-cube coordinates."""
         g = np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(-1)
-        P = 0 if self._points is None else len(self._points)
+        P = self._n_points
         if g.shape[0] != P:
             raise ValueError("dL_dF must have one entry per splatted point")
         gpos = np.empty((P, 3))
@@ -257,6 +275,15 @@ class FEMTree:
         rep = L.VjpReport()
         check(L.lib().prx_vjp_points(self._h, ptr(g), ptr(gpos), ptr(gnrm), int(precision), float(eps), ctypes.byref(rep)))
         return gpos, gnrm, {k: getattr(rep, k) for k, _ in rep._fields_}
+
+    def vjp_points_device(self, g_ptr: int, gpos_ptr: int, gnrm_ptr: int, precision: int = L.PRX_PREC_MIXED,
+                          eps: float = 1e-10) -> dict:
+        """vjp_points with dL/dF [P], dL/dpositions [P][3] and dL/dnormals
+        [P][3] float64 in device memory; returns the report once they are written."""
+        rep = L.VjpReport()
+        check(L.lib().prx_vjp_points_device(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(gpos_ptr),
+                                            ctypes.c_void_p(gnrm_ptr), int(precision), float(eps), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
 
     def iso_value(self) -> float:
         """Mean field over the splatted points (src/reconstructor.rs:56)."""
