@@ -496,6 +496,7 @@ def operator_roofline(L, tree, N, P, reps):
         L.check(L.lib().prx_time_stencil_op(tree._h, prec, op, reps, ctypes.byref(g), ctypes.byref(k)))
         return g.value, k.value
     gc, kc = timed(L.PRX_PREC_MIXED, 2)
+    gf, kf = timed(L.PRX_PREC_MIXED, 3)
     gd, kd = timed(L.PRX_PREC_MIXED, 1)
     ga, ka = timed(L.PRX_PREC_MIXED, 0)
     gc64, kc64 = timed(L.PRX_PREC_FP64, 2)
@@ -537,6 +538,10 @@ def operator_roofline(L, tree, N, P, reps):
                                      traffic=measured_traffic("stencil_cg_step_fp64"),
                                      frac_of_pattern_ceiling=round((48 * N + 24 * P) / (kc64 * 1e-3) / 1e9
                                                                    / pattern["fp64_3x3_GBs"], 4)),
+                # the face form: r, d_old in and r', d' out (16N), the bricks' faces of q in and out
+                # (2 x 1536 values per 4096 nodes = 3N); gather_ms = its per-point gather and
+                # per-touched-node screening sums
+                "cg_step_faces_fp32": sub("k_cg_faces<float>", 19 * N, kf, gf),
                 "dir_apply_fp32": sub("k_stencil_tile<float,16,OP_DIR_APPLY>", dir_bytes, kd, gd),
                 "dir_apply_fp64": sub("k_stencil_tile<double,16,OP_DIR_APPLY>", 32 * N + 24 * P, kd64, gd64),
                 "apply_fp32": sub("k_stencil_tile<float,16,OP_APPLY>", apply_bytes, ka, ga),

@@ -120,13 +120,20 @@ typedef struct prx_solve_opts {
     double mg_omega;        /* smoothing: > 0 damped Jacobi with this weight; 0 (default) Chebyshev weights on [lmax/6, lmax]; < 0 Chebyshev on [lmax/-mg_omega, lmax] */
     double mg_inner_tol;    /* mixed MG: FP32 residual reduction per FP64 refinement step (default 1e-4) */
     /* CG iteration form (no counterpart in the reference; the same
-     * iteration): 0 (default) one fused pass per iteration on grids of depth
-     * >= 3 (the residual update r -= alpha q joins the next operator pass,
-     * whose |r|^2 for the stopping test and beta is expanded from the pass's
-     * d.q, r.q, q.q and r.r); 1 the two-pass form
-     * (operator pass, then the vector update with its own |r|^2) */
-    int32_t cg_two_pass;
+     * iteration): PRX_CG_FUSED (default) one fused pass per iteration on
+     * grids of depth >= 3 (the residual update r -= alpha q joins the next
+     * operator pass, whose |r|^2 for the stopping test and beta is expanded
+     * from the pass's d.q, r.q, q.q and r.r), in FP32 on grids of depth >= 4
+     * without storing q = A d (the pass recomputes it inside each 16^3 brick
+     * and reads it on the brick's boundary from the neighbours' faces);
+     * PRX_CG_TWO_PASS the two-pass form (operator pass, then the vector
+     * update with its own |r|^2); PRX_CG_FUSED_STORED_Q the fused pass with
+     * q stored as a full vector in every precision; PRX_CG_FUSED_FACES the
+     * face form in every precision where it applies */
+    int32_t cg_form;
 } prx_solve_opts;
+
+enum { PRX_CG_FUSED = 0, PRX_CG_TWO_PASS = 1, PRX_CG_FUSED_STORED_Q = 2, PRX_CG_FUSED_FACES = 3 };
 
 typedef struct prx_solve_report {
     uint64_t coarse_iters, fine_iters, outer_iters, applies;
@@ -262,7 +269,8 @@ int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, d
  * op 1 = the fused direction update and apply (d_new = r + beta d_old,
  * q = A d_new, beta = 0.5), op 2 = the fused CG step (r' = r - alpha q,
  * d_new = r' + beta d_old, q' = A d_new, alpha = 0.25, beta = 0.5; depth >= 3),
- * each with its screening gather. */
+ * each with its screening gather; op 3 = the same step in face form (q
+ * recomputed in each brick, faces of q' written; no gather; depth >= 4). */
 int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil);
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms);
 /* Measurement: the tile kernels' memory access pattern without arithmetic.

@@ -28,6 +28,10 @@ PRX_RESID_REFERENCE = 0
 PRX_RESID_TRUE = 1
 PRX_PRECOND_NONE = 0
 PRX_PRECOND_MG = 1
+PRX_CG_FUSED = 0
+PRX_CG_TWO_PASS = 1
+PRX_CG_FUSED_STORED_Q = 2
+PRX_CG_FUSED_FACES = 3
 PRX_MAX_DEPTH = 10
 
 STATUS_NAMES = {0: "ok", 1: "invalid", 2: "nonfinite", 3: "not_converged", 4: "breakdown", 5: "device", 6: "oom", 7: "state"}
@@ -68,7 +72,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
                 ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
                 ("coarse_eps", ctypes.c_double), ("precond", ctypes.c_int32), ("mg_smooth", ctypes.c_int32),
-                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double), ("cg_two_pass", ctypes.c_int32)]
+                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double), ("cg_form", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):

@@ -74,7 +74,7 @@ class Batch:
         o.residual_mode = int(residual_mode)
         o.refresh_period = int(refresh_period)
         o.precond = int(precond)
-        o.cg_two_pass = 1 if cg_two_pass else 0
+        o.cg_form = L.PRX_CG_TWO_PASS if cg_two_pass else L.PRX_CG_FUSED
         rep = L.BatchReport()
         per = (L.SolveReport * max(len(self._meta), 1))()
         check(L.lib().prx_solve_batched(self._h, int(gs), int(cg_iters), float(eps), ctypes.byref(o), 1 if autotune else 0,

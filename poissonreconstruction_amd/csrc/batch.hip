@@ -12,7 +12,7 @@
 //
 // One CG iteration of all instances is the fused CG step of the single tree
 // (screening gather, one operator pass, a two-stage fold; x every CG_XM
-// iterations) or, with cg_two_pass, five launches (gather, direction update +
+// iterations) or, with the two-pass form, five launches (gather, direction update +
 // operator, alpha, vector update, beta); a chunk of iterations is captured
 // once into a hipGraph and replayed until every instance is done, and the GS
 // sweeps are captured the same way.  An
@@ -121,7 +121,7 @@ struct prx_batch {
     prx::DArr<int32_t> flags, ones;
     std::vector<int32_t> flags_h;
     int tile = 16, chunk = 16, gs_block = 16;  // gs_block: brick edge of the GS passes
-    bool fused = true;                         // the fused CG step (prx_solve_opts.cg_two_pass == 0)
+    bool fused = true;                         // the fused CG step (prx_solve_opts.cg_form != PRX_CG_TWO_PASS)
     bool tuned = false;
     double tune_ms = 0.0;
     uint64_t graph_launches = 0;
@@ -990,7 +990,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     }
     const bool mixed = o.precision == PRX_PREC_MIXED;
     if (o.precision != PRX_PREC_FP64 && !mixed) throw Error(PRX_ERR_INVALID, "precision must be PRX_PREC_FP64 or PRX_PREC_MIXED");
-    if (mixed && o.cg_two_pass) throw Error(PRX_ERR_INVALID, "the batched mixed solve runs the fused CG step (cg_two_pass = 0)");
+    if (mixed && o.cg_form == PRX_CG_TWO_PASS) throw Error(PRX_ERR_INVALID, "the batched mixed solve runs the fused CG step");
     // the instance trees' own solver workspace (left by multigrid or single
     // solves) is not used here; return it before the concatenated levels grow
     PRX_HIP(hipStreamSynchronize(B->stream));
@@ -999,7 +999,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         t->release_workspace();
     }
     build(B);
-    B->fused = o.cg_two_pass == 0;
+    B->fused = o.cg_form != PRX_CG_TWO_PASS;
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;
     BLevel& C = B->coarse;

@@ -565,6 +565,186 @@ void launch_screen_gather(const LevelView& L, const T* x, const double* x64, con
 }
 template void launch_screen_gather<double>(const LevelView&, const double*, const double*, const float*, double*, hipStream_t,
                                            const double*, const double*, const double*, const double*, const CgState*);
+
+// ------------------------------------------------------------------ occupied cells (face-form screening)
+// A cell starts where the sorted points' trunc cell changes.
+__global__ void k_cell_heads(int64_t P, const uint32_t* __restrict__ cell, uint32_t* head) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    head[i] = (i == 0 || cell[i] != cell[i - 1] || cell[P + i] != cell[P + i - 1] || cell[2 * P + i] != cell[2 * P + i - 1]) ? 1u : 0u;
+}
+__global__ void k_cell_starts(int64_t P, const uint32_t* __restrict__ head, const uint32_t* __restrict__ pos, uint32_t* cstart) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    if (head[i]) cstart[pos[i]] = (uint32_t)i;
+}
+// key = corner node of cell c (N outside the grid), val = 8 c + corner
+__global__ void k_cell_corner_keys(uint32_t D, int64_t C, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cell,
+                                   int64_t P, uint32_t* key, uint32_t* val) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t res = 1u << D, N = 1u << (3 * D), p = cstart[c];
+    const uint32_t cx = cell[p], cy = cell[P + p], cz = cell[2 * P + p];
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t ox = cx + (k & 1), oy = cy + ((k >> 1) & 1), oz = cz + (k >> 2);
+        key[8 * c + k] = (ox < res && oy < res && oz < res) ? morton3(ox, oy, oz) : N;
+        val[8 * c + k] = (uint32_t)(8 * c + k);
+    }
+}
+// coff[t] = the first sorted pair of node[t]; coff[T] = the number of pairs inside the grid
+__global__ void k_cell_offsets(int64_t T, const uint32_t* __restrict__ node, const uint32_t* __restrict__ skey, int64_t M,
+                               uint32_t N, uint32_t* coff) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > T) return;
+    const uint32_t v = t < T ? node[t] : N;
+    int64_t lo = 0, hi = M;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (skey[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    coff[t] = (uint32_t)lo;
+}
+void build_cell_lists(CellLists& CL, uint32_t D, const DevPoints& pts, const NodeList& S, hipStream_t s) {
+    const int64_t P = pts.P;
+    const uint32_t N = 1u << (3 * D);
+    CL.C = 0;
+    PRX_HIP(hipMalloc((void**)&CL.coff, sizeof(uint32_t) * (size_t)(S.T + 1)));
+    if (P <= 0) {
+        PRX_HIP(hipMalloc((void**)&CL.cstart, sizeof(uint32_t)));
+        PRX_HIP(hipMalloc((void**)&CL.cval, sizeof(uint32_t)));
+        PRX_HIP(hipMemsetAsync(CL.coff, 0, sizeof(uint32_t) * (size_t)(S.T + 1), s));
+        return;
+    }
+    uint32_t *head, *pos;
+    PRX_HIP(hipMallocAsync((void**)&head, sizeof(uint32_t) * P, s));
+    PRX_HIP(hipMallocAsync((void**)&pos, sizeof(uint32_t) * P, s));
+    hipLaunchKernelGGL(k_cell_heads, dim3(nblk(P)), dim3(256), 0, s, P, pts.cell, head);
+    size_t need = 0;
+    PRX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, head, pos, (int)P, s));
+    void* scratch;
+    PRX_HIP(hipMallocAsync(&scratch, need, s));
+    PRX_HIP(hipcub::DeviceScan::ExclusiveSum(scratch, need, head, pos, (int)P, s));
+    uint32_t lastpos = 0, lasthead = 0;
+    PRX_HIP(hipMemcpyAsync(&lastpos, pos + P - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipMemcpyAsync(&lasthead, head + P - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipStreamSynchronize(s));
+    PRX_HIP(hipFreeAsync(scratch, s));
+    const int64_t C = (int64_t)lastpos + lasthead;
+    CL.C = C;
+    PRX_HIP(hipMalloc((void**)&CL.cstart, sizeof(uint32_t) * (size_t)(C + 1)));
+    hipLaunchKernelGGL(k_cell_starts, dim3(nblk(P)), dim3(256), 0, s, P, head, pos, CL.cstart);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, CL.cstart + C, (uint32_t)P);
+    uint32_t *key, *val, *skey;
+    PRX_HIP(hipMallocAsync((void**)&key, sizeof(uint32_t) * 8 * C, s));
+    PRX_HIP(hipMallocAsync((void**)&val, sizeof(uint32_t) * 8 * C, s));
+    PRX_HIP(hipMallocAsync((void**)&skey, sizeof(uint32_t) * 8 * C, s));
+    PRX_HIP(hipMalloc((void**)&CL.cval, sizeof(uint32_t) * 8 * (size_t)C));
+    hipLaunchKernelGGL(k_cell_corner_keys, dim3(nblk(C)), dim3(256), 0, s, D, C, CL.cstart, pts.cell, P, key, val);
+    const int kbits = 3 * (int)D + 1;
+    need = 0;
+    PRX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, key, skey, val, CL.cval, (int)(8 * C), 0, kbits, s));
+    PRX_HIP(hipMallocAsync(&scratch, need, s));
+    // stable: the pairs of one node stay in cell order
+    PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch, need, key, skey, val, CL.cval, (int)(8 * C), 0, kbits, s));
+    hipLaunchKernelGGL(k_cell_offsets, dim3(nblk(S.T + 1)), dim3(256), 0, s, S.T, S.node, skey, 8 * C, N, CL.coff);
+    PRX_LAUNCH_CHECK();
+    PRX_HIP(hipFreeAsync(scratch, s));
+    PRX_HIP(hipFreeAsync(key, s));
+    PRX_HIP(hipFreeAsync(val, s));
+    PRX_HIP(hipFreeAsync(skey, s));
+    PRX_HIP(hipFreeAsync(head, s));
+    PRX_HIP(hipFreeAsync(pos, s));
+    PRX_HIP(hipStreamSynchronize(s));
+}
+void free_cell_lists(CellLists& CL) {
+    if (CL.cstart) (void)hipFree(CL.cstart);
+    if (CL.coff) (void)hipFree(CL.coff);
+    if (CL.cval) (void)hipFree(CL.cval);
+    CL = CellLists{};
+}
+
+// The screening term of the face-form CG pass, cell by cell.  Each occupied
+// cell loads its 8 corners' operand once ((r - alpha q) + beta d_old, as the
+// OP_CG-mode gather forms it; 0 outside the grid, where the weight is 0 too),
+// forms s_p = wh3 * sum_k phi_k x_k for each of its points (the gather's
+// expression) and sums sig[8c + k] = sum_p phi_k(p) s_p.  Skipped when *done.
+template <typename T>
+__global__ void k_cell_gather(uint32_t D, int64_t C, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cell,
+                              const T* __restrict__ tt, int64_t P, T wh3, const T* __restrict__ r, const T* __restrict__ dd,
+                              const T* __restrict__ qq, const CgState* cst, const int32_t* done, T* sig) {
+    if (done && *done) return;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double bt = cst->beta, al = cst->alpha;
+    const int rp = cst->rpend;
+    const uint32_t res = 1u << D, p0 = cstart[c], p1 = cstart[c + 1];
+    const uint32_t cx = cell[p0], cy = cell[P + p0], cz = cell[2 * P + p0];
+    bool in[8];
+    uint32_t m[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t ox = cx + (k & 1), oy = cy + ((k >> 1) & 1), oz = cz + (k >> 2);
+        in[k] = ox < res && oy < res && oz < res;
+        m[k] = in[k] ? morton3(ox, oy, oz) : 0u;
+    }
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = in[k] ? cg_form(bt, al, rp, dd[m[k]], r[m[k]], qq[m[k]]) : T(0);
+    T sg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sg[k] = T(0);
+    for (uint32_t p = p0; p < p1; ++p) {
+        const T t0 = tt[p], t1 = tt[P + p], t2 = tt[2 * P + p];
+        T w[8], acc = T(0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const T bx = (k & 1) ? t0 : T(1) - t0;
+            const T by = ((k >> 1) & 1) ? t1 : T(1) - t1;
+            const T bz = (k >> 2) ? t2 : T(1) - t2;
+            w[k] = in[k] ? bx * by * bz : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += w[k] * v[k];
+        const T sp = wh3 * acc;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sg[k] += w[k] * sp;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sig[8 * c + k] = sg[k];
+}
+// sc[t] = sum of sig over the (cell, corner) pairs of touched node t, in cell order
+template <typename T>
+__global__ void k_cell_sums(int64_t Tn, const uint32_t* __restrict__ coff, const uint32_t* __restrict__ cval,
+                            const T* __restrict__ sig, T* sc, const int32_t* done) {
+    if (done && *done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Tn) return;
+    T acc = T(0);
+    for (uint32_t j = coff[t]; j < coff[t + 1]; ++j) acc += sig[cval[j]];
+    sc[t] = acc;
+}
+template <typename T>
+void launch_cell_screening(const LevelView& L, const CellLists& CL, const T* r, const T* d, const T* q, const CgState* cst,
+                           const int32_t* done, T* sig, T* sc, hipStream_t s) {
+    const DevPoints& pts = *L.pts;
+    const NodeList& S = *L.scr;
+    if (CL.C <= 0 || S.T <= 0) return;
+    const T* tt;
+    T wh3;
+    if constexpr (sizeof(T) == 8) { tt = (const T*)pts.t64; wh3 = (T)L.k64.wh3; }
+    else { tt = (const T*)pts.t32; wh3 = (T)L.k32.wh3; }
+    hipLaunchKernelGGL((k_cell_gather<T>), dim3(nblk(CL.C)), dim3(256), 0, s, L.D, CL.C, CL.cstart, pts.cell, tt, pts.P, wh3, r, d,
+                       q, cst, done, sig);
+    hipLaunchKernelGGL((k_cell_sums<T>), dim3(nblk(S.T)), dim3(256), 0, s, S.T, CL.coff, CL.cval, sig, sc, done);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_cell_screening<double>(const LevelView&, const CellLists&, const double*, const double*, const double*,
+                                            const CgState*, const int32_t*, double*, double*, hipStream_t);
+template void launch_cell_screening<float>(const LevelView&, const CellLists&, const float*, const float*, const float*,
+                                           const CgState*, const int32_t*, float*, float*, hipStream_t);
+
+
 template void launch_screen_gather<float>(const LevelView&, const float*, const double*, const float*, float*, hipStream_t,
                                           const float*, const float*, const double*, const float*, const CgState*);
 

@@ -169,6 +169,60 @@ __device__ __forceinline__ void column_of(int tid, int& tx, int& ty) {
     }
 }
 
+// The x neighbours of a 16^3 brick's column sweep come from the adjacent
+// lanes: with column_of<16>, lane & 15 is x, so each 16-lane DPP row holds
+// one x row.  row_from_left(v, edge) is v of lane x - 1 (row_shr:1) and
+// row_from_right v of lane x + 1 (row_shl:1); lane 0 / 15 of the row gets
+// `edge` (the halo column) instead.
+template <typename T>
+__device__ __forceinline__ T dpp_row(T v, T edge, int ctrl);
+template <>
+__device__ __forceinline__ float dpp_row<float>(float v, float edge, int ctrl) {
+    const int e = __builtin_bit_cast(int, edge), s = __builtin_bit_cast(int, v);
+    return ctrl == 0x111 ? __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(e, s, 0x111, 0xF, 0xF, false))
+                         : __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(e, s, 0x101, 0xF, 0xF, false));
+}
+template <>
+__device__ __forceinline__ double dpp_row<double>(double v, double edge, int ctrl) {
+    const long long e = __builtin_bit_cast(long long, edge), s = __builtin_bit_cast(long long, v);
+    int lo, hi;
+    if (ctrl == 0x111) {
+        lo = __builtin_amdgcn_update_dpp((int)e, (int)s, 0x111, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(s >> 32), 0x111, 0xF, 0xF, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp((int)e, (int)s, 0x101, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(s >> 32), 0x101, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <typename T>
+__device__ __forceinline__ T row_from_left(T v, T edge) { return dpp_row<T>(v, edge, 0x111); }
+template <typename T>
+__device__ __forceinline__ T row_from_right(T v, T edge) { return dpp_row<T>(v, edge, 0x101); }
+
+// One z plane of a 16^3 brick's 18^3 tile at column (tx, ty): C the node,
+// F its 4 in-plane face neighbours, Dg its 4 in-plane corner neighbours, in
+// the sums' fixed order.  Six LDS reads per plane instead of nine: the
+// three row values (rows y-1, y, y+1) and the three halo-column values that
+// only the edge lanes use (read by every lane, which keeps the code
+// branch-free); the x neighbours come through DPP.
+template <typename T>
+__device__ __forceinline__ void plane16(const T* tile, int p, int tx, int ty, T& C, T& F, T& Dg) {
+    constexpr int E = 18;
+    const T* a0 = tile + (p * E + ty) * E + tx;
+    const T* a1 = a0 + E;
+    const T* a2 = a1 + E;
+    const T c0 = a0[1], c1 = a1[1], c2 = a2[1];
+    const int o = tx == 0 ? 0 : 2;
+    const T e0 = a0[o], e1 = a1[o], e2 = a2[o];
+    const T l0 = row_from_left(c0, e0), r0 = row_from_right(c0, e0);
+    const T l1 = row_from_left(c1, e1), r1 = row_from_right(c1, e1);
+    const T l2 = row_from_left(c2, e2), r2 = row_from_right(c2, e2);
+    C = c1;
+    F = (c0 + c2) + (l1 + r1);
+    Dg = (l0 + r0) + (l2 + r2);
+}
+
 // halo shell index -> (hx,hy,hz) in [-1,B]^3 \\ [0,B)^3
 template <int B>
 __device__ inline void shell_pos(int i, int& hx, int& hy, int& hz) {
@@ -523,12 +577,16 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
         const T sc = L.s, wg = L.wg;
         T Cm, Fm, Dm, Cc, Fc, Dc;
         auto plane = [&](int p, T& C, T& F, T& Dg) {
-            const T* a0 = tile + (p * E + ty) * E + tx;
-            const T* a1 = a0 + E;
-            const T* a2 = a1 + E;
-            C = a1[1];
-            F = (a0[1] + a2[1]) + (a1[0] + a1[2]);
-            Dg = (a0[0] + a0[2]) + (a2[0] + a2[2]);
+            if constexpr (B == 16) {
+                plane16<T>(tile, p, tx, ty, C, F, Dg);
+            } else {
+                const T* a0 = tile + (p * E + ty) * E + tx;
+                const T* a1 = a0 + E;
+                const T* a2 = a1 + E;
+                C = a1[1];
+                F = (a0[1] + a2[1]) + (a1[0] + a1[2]);
+                Dg = (a0[0] + a0[2]) + (a2[0] + a2[2]);
+            }
         };
         plane(0, Cm, Fm, Dm);
         plane(1, Cc, Fc, Dc);
@@ -725,6 +783,376 @@ __global__ __launch_bounds__(B* B) void k_stencil_tile(LevelArgs<T> L, StencilIO
     if (tid == 0 && io.partial) io.partial[bid] = sacc;
 }
 
+
+// ------------------------------------------------------------------ fused CG step, face form
+// The fused CG step of OP_CG without a stored q = A d.  Pass k needs
+// q = A d_old only to form r' = r - alpha q.  Inside the brick it is
+// recomputed from d_old: the pass loads d_old with its halo anyway, and the
+// previous pass computed q there from the same LDS tile with the same code,
+// so the values are bit for bit those it would have stored.  On the one-node
+// halo, q is read from the neighbours' faces, which every pass writes: per
+// brick the six 16x16 boundary planes of its A d' (FACE_VALS values; face
+// f = 2 * axis + (plane at 15), index a + 16 b over the other two axes in
+// x, y, z order).  The screening term needs q at the touched nodes too: the
+// pass writes q' there into a node-indexed vector (io.qt, only touched nodes
+// written), from which the per-point gather forms the next operand's s_p as
+// it does for the stored-q form.  The recomputed q at a touched node is then
+// simply read back from io.qt (the brick itself wrote it, from the same tile,
+// in the previous pass), so only q' adds the screening sums, which come
+// precomputed per touched node (io.sc, launch_cell_screening).  Per iteration this reads r and d_old
+// (with halo) and the halo's faces and writes r', d' and the faces: 16 bytes
+// per node of vectors plus 3 of faces in FP32, where the stored-q form
+// moves 24.  The halo's d' is formed as soon as its loads arrive, so only
+// the interior's r and d_old stay in registers across the first sweep.
+template <typename T, bool BATCH = false>
+__global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> io, uint32_t nbricks, int swz, BatchTab tab) {
+    if constexpr (!BATCH) {
+        if (io.done && *io.done) return;
+    }
+    constexpr int B = 16, NT = 256, B3 = 4096, E = B + 2, E3 = E * E * E;
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T), PER = B3 / NT, NVEC = PER / VEC;
+    __shared__ T tile[E3];
+    __shared__ T stg[B3];
+    __shared__ double wsum[4 * (NT / 64)];
+    const int tid = threadIdx.x;
+    uint32_t bid = blockIdx.x;
+    if (swz) {
+        const uint32_t j = bid >> 3, c = (uint32_t)swz;
+        bid = (j / c) * 8u * c + (bid & 7u) * c + j % c;
+    }
+    const uint64_t base = (uint64_t)bid * B3;
+    uint32_t lb = bid, res = L.res;
+    uint64_t ib = 0;
+    double bt, al;
+    int rp;
+    if constexpr (BATCH) {
+        const uint32_t inst = tab.brick_inst[base >> 12];
+        if (tab.run[(int64_t)inst * tab.run_stride] != tab.run_when) return;
+        const BCoef c = tab.coef[inst];
+        res = c.res;
+        if constexpr (sizeof(T) == 8) { L.s = (T)c.s64; L.wg = (T)c.wg64; }
+        else { L.s = (T)c.s32; L.wg = (T)c.wg32; }
+        ib = tab.off[inst];
+        lb = bid - (uint32_t)(ib / B3);
+        bt = tab.st[inst].beta;
+        al = tab.st[inst].alpha;
+        rp = tab.st[inst].rpend;
+    } else {
+        bt = io.cst->beta;
+        al = io.cst->alpha;
+        rp = io.cst->rpend;
+    }
+    uint32_t ox, oy, oz;
+    demorton3(lb, ox, oy, oz);
+    ox *= B; oy *= B; oz *= B;
+
+    // ---- 0. the brick's touched nodes (screening): the first 4*NT with their
+    // q = A d_old and the screening sums of d'
+    uint32_t st0 = 0, st1 = 0;
+    if (L.has_screen) {
+        st0 = L.brick[bid];
+        st1 = L.brick[bid + 1];
+    }
+    uint32_t nl[4];
+    T qo[4], sn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t t = st0 + k * NT + tid;
+        const bool in = t < st1;
+        nl[k] = in ? (uint32_t)(L.node[t] - base) : 0u;
+        qo[k] = in && rp ? io.qt[base + nl[k]] : T(0);
+        sn[k] = in ? io.sc[t] : T(0);
+    }
+
+    // ---- 1. interior r and d_old (16-byte, coalesced)
+    NV ra[NVEC], da[NVEC];
+#pragma unroll
+    for (int v = 0; v < NVEC; ++v) {
+        const uint64_t g = base + (uint32_t)(v * NT + tid) * VEC;
+        ra[v] = ld_v(io.r + g);
+        da[v] = ld_v(io.dold + g);
+    }
+    const T r0d = io.dold[base];
+    // ---- 2. halo r, d_old from the neighbour bricks (the OP_CG tile's face /
+    // edge / corner split) and, when an update is pending, q from the
+    // neighbours' faces; d_old - r0d goes to the tile (for the recomputed q),
+    // d' is formed at once and kept
+    const int bx = (int)(ox >> 4), by = (int)(oy >> 4), bz = (int)(oz >> 4), nbr = (int)(res >> 4);
+    auto nb_base = [&](int dx, int dy, int dz, bool& ok) -> uint64_t {
+        const int cx = bx + dx, cy = by + dy, cz = bz + dz;
+        ok = (unsigned)cx < (unsigned)nbr && (unsigned)cy < (unsigned)nbr && (unsigned)cz < (unsigned)nbr;
+        return ok ? ib + (uint64_t)morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) * B3 : 0;
+    };
+    auto faces_of = [&](uint64_t nb) { return io.q + (nb >> 12) * FACE_VALS; };
+    auto tix = [](int hx, int hy, int hz) { return (hx + 1) + E * (hy + 1) + E * E * (hz + 1); };
+    // every halo load is issued before any is used
+    T zr[4], zd[4], zq[4], yr[2][2], yd[2][2], yq[2][2], xr[2], xd[2], xq[2], er = T(0), ed = T(0), eq = T(0);
+    int zi = 0, yi[2], xi[2], ei = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) zr[k] = zd[k] = zq[k] = T(0);
+    if (tid < 128) {  // z faces: a 2x2 run per thread
+        const int side = tid >> 6, a = tid & 7, b = (tid >> 3) & 7;
+        bool ok;
+        const uint64_t sb0 = nb_base(0, 0, side ? 1 : -1, ok);
+        const uint32_t m0 = spread3(2 * a) | (spread3(2 * b) << 1) | (side ? 0u : (spread3(15u) << 2));
+        if (ok) {
+            ldrun<T, 4>(io.r + sb0 + m0, zr);
+            ldrun<T, 4>(io.dold + sb0 + m0, zd);
+            if (rp) {
+                const T* f = faces_of(sb0) + (side ? 4 : 5) * 256 + 2 * a + 32 * b;
+                ldrun<T, 2>(f, zq);
+                ldrun<T, 2>(f + 16, zq + 2);
+            }
+        }
+        zi = tix(2 * a, 2 * b, side ? B : -1);
+    }
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {  // y faces: threads 128..255, an x pair on each side
+        yr[sd][0] = yr[sd][1] = yd[sd][0] = yd[sd][1] = yq[sd][0] = yq[sd][1] = T(0);
+        const int t = tid & 127, a = t & 7, c = t >> 3;
+        yi[sd] = tix(2 * a, sd ? B : -1, c);
+        if (tid >= 128) {
+            bool ok;
+            const uint64_t sb0 = nb_base(0, sd ? 1 : -1, 0, ok);
+            const uint32_t m0 = spread3(2 * a) | ((sd ? 0u : spread3(15u)) << 1) | (spread3((uint32_t)c) << 2);
+            if (ok) {
+                ldrun<T, 2>(io.r + sb0 + m0, yr[sd]);
+                ldrun<T, 2>(io.dold + sb0 + m0, yd[sd]);
+                if (rp) ldrun<T, 2>(faces_of(sb0) + (sd ? 2 : 3) * 256 + 2 * a + 16 * c, yq[sd]);
+            }
+        }
+    }
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {  // x faces: one node on each side per thread
+        const int yy = tid & 15, zz = tid >> 4;
+        xr[sd] = xd[sd] = xq[sd] = T(0);
+        bool ok;
+        const uint64_t sb0 = nb_base(sd ? 1 : -1, 0, 0, ok);
+        const uint32_t m = (sd ? 0u : spread3(15u)) | (spread3((uint32_t)yy) << 1) | (spread3((uint32_t)zz) << 2);
+        if (ok) {
+            xr[sd] = io.r[sb0 + m];
+            xd[sd] = io.dold[sb0 + m];
+            if (rp) xq[sd] = faces_of(sb0)[(sd ? 0 : 1) * 256 + yy + 16 * zz];
+        }
+        xi[sd] = tix(sd ? B : -1, yy, zz);
+    }
+    if (tid < 200) {  // 12 edges of 16 nodes (threads 0..191), 8 corners (192..199)
+        int hx, hy, hz;
+        if (tid < 192) {
+            const int e = tid >> 4, u = tid & 15, axis = e >> 2, s1 = (e & 1) ? B : -1, s2 = (e & 2) ? B : -1;
+            if (axis == 0) { hx = u; hy = s1; hz = s2; }
+            else if (axis == 1) { hy = u; hx = s1; hz = s2; }
+            else { hz = u; hx = s1; hy = s2; }
+        } else {
+            const int c = tid - 192;
+            hx = (c & 1) ? B : -1;
+            hy = (c & 2) ? B : -1;
+            hz = (c & 4) ? B : -1;
+        }
+        bool ok;
+        const uint64_t sb0 = nb_base(hx < 0 ? -1 : (hx >= B ? 1 : 0), hy < 0 ? -1 : (hy >= B ? 1 : 0), hz < 0 ? -1 : (hz >= B ? 1 : 0), ok);
+        const int lx = hx & 15, ly = hy & 15, lz = hz & 15;
+        const uint32_t m = spread3((uint32_t)lx) | (spread3((uint32_t)ly) << 1) | (spread3((uint32_t)lz) << 2);
+        if (ok) {
+            er = io.r[sb0 + m];
+            ed = io.dold[sb0 + m];
+            if (rp) {
+                // an x-side node lies on the neighbour's x face, an x-axis edge node on its y face
+                const bool xs = hx < 0 || hx >= B;
+                eq = faces_of(sb0)[xs ? (hx < 0 ? 1 : 0) * 256 + ly + 16 * lz : (hy < 0 ? 3 : 2) * 256 + lx + 16 * lz];
+            }
+        }
+        ei = tix(hx, hy, hz);
+    }
+    // d_old - r0d of the halo into the tile (for the recomputed q); the halo's d' is formed now and kept
+    if (rp) {
+        if (tid < 128) {
+            tile[zi] = zd[0] - r0d;
+            tile[zi + 1] = zd[1] - r0d;
+            tile[zi + E] = zd[2] - r0d;
+            tile[zi + E + 1] = zd[3] - r0d;
+        } else {
+#pragma unroll
+            for (int sd = 0; sd < 2; ++sd) {
+                tile[yi[sd]] = yd[sd][0] - r0d;
+                tile[yi[sd] + 1] = yd[sd][1] - r0d;
+            }
+        }
+        tile[xi[0]] = xd[0] - r0d;
+        tile[xi[1]] = xd[1] - r0d;
+        if (ei >= 0) tile[ei] = ed - r0d;
+    }
+    T hz4[4], hy2[2][2], hx1[2], he;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) hz4[k] = cg_form(bt, al, rp, zd[k], zr[k], zq[k]);
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        hy2[sd][0] = cg_form(bt, al, rp, yd[sd][0], yr[sd][0], yq[sd][0]);
+        hy2[sd][1] = cg_form(bt, al, rp, yd[sd][1], yr[sd][1], yq[sd][1]);
+        hx1[sd] = cg_form(bt, al, rp, xd[sd], xr[sd], xq[sd]);
+    }
+    he = cg_form(bt, al, rp, ed, er, eq);
+
+    // ---- helpers: interior placement, z sweep into the staging, screening
+    // (the tile holds x - r0, as the OP_CG tile)
+    const int lbase = (E * E + E + 1) + (int)(compact3(tid * VEC) + E * compact3((tid * VEC) >> 1) + E * E * compact3((tid * VEC) >> 2));
+    int tx, ty;
+    column_of<B>(tid, tx, ty);
+    const uint32_t mxy = spread3((uint32_t)tx) | (spread3((uint32_t)ty) << 1);
+    // (L + wg I) x on the interior, from the tile, into stg
+    auto sweep = [&](T r0) {
+        const T sc = L.s, wg = L.wg;
+        T Cm, Fm, Dm, Cc, Fc, Dc;
+        plane16<T>(tile, 0, tx, ty, Cm, Fm, Dm);
+        plane16<T>(tile, 1, tx, ty, Cc, Fc, Dc);
+#pragma unroll
+        for (int z = 0; z < B; ++z) {
+            T Cp, Fp, Dp;
+            plane16<T>(tile, z + 2, tx, ty, Cp, Fp, Dp);
+            const T lap = sc * (T(32) * Cc - T(2) * ((Dc + Fm) + Fp) - (Dm + Dp));
+            stg[stage_at<B>(mxy | (spread3c((uint32_t)z) << 2))] = lap + wg * (Cc + r0);
+            Cm = Cc; Fm = Fc; Dm = Dc;
+            Cc = Cp; Fc = Fp; Dc = Dp;
+        }
+    };
+    // q = A d_old at the touched nodes: the values the previous pass wrote
+    auto touched_from_qt = [&]() {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (st0 + k * NT + tid < st1) stg[stage_at<B>(nl[k])] = qo[k];
+        for (uint32_t t = st0 + 4 * NT + tid; t < st1; t += NT) {
+            const uint32_t loc = (uint32_t)(L.node[t] - base);
+            stg[stage_at<B>(loc)] = io.qt[base + loc];
+        }
+    };
+    // S d' at the touched nodes added to stg, and the results to io.qt
+    auto screening = [&]() {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (st0 + k * NT + tid < st1) {
+                const uint32_t at = stage_at<B>(nl[k]);
+                const T v = stg[at] + sn[k];
+                stg[at] = v;
+                io.qt[base + nl[k]] = v;
+            }
+        for (uint32_t t = st0 + 4 * NT + tid; t < st1; t += NT) {
+            const uint32_t loc = (uint32_t)(L.node[t] - base);
+            const uint32_t at = stage_at<B>(loc);
+            const T v = stg[at] + io.sc[t];
+            stg[at] = v;
+            io.qt[base + loc] = v;
+        }
+    };
+
+    // ---- 3. q = A d_old on the interior (when an update is pending), then
+    // r' = r - alpha q and d' = r' + beta d_old on the interior
+    T rn[PER], dn[PER];
+    T r0n;
+    if (rp) {
+#pragma unroll
+        for (int v = 0; v < NVEC; ++v)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+                tile[lbase + lds_lin<E>((uint32_t)(v * NT * VEC)) + lds_lin<E>((uint32_t)e)] = (T)da[v][e] - r0d;
+        __syncthreads();
+        sweep(r0d);
+        __syncthreads();
+        if (L.has_screen) touched_from_qt();
+        __syncthreads();
+        r0n = dir_form(bt, r0d, r_form(al, stg[stage_at<B>(0u)], io.r[base]));
+#pragma unroll
+        for (int v = 0; v < NVEC; ++v) {
+            const uint32_t m = (uint32_t)(v * NT + tid) * VEC;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                rn[v * VEC + e] = r_form(al, stg[stage_at<B>(m) + e], (T)ra[v][e]);
+                dn[v * VEC + e] = dir_form(bt, (T)da[v][e], rn[v * VEC + e]);
+            }
+        }
+    } else {
+        r0n = dir_form(bt, r0d, io.r[base]);
+#pragma unroll
+        for (int v = 0; v < NVEC; ++v)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                rn[v * VEC + e] = (T)ra[v][e];
+                dn[v * VEC + e] = dir_form(bt, (T)da[v][e], rn[v * VEC + e]);
+            }
+    }
+    double c3[4] = {0.0, 0.0, 0.0, 0.0};  // d'.q', r'.q', q'.q', r'.r'
+#pragma unroll
+    for (int v = 0; v < NVEC; ++v) {
+        const uint64_t g = base + (uint32_t)(v * NT + tid) * VEC;
+        NV nr, nd;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            nr[e] = rn[v * VEC + e];
+            nd[e] = dn[v * VEC + e];
+            c3[3] += (double)nr[e] * (double)nr[e];
+        }
+        st_nt(io.rout + g, nr);
+        st_nt(io.dnew + g, nd);
+    }
+    // the tile now takes d' (every read of d_old's tile ended at the barrier above)
+#pragma unroll
+    for (int v = 0; v < NVEC; ++v)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+            tile[lbase + lds_lin<E>((uint32_t)(v * NT * VEC)) + lds_lin<E>((uint32_t)e)] = dn[v * VEC + e] - r0n;
+    if (tid < 128) {
+        tile[zi] = hz4[0] - r0n;
+        tile[zi + 1] = hz4[1] - r0n;
+        tile[zi + E] = hz4[2] - r0n;
+        tile[zi + E + 1] = hz4[3] - r0n;
+    } else {
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            tile[yi[sd]] = hy2[sd][0] - r0n;
+            tile[yi[sd] + 1] = hy2[sd][1] - r0n;
+        }
+    }
+    tile[xi[0]] = hx1[0] - r0n;
+    tile[xi[1]] = hx1[1] - r0n;
+    if (ei >= 0) tile[ei] = he - r0n;
+    __syncthreads();
+
+    // ---- 4. q' = A d' on the interior
+    sweep(r0n);
+    __syncthreads();
+    if (L.has_screen) screening();
+    __syncthreads();
+
+    // ---- 5. the pass's dot partials and the faces of q'
+#pragma unroll
+    for (int v = 0; v < NVEC; ++v) {
+        const uint32_t m = (uint32_t)(v * NT + tid) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            const T q = stg[stage_at<B>(m) + e];
+            c3[0] += (double)dn[v * VEC + e] * (double)q;
+            c3[1] += (double)rn[v * VEC + e] * (double)q;
+            c3[2] += (double)q * (double)q;
+        }
+    }
+    {
+        T* fo = io.y + (uint64_t)bid * FACE_VALS;
+        const uint32_t a = (uint32_t)tid & 15u, b = (uint32_t)tid >> 4;
+        fo[tid] = stg[stage_at<B>(morton3(0u, a, b))];
+        fo[256 + tid] = stg[stage_at<B>(morton3(15u, a, b))];
+        fo[512 + tid] = stg[stage_at<B>(morton3(a, 0u, b))];
+        fo[768 + tid] = stg[stage_at<B>(morton3(a, 15u, b))];
+        fo[1024 + tid] = stg[stage_at<B>(morton3(a, b, 0u))];
+        fo[1280 + tid] = stg[stage_at<B>(morton3(a, b, 15u))];
+    }
+    double s4[4];
+    block_sum_redn<4>(c3, wsum, s4);
+    if (tid == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) io.partial[4 * (uint64_t)bid + j] = s4[j];
+    }
+}
+
 // ------------------------------------------------------------------ per-node form (small grids, GS)
 template <typename T>
 __device__ inline int64_t touched_index(const LevelArgs<T>& L, uint32_t i) {
@@ -900,6 +1328,20 @@ void launch_stencil(const LevelView& Lv, int op, const StencilIO<T>& io, int* nb
 }
 template void launch_stencil<double>(const LevelView&, int, const StencilIO<double>&, int*, int, hipStream_t);
 template void launch_stencil<float>(const LevelView&, int, const StencilIO<float>&, int*, int, hipStream_t);
+
+
+template <typename T>
+void launch_cg_faces(const LevelView& Lv, const StencilIO<T>& io, int* nblk, hipStream_t s) {
+    if (Lv.D < 4) throw Error(PRX_ERR_INVALID, "the face form of the fused CG pass needs 16^3 bricks (depth >= 4)");
+    const LevelArgs<T> a = make_args<T>(Lv, 12);
+    if (a.has_screen && !(io.sc && io.qt)) throw Error(PRX_ERR_STATE, "the face form needs sc and qt");
+    const uint32_t nbricks = (uint32_t)(Lv.N >> 12);
+    hipLaunchKernelGGL((k_cg_faces<T, false>), dim3(nbricks), dim3(256), 0, s, a, io, nbricks, xcd_chunk(nbricks), BatchTab{});
+    PRX_LAUNCH_CHECK();
+    *nblk = (int)nbricks;
+}
+template void launch_cg_faces<double>(const LevelView&, const StencilIO<double>&, int*, hipStream_t);
+template void launch_cg_faces<float>(const LevelView&, const StencilIO<float>&, int*, hipStream_t);
 
 // ------------------------------------------------------------------ multicolour Gauss-Seidel
 // gauss_seidel_sweep (src/solvers.rs:164-175) recoloured: the 27-point

@@ -82,7 +82,8 @@ struct LevelView {
 // to dnew and y; four partials per brick, interleaved (d'.y, r'.y, y.y, r'.r').
 // alpha, beta and rpend come from io.cst.  rout, dnew and y must not alias
 // r, dold and q: neighbouring workgroups still read those as halo.
-enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5, OP_GS = 6, OP_CG = 7 };
+enum StencilOp { OP_APPLY = 0, OP_RESID = 1, OP_RESID_UPD = 2, OP_DIR_APPLY = 3, OP_JACOBI = 4, OP_JACOBI2 = 5, OP_GS = 6, OP_CG = 7,
+                 OP_CG_FACES = 8 /* timing selector of launch_cg_faces */ };
 
 // ---- k_grid.hip
 void launch_export_offsets(uint32_t D, uint32_t* i2o, hipStream_t s);
@@ -155,6 +156,16 @@ template <typename T>
 void launch_screen_gather(const LevelView& L, const T* x, const double* x64, const float* e32, T* sbuf, hipStream_t s,
                           const T* r = nullptr, const T* d = nullptr, const double* beta = nullptr, const T* q = nullptr,
                           const CgState* cst = nullptr);
+// The occupied cells and each touched node's (cell, corner) pairs (built once
+// per splat), and the face-form screening sums from them: sig[8 C] per cell
+// corner, then sc[t] per touched node, for the OP_CG operand of r, d, q and
+// the CG state; skipped when *done is set.
+void build_cell_lists(CellLists& CL, uint32_t D, const DevPoints& pts, const NodeList& S, hipStream_t s);
+void free_cell_lists(CellLists& CL);
+template <typename T>
+void launch_cell_screening(const LevelView& L, const CellLists& CL, const T* r, const T* d, const T* q, const CgState* cst,
+                           const int32_t* done, T* sig, T* sc, hipStream_t s);
+
 void launch_csr_count(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, int64_t* rowcnt, hipStream_t s);
 void launch_csr_fill(uint32_t D, const NodeList* L, const DevPoints* pts, int has_diag, double K0, double Ke, double Kc,
                      double wg, double wh3, const int64_t* off, int64_t* col, double* val, hipStream_t s);
@@ -186,12 +197,23 @@ struct StencilIO {
     double jw = 0.0;   // OP_JACOBI weight; OP_JACOBI2: the first step's
     double jw2 = 0.0;  // OP_JACOBI2: the second step's weight
     int color = 0;     // OP_GS colour
-    const T* q = nullptr;  // OP_CG: the previous pass's A d
+    const T* q = nullptr;  // OP_CG: the previous pass's A d (face form: its faces)
+    const T* sc = nullptr;       // face form: the screening sums of d' per touched node
+    T* qt = nullptr;             // face form: A d at the touched nodes (in: of d_old; out: of d'; node-indexed)
     T* rout = nullptr;     // OP_CG: the updated residual
     const CgState* cst = nullptr;  // OP_CG: alpha, beta, rpend
 };
 template <typename T>
 void launch_stencil(const LevelView& L, int op, const StencilIO<T>& io, int* nblk, int tile, hipStream_t s);
+// The fused CG step in face form (16^3 bricks, depth >= 4): as OP_CG, but q
+// is not stored: io.q holds the previous pass's faces of A d (FACE_VALS per
+// brick), io.y receives this pass's; io.qt (a node-indexed vector) holds
+// A d_old at the touched nodes and receives A d' there, where the OP_CG-mode
+// gather reads q; io.sc holds the screening sums of d' per touched node
+// (launch_cell_screening).  Four partials per brick as OP_CG.
+constexpr int FACE_VALS = 1536;
+template <typename T>
+void launch_cg_faces(const LevelView& L, const StencilIO<T>& io, int* nblk, hipStream_t s);
 template <typename T>
 void launch_gs_color(const LevelView& L, int color, T* x, const T* b, const T* sbuf, hipStream_t s);
 

@@ -70,9 +70,9 @@ int prx_time_operator(prx_tree* t, int precision, int reps, double* ms_gather, d
 }
 
 int prx_time_stencil_op(prx_tree* t, int precision, int op, int reps, double* ms_gather, double* ms_stencil) {
-    if (!t || reps <= 0 || op < 0 || op > 2) return PRX_ERR_INVALID;
+    if (!t || reps <= 0 || op < 0 || op > 3) return PRX_ERR_INVALID;
     return prx::time_operator(t, precision, reps, ms_gather, ms_stencil,
-                              op == 0 ? prx::OP_APPLY : op == 1 ? prx::OP_DIR_APPLY : prx::OP_CG);
+                              op == 0 ? prx::OP_APPLY : op == 1 ? prx::OP_DIR_APPLY : op == 2 ? prx::OP_CG : prx::OP_CG_FACES);
 }
 
 int prx_copy_bandwidth(size_t bytes, int reps, int device, double* ms) {

@@ -139,5 +139,15 @@ struct NodeList {
     uint32_t* brick8 = nullptr;   // [N/512+1] the same for 8^3 bricks (D >= 3)
 };
 
+// The occupied trunc cells of the sorted points and, per touched node of the
+// screening list, its (cell, corner) pairs: the face-form CG pass forms the
+// screening term cell by cell (every point of a cell has the same 8 corners).
+struct CellLists {
+    int64_t C = 0;              // occupied cells
+    uint32_t* cstart = nullptr; // [C+1] first sorted point of each cell
+    uint32_t* coff = nullptr;   // [T+1] per touched node: its range in cval
+    uint32_t* cval = nullptr;   // [8C] 8 * cell + corner, grouped by node (cell order inside a node)
+};
+
 
 }  // namespace prx

@@ -25,6 +25,32 @@ template <>
 double* sbuf_for<double>(prx_tree* t) { t->s64.ensure(std::max<int64_t>(t->P, 1)); return t->s64.p; }
 template <>
 float* sbuf_for<float>(prx_tree* t) { t->s32.ensure(std::max<int64_t>(t->P, 1)); return t->s32.p; }
+// The face form's screening data: the cell lists (built once per splat) and
+// one array of [0, N) q at the touched nodes (node-indexed; the other nodes
+// stay 0, which the cell gather never reads), [T] the screening sums per
+// touched node, [8C] the sums per cell corner.
+static const CellLists& cells_for(prx_tree* t, const LevelView& L) {
+    if (!t->cells_ok) {
+        free_cell_lists(t->cells);
+        build_cell_lists(t->cells, t->D, t->pts, *L.scr, t->stream);
+        t->cells_ok = true;
+    }
+    return t->cells;
+}
+template <typename T>
+static T* qt_for(prx_tree* t, const LevelView& L) {
+    const CellLists& CL = cells_for(t, L);
+    const int64_t n = L.N + std::max<int64_t>(L.scr->T, 1) + 8 * std::max<int64_t>(CL.C, 1);
+    bool fresh;
+    T* p;
+    if constexpr (sizeof(T) == 8) { fresh = !t->qt64.p || t->qt64.n < n; t->qt64.ensure(n); p = (T*)t->qt64.p; }
+    else { fresh = !t->qt32.p || t->qt32.n < n; t->qt32.ensure(n); p = (T*)t->qt32.p; }
+    if (fresh || !t->qt_ok) {
+        launch_fill<T>(p, T(0), n, t->stream);
+        t->qt_ok = true;
+    }
+    return p;
+}
 
 // y = A x (OP_APPLY) or y = b - A x (OP_RESID); partial sums into t->partial.
 template <typename T>
@@ -135,7 +161,9 @@ static double dev_dot(prx_tree* t, const T* a, const T* b, int64_t n) {
 // (d_j sits in slot (j+1) % CG_XM).  x takes CG_XM terms at once, before the
 // pass that overwrites the oldest direction.  Per iteration this reads r, q,
 // d and writes r, d, q (24 bytes per node in FP32) plus 6 for x, where the
-// two-pass form moves 28 plus 8.
+// two-pass form moves 28 plus 8.  In face form (FP32, depth >= 4) qq[k&1]
+// hold the bricks' faces of q instead of q, and a pass moves 16 bytes per
+// node plus 3 of faces (launch_cg_faces).
 template <typename T>
 struct Cg {
     prx_tree* t;
@@ -148,6 +176,11 @@ struct Cg {
     T* rr[2] = {nullptr, nullptr};
     T* qq[2] = {nullptr, nullptr};
     bool fused = false;
+    bool faces = false;  // fused pass in face form: qq hold the bricks' faces of q, not q
+    T* qt = nullptr;                // face form: q at the touched nodes (node-indexed)
+    T* sc = nullptr;                // face form: the screening sums per touched node
+    T* sig = nullptr;               // face form: the screening sums per cell corner
+    const CellLists* cells = nullptr;
     int xn = 0;  // fused: x terms pending on the device (as FIN_CG counts them)
     int refresh = 0;
     uint64_t it = 0;  // completed iterations (the device's count after run())
@@ -206,13 +239,22 @@ struct Cg {
         io.cst = t->st;
         io.partial = t->partial.p;
         io.done = done;
-        if (L.scr) {
-            T* sb = sbuf_for<T>(t);
-            launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, io.r, dold, nullptr, io.q, t->st);
-            io.s = sb;
-        }
         int nb = 0;
-        launch_stencil<T>(L, OP_CG, io, &nb, t->tile, s);
+        if (faces) {
+            if (L.scr) {
+                launch_cell_screening<T>(L, *cells, io.r, dold, qt, t->st, done, sig, sc, s);
+                io.sc = sc;
+                io.qt = qt;
+            }
+            launch_cg_faces<T>(L, io, &nb, s);
+        } else {
+            if (L.scr) {
+                T* sb = sbuf_for<T>(t);
+                launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, io.r, dold, nullptr, io.q, t->st);
+                io.s = sb;
+            }
+            launch_stencil<T>(L, OP_CG, io, &nb, t->tile, s);
+        }
         applies++;
         const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
         launch_fin(t->st, t->partial.p, nb, FIN_CG, fresh ? 1 : 0, s);
@@ -317,7 +359,17 @@ static Cg<T> make_cg(prx_tree* t, const LevelView& L, T* x, const T* b, T* r, T*
     for (int i = 0; i < CG_XM; ++i) c.dd[i] = i < nd ? d + (int64_t)i * L.N : nullptr;
     c.q = q;
     c.refresh = refresh;
-    c.fused = r1 && q1 && nd >= CG_XM && L.D >= 3 && t->fused_cg;
+    c.fused = r1 && q1 && nd >= CG_XM && L.D >= 3 && t->cg_form != PRX_CG_TWO_PASS;
+    // the face form: FP32, 16^3 bricks, and the screening's point data when the level has screening
+    c.faces = c.fused && L.D >= 4 && t->tile == 16 &&
+              ((t->cg_form == PRX_CG_FUSED && sizeof(T) == 4) || t->cg_form == PRX_CG_FUSED_FACES) &&
+              (!L.scr || (L.scr->spc && L.pts && L.pts->P > 0));
+    if (c.faces && L.scr) {
+        c.qt = qt_for<T>(t, L);
+        c.sc = c.qt + L.N;
+        c.sig = c.sc + std::max<int64_t>(L.scr->T, 1);
+        c.cells = &t->cells;
+    }
     c.rr[0] = r;
     c.rr[1] = r1;
     c.qq[0] = q;
@@ -949,7 +1001,7 @@ static void solve_cascadic(prx_tree* t, const SolveCfg& c, prx_solve_report& rep
     std::memset(&rep, 0, sizeof rep);
     t->ensure_state();
     t->precond = c.o.precond;
-    t->fused_cg = c.o.cg_two_pass ? 0 : 1;
+    t->cg_form = c.o.cg_form;
     if (c.o.precond == PRX_PRECOND_MG) {
         solve_mg(t, c, rep);
         return;
@@ -1025,6 +1077,8 @@ void splat(prx_tree* t) {
     free_list(t->floor_list);
     free_list(t->trunc_list);
     free_list(t->eval_list);
+    free_cell_lists(t->cells);
+    t->cells_ok = false;
     t->perm.ensure(std::max<int64_t>(P, 1));
     t->cell.ensure(3 * std::max<int64_t>(P, 1));
     t->t64.ensure(3 * std::max<int64_t>(P, 1));
@@ -1069,12 +1123,16 @@ void splat(prx_tree* t) {
         Timer ttrunc(s);
         PRX_HIP(hipcub::DeviceRadixSort::SortPairs(scratch.p, sb, kt.p, kt2.p, val.p, val3.p, (int)M8, 0, kbits, s));
         build_node_list(t->trunc_list, t->D, kt2.p, val3.p, M8, t->pts, inv.p, scratch.p, sb, s);
+        // and the occupied cells with each touched node's (cell, corner) pairs (face-form CG)
+        build_cell_lists(t->cells, t->D, t->pts, t->trunc_list, s);
+        t->cells_ok = true;
         t->setup_ms[3] = ttrunc.ms();
     }
     Timer tsplat(s);
     launch_splat(t->D, t->floor_list, t->pts, t->nf.p, t->w.p, s);
     t->setup_ms[4] = tsplat.ms();
     t->splatted = true;
+    t->qt_ok = false;
     t->assembled = false;
 }
 
@@ -1247,6 +1305,24 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
             int nb = 0;
             for (int r = -1; r < reps; ++r) {  // r = -1 warms up
                 PRX_HIP(hipEventRecord(e0, s));
+                if (op == OP_CG_FACES) {
+                    if (L.scr) {
+                        io.qt = qt_for<T>(t, L);
+                        T* scs = io.qt + L.N;
+                        io.sc = scs;
+                        launch_cell_screening<T>(L, t->cells, xin, dold, io.qt, t->st, nullptr, scs + std::max<int64_t>(L.scr->T, 1),
+                                                 scs, s);
+                    }
+                    PRX_HIP(hipEventRecord(e1, s));
+                    launch_cg_faces<T>(L, io, &nb, s);
+                    PRX_HIP(hipEventRecord(e2, s));
+                    PRX_HIP(hipEventSynchronize(e2));
+                    float a = 0.f, b = 0.f;
+                    PRX_HIP(hipEventElapsedTime(&a, e0, e1));
+                    PRX_HIP(hipEventElapsedTime(&b, e1, e2));
+                    if (r >= 0) { g += a; k += b; }
+                    continue;
+                }
                 if (L.scr) {
                     if (op == OP_DIR_APPLY) launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, xin, dold, &t->st->beta);
                     else if (op == OP_CG) launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, xin, dold, nullptr, qin, t->st);
@@ -1262,9 +1338,14 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
                 if (r >= 0) { g += a; k += b; }
             }
         };
-        if (op != OP_APPLY && op != OP_DIR_APPLY && op != OP_CG) return PRX_ERR_INVALID;
+        if (op != OP_APPLY && op != OP_DIR_APPLY && op != OP_CG && op != OP_CG_FACES) return PRX_ERR_INVALID;
         if (op == OP_CG && L.D < 3) return PRX_ERR_INVALID;
-        if (op == OP_DIR_APPLY || op == OP_CG) {  // beta = 0.5, alpha = 0.25, r update pending, not done
+        if (op == OP_CG_FACES && L.D < 4) return PRX_ERR_INVALID;
+        if (op == OP_CG_FACES && L.scr) {  // defined screening inputs for the timed pass
+            if (precision == PRX_PREC_MIXED) qt_for<float>(t, L);
+            else qt_for<double>(t, L);
+        }
+        if (op == OP_DIR_APPLY || op == OP_CG || op == OP_CG_FACES) {  // beta = 0.5, alpha = 0.25, r update pending, not done
             CgState h{};
             h.beta = 0.5;
             h.alpha = 0.25;
@@ -1281,7 +1362,7 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
             launch_fill<float>(t->q32.p, 0.125f, n, s);
             float* sb = L.scr ? sbuf_for<float>(t) : (float*)nullptr;
             if (op == OP_DIR_APPLY) run(t->ri32.p, t->q32.p, sb, t->d32.p, t->d32.p + n, (float*)nullptr, (float*)nullptr);
-            else if (op == OP_CG) run(t->ri32.p, t->q32.p + n, sb, t->d32.p, t->d32.p + n, t->q32.p, t->ri32.p + n);
+            else if (op == OP_CG || op == OP_CG_FACES) run(t->ri32.p, t->q32.p + n, sb, t->d32.p, t->d32.p + n, t->q32.p, t->ri32.p + n);
             else run(t->d32.p, t->q32.p, sb, (float*)nullptr, (float*)nullptr, (float*)nullptr, (float*)nullptr);
         } else {
             t->d64.ensure(2 * n);
@@ -1292,7 +1373,7 @@ int time_operator(prx_tree* t, int precision, int reps, double* ms_gather, doubl
             launch_fill<double>(t->q64.p, 0.125, n, s);
             double* sb = L.scr ? sbuf_for<double>(t) : (double*)nullptr;
             if (op == OP_DIR_APPLY) run(t->r64.p, t->q64.p, sb, t->d64.p, t->d64.p + n, (double*)nullptr, (double*)nullptr);
-            else if (op == OP_CG) run(t->r64.p, t->q64.p + n, sb, t->d64.p, t->d64.p + n, t->q64.p, t->r64.p + n);
+            else if (op == OP_CG || op == OP_CG_FACES) run(t->r64.p, t->q64.p + n, sb, t->d64.p, t->d64.p + n, t->q64.p, t->r64.p + n);
             else run(t->d64.p, t->q64.p, sb, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr);
         }
         (void)hipEventDestroy(e0);
@@ -1356,7 +1437,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->mg_omega = 0.0;
     o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
-    o->cg_two_pass = 0;
+    o->cg_form = PRX_CG_FUSED;
 }
 
 int prx_tree_create(uint32_t depth, int device, prx_tree** out) {
@@ -1693,6 +1774,7 @@ int prx_solve_cascadic(prx_tree* t, uint32_t base_depth, uint32_t gs, uint64_t c
     c.eps = eps;
     prx_solve_opts_default(&c.o);
     if (opts) c.o = *opts;
+    PRX_REQUIRE(c.o.cg_form >= PRX_CG_FUSED && c.o.cg_form <= PRX_CG_FUSED_FACES, PRX_ERR_INVALID, "unknown cg_form");
     prx_solve_report rep;
     prx::solve_cascadic(t, c, rep);
     if (report) *report = rep;

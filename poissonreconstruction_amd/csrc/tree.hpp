@@ -115,6 +115,13 @@ struct prx_tree {
     // solver workspace
     DArr<double> r64, d64, q64, s64, partial, tmp64;
     DArr<float> e32, r32, ri32, d32, q32, s32;
+    // face-form CG: q at the touched nodes (node-indexed, zero elsewhere), the
+    // screening sums per touched node and per cell corner, and the cell lists
+    DArr<float> qt32;
+    DArr<double> qt64;
+    bool qt_ok = false;
+    CellLists cells;
+    bool cells_ok = false;
     CgState* st = nullptr;
     CgState* st_host = nullptr;
     CgState* st_snap = nullptr;  // [2] pinned: per-iteration state copies (PCG without host waits)
@@ -135,13 +142,14 @@ struct prx_tree {
     // last prx_splat_normal_field, ms: host packing + upload, spatial point
     // sort, splat lists, screening lists, splat kernel
     double setup_ms[5] = {0, 0, 0, 0, 0};
-    int fused_cg = 1;  // CG iteration form of the last solve (prx_solve_opts.cg_two_pass == 0)
+    int cg_form = PRX_CG_FUSED;  // CG iteration form of the last solve (prx_solve_opts.cg_form)
 
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
         free_list(floor_list);
         free_list(trunc_list);
         free_list(eval_list);
+        free_cell_lists(cells);
         if (st) (void)hipFree(st);
         if (st_host) (void)hipHostFree(st_host);
         if (st_snap) (void)hipHostFree(st_snap);
@@ -155,6 +163,7 @@ struct prx_tree {
     void release_workspace() {
         r64.release(); d64.release(); q64.release(); s64.release(); tmp64.release();
         e32.release(); r32.release(); ri32.release(); d32.release(); q32.release(); s32.release();
+        qt32.release(); qt64.release(); qt_ok = false;
         mg64.release(); mg32.release();
     }
 

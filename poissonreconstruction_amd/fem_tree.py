@@ -186,12 +186,13 @@ class FEMTree:
                        precision: int = L.PRX_PREC_FP64, residual_mode: int = L.PRX_RESID_REFERENCE,
                        refresh_period: int = 10, inner_tol: float = 0.05, max_outer: int = 64,
                        precond: int = L.PRX_PRECOND_NONE, mg_smooth: int = 2, mg_omega: float = 0.0,
-                       mg_inner_tol: float = 1e-4, cg_two_pass: bool = False) -> dict:
+                       mg_inner_tol: float = 1e-4, cg_two_pass: bool = False, cg_form: int = None) -> dict:
         """src/fem_tree.rs:458-487; returns the solve report.  precond
         PRX_PRECOND_MG solves the same system with multigrid-preconditioned
         CG from x = 0 instead (stopping at ||b - A x|| <= eps ||b||).
         cg_two_pass selects the two-pass CG iteration instead of the fused
-        one (the same iteration, other rounding)."""
+        one (the same iteration, other rounding); cg_form (PRX_CG_*) selects
+        any form and overrides it."""
         self._assemble(points, pw)
         o = L.default_solve_opts()
         o.precision = int(precision)
@@ -204,7 +205,7 @@ class FEMTree:
         o.mg_smooth = int(mg_smooth)
         o.mg_omega = float(mg_omega)
         o.mg_inner_tol = float(mg_inner_tol)
-        o.cg_two_pass = 1 if cg_two_pass else 0
+        o.cg_form = int(cg_form) if cg_form is not None else (L.PRX_CG_TWO_PASS if cg_two_pass else L.PRX_CG_FUSED)
         rep = L.SolveReport()
         check(L.lib().prx_solve_cascadic(self._h, int(base_depth), int(gs), int(cg), float(eps), ctypes.byref(o),
                                          ctypes.byref(rep)))

@@ -95,7 +95,7 @@ def test_batch_rejects_what_it_does_not_do(gpu):
     from poissonreconstruction_amd import _lib as L
     o = L.default_solve_opts()
     o.precision = L.PRX_PREC_MIXED
-    o.cg_two_pass = 1  # the batched mixed solve runs the fused CG step only
+    o.cg_form = L.PRX_CG_TWO_PASS  # the batched mixed solve runs the fused CG step only
     rc = L.lib().prx_solve_batched(B._h, 20, 1000, 1e-6, ctypes.byref(o), 0, None, None)
     assert rc == L.PRX_ERR_INVALID
 

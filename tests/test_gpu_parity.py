@@ -248,6 +248,30 @@ def test_fused_cg_step_equals_two_pass(gpu, prec, mode, refresh, pw):
     assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
 
 
+@pytest.mark.parametrize("D,pw,shape,prec", [(6, 4.0, "torus", pr.PRX_PREC_FP64), (7, 4.0, "sphere", pr.PRX_PREC_FP64),
+                                             (6, 4.0, "torus", pr.PRX_PREC_MIXED), (7, 4.0, "sphere", pr.PRX_PREC_MIXED),
+                                             (6, 0.0, "torus", pr.PRX_PREC_MIXED)])
+def test_fused_cg_face_form_equals_stored_q(gpu, D, pw, shape, prec):
+    # The FP32 fused step in face form (q = A d recomputed inside each 16^3 brick, read on the brick's boundary from
+    # the neighbours' faces, screening values formed from the brick's tile) against the same step with q stored as a
+    # vector and the screening values from the per-point gather: the same iteration, other rounding of the screening
+    # term only.  Inner-iteration counts within 2 %, the same FP64 true residual, solutions within the solve accuracy.
+    pts = unit_points(shape, 20000, seed=11)
+    sols, reps = [], []
+    for form in (pr.PRX_CG_FUSED_STORED_Q, pr.PRX_CG_FUSED_FACES):
+        t = make_tree(D, pts, pw)
+        rep = t.solve_cascadic(pts, pw, D - 2, 20, 40000, 1e-8, precision=prec,
+                               residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, cg_form=form)
+        assert rep["status"] == 0 and rep["true_rel"] <= 1e-8, (form, rep)
+        sols.append(t.solution)
+        reps.append(rep)
+    print(f"stored-q {reps[0]['fine_iters']} iters {reps[0]['outer_iters']} outer, face form {reps[1]['fine_iters']} "
+          f"iters {reps[1]['outer_iters']} outer", flush=True)
+    assert abs(int(reps[0]["fine_iters"]) - int(reps[1]["fine_iters"])) <= max(3, int(reps[0]["fine_iters"]) // 50)
+    rng_f = sols[0].max() - sols[0].min()
+    assert np.abs(sols[0] - sols[1]).max() <= 1e-5 * rng_f
+
+
 @pytest.mark.parametrize("cap,refresh", [(37, 0), (37, 10), (45, 10), (8, 0), (40, 10)])
 def test_fused_cg_stops_at_the_cap_with_x_terms_pending(gpu, cap, refresh):
     # Stopped by the iteration cap (the reference's cg_iters / 4 on the fine level, src/fem_tree.rs:458-487) after
