@@ -847,32 +847,19 @@ __global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> i
     demorton3(lb, ox, oy, oz);
     ox *= B; oy *= B; oz *= B;
 
-    // ---- 0. the brick's touched nodes (screening): the first 4*NT with their
-    // q = A d_old and the screening sums of d'
+    // the brick's touched nodes (screening)
     uint32_t st0 = 0, st1 = 0;
     if (L.has_screen) {
         st0 = L.brick[bid];
         st1 = L.brick[bid + 1];
     }
-    uint32_t nl[4];
-    T qo[4], sn[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t t = st0 + k * NT + tid;
-        const bool in = t < st1;
-        nl[k] = in ? (uint32_t)(L.node[t] - base) : 0u;
-        qo[k] = in && rp ? io.qt[base + nl[k]] : T(0);
-        sn[k] = in ? io.sc[t] : T(0);
-    }
 
-    // ---- 1. interior r and d_old (16-byte, coalesced)
+    // ---- 1. interior d_old (16-byte, coalesced); the interior r is loaded
+    // after the first sweep, which does not need it (fewer registers live
+    // across the sweep)
     NV ra[NVEC], da[NVEC];
 #pragma unroll
-    for (int v = 0; v < NVEC; ++v) {
-        const uint64_t g = base + (uint32_t)(v * NT + tid) * VEC;
-        ra[v] = ld_v(io.r + g);
-        da[v] = ld_v(io.dold + g);
-    }
+    for (int v = 0; v < NVEC; ++v) da[v] = ld_v(io.dold + base + (uint32_t)(v * NT + tid) * VEC);
     const T r0d = io.dold[base];
     // ---- 2. halo r, d_old from the neighbour bricks (the OP_CG tile's face /
     // edge / corner split) and, when an update is pending, q from the
@@ -1017,31 +1004,43 @@ __global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> i
         }
     };
     // q = A d_old at the touched nodes: the values the previous pass wrote
+    // (4 nodes per thread with all loads issued first)
     auto touched_from_qt = [&]() {
+        for (uint32_t t0 = st0; t0 < st1; t0 += 4 * NT) {
+            uint32_t loc[4];
+            T v[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (st0 + k * NT + tid < st1) stg[stage_at<B>(nl[k])] = qo[k];
-        for (uint32_t t = st0 + 4 * NT + tid; t < st1; t += NT) {
-            const uint32_t loc = (uint32_t)(L.node[t] - base);
-            stg[stage_at<B>(loc)] = io.qt[base + loc];
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t t = t0 + k * NT + tid;
+                loc[k] = t < st1 ? (uint32_t)(L.node[t] - base) : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = t0 + k * NT + tid < st1 ? io.qt[base + loc[k]] : T(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (t0 + k * NT + tid < st1) stg[stage_at<B>(loc[k])] = v[k];
         }
     };
     // S d' at the touched nodes added to stg, and the results to io.qt
     auto screening = [&]() {
+        for (uint32_t t0 = st0; t0 < st1; t0 += 4 * NT) {
+            uint32_t loc[4];
+            T sv[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (st0 + k * NT + tid < st1) {
-                const uint32_t at = stage_at<B>(nl[k]);
-                const T v = stg[at] + sn[k];
-                stg[at] = v;
-                io.qt[base + nl[k]] = v;
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t t = t0 + k * NT + tid;
+                const bool in = t < st1;
+                loc[k] = in ? (uint32_t)(L.node[t] - base) : 0u;
+                sv[k] = in ? io.sc[t] : T(0);
             }
-        for (uint32_t t = st0 + 4 * NT + tid; t < st1; t += NT) {
-            const uint32_t loc = (uint32_t)(L.node[t] - base);
-            const uint32_t at = stage_at<B>(loc);
-            const T v = stg[at] + io.sc[t];
-            stg[at] = v;
-            io.qt[base + loc] = v;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (t0 + k * NT + tid < st1) {
+                    const uint32_t at = stage_at<B>(loc[k]);
+                    const T v = stg[at] + sv[k];
+                    stg[at] = v;
+                    io.qt[base + loc[k]] = v;
+                }
         }
     };
 
@@ -1049,6 +1048,10 @@ __global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> i
     // r' = r - alpha q and d' = r' + beta d_old on the interior
     T rn[PER], dn[PER];
     T r0n;
+    auto load_r = [&]() {
+#pragma unroll
+        for (int v = 0; v < NVEC; ++v) ra[v] = ld_v(io.r + base + (uint32_t)(v * NT + tid) * VEC);
+    };
     if (rp) {
 #pragma unroll
         for (int v = 0; v < NVEC; ++v)
@@ -1057,6 +1060,7 @@ __global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> i
                 tile[lbase + lds_lin<E>((uint32_t)(v * NT * VEC)) + lds_lin<E>((uint32_t)e)] = (T)da[v][e] - r0d;
         __syncthreads();
         sweep(r0d);
+        load_r();
         __syncthreads();
         if (L.has_screen) touched_from_qt();
         __syncthreads();
@@ -1071,6 +1075,7 @@ __global__ __launch_bounds__(256) void k_cg_faces(LevelArgs<T> L, StencilIO<T> i
             }
         }
     } else {
+        load_r();
         r0n = dir_form(bt, r0d, io.r[base]);
 #pragma unroll
         for (int v = 0; v < NVEC; ++v)
