@@ -215,6 +215,7 @@ def main():
         log(f"field error vs FP64 1e-12 solve: {field}")
 
     roofline = operator_roofline(L, tree, N, P, args.op_reps)
+    sizes = screening_sizes(L, tree)
 
     adjoint = None if args.skip_adjoint else adjoint_bench(pr, L, synthetic, args.tol)
     del tree
@@ -249,7 +250,7 @@ def main():
         "field_error": field or None,
         "multigrid": mg,
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
-        "cg_phase": cg_phase_bandwidth(N, P, rep_mixed, fp64[-1][1] if fp64 else None),
+        "cg_phase": cg_phase_bandwidth(N, P, rep_mixed, fp64[-1][1] if fp64 else None, sizes),
         "roofline": roofline,
         "marching_cubes": mc,
         "adjoint": adjoint,
@@ -464,39 +465,72 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
     return out
 
 
-def cg_phase_bandwidth(N, P, rep_mixed, rep_fp64):
+def face_pass_bytes(N, T):
+    """Algorithmic bytes of one FP32 face-form CG pass (k_cg_faces): r and
+    d_old read and r', d' written (16N), the faces of q written (1536 values
+    per 4096-node brick) and read as the halo (1736 per brick), and q at the
+    T touched nodes read and written plus their screening sums read (12T)."""
+    nb = N // 4096
+    return 16 * N + 4 * nb * (1536 + 1736) + 12 * T
+
+
+def screening_bytes(P, T, C):
+    """Algorithmic bytes of the face form's per-iteration screening kernels:
+    per occupied cell its 8 corners' r, d and q (96) and its 8 corner sums
+    written (32), the points' in-cell offsets (12P), and per touched node its
+    offsets (8) and sum (4) plus 8 bytes per (cell, corner) pair (at most 8C)."""
+    return 128 * C + 12 * P + 12 * T + 64 * C
+
+
+def cg_phase_bandwidth(N, P, rep_mixed, rep_fp64, sizes):
     """Effective bandwidth of the fine CG phase: the iterations' algorithmic
-    bytes over the phase's time.  One fused CG iteration moves 24 bytes per
-    node in the operator pass (r, q, d read; r, d, q written) plus 5 for x
-    (eight directions and x read, x written, every eighth iteration), and the
-    points' positions (12 bytes each) for the screening; twice that in FP64.
-    The phase time also holds the mixed solve's FP64 refinement passes and the
-    folds, so this is a lower bound on what the kernels sustain."""
+    bytes over the phase's time.  A mixed (FP32) iteration is the face-form
+    pass, its screening kernels and 5 bytes per node for x (eight directions
+    and x read, x written, every eighth iteration); an FP64 iteration is the
+    stored-q fused pass (48 bytes per node: r, q, d read, r, d, q written),
+    10 for x and the points' offsets (24P).  The phase time also holds the
+    mixed solve's FP64 refinement passes and the folds, so this is a lower
+    bound on what the kernels sustain."""
+    T, M, C = sizes
     out = {}
     for name, rep, w in (("mixed", rep_mixed, 4), ("fp64", rep_fp64, 8)):
         if rep is None or not rep.get("ms_fine_cg"):
             continue
-        per_iter = (29 * N + 12 * P) * w // 4
+        per_iter = (face_pass_bytes(N, T) + screening_bytes(P, T, C) + 5 * N if w == 4 else 58 * N + 24 * P)
         gbs = rep["fine_iters"] * per_iter / (rep["ms_fine_cg"] * 1e-3) / 1e9
         out[name] = {"bytes_per_iter": per_iter, "iters": rep["fine_iters"], "ms": round(rep["ms_fine_cg"], 2),
                      "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     return out
 
 
+def screening_sizes(L, tree):
+    """(touched nodes, screening entries, occupied cells) of the tree's points"""
+    out = (ctypes.c_int64 * 3)()
+    L.check(L.lib().prx_screening_sizes(tree._h, out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
 def operator_roofline(L, tree, N, P, reps):
-    """The dominant kernel's achieved HBM bandwidth.  The CG iteration's
-    operator pass is the FP32 fused CG step (OP_CG): it reads r, q and d_old,
-    writes r' = r - alpha q, d_new = r' + beta d_old and q' = A d_new (24N
-    bytes) and reads the P screening points' positions once (12P).  Launches
-    are timed with hipEvents on the library's stream and averaged over
-    `reps`.  The two-pass form's direction pass (16N + 12P), the plain apply
-    y = A x (8N + 12P) and the FP64 forms are reported beside it."""
+    """The dominant kernel's achieved HBM bandwidth.  The mixed solve's CG
+    iterations run the FP32 fused CG step in face form (k_cg_faces, about 75%
+    of the solve): it reads r and d_old, writes r' = r - alpha q and
+    d' = r' + beta d_old, recomputes q = A d_old inside each 16^3 brick, reads
+    q on the brick's halo from the neighbours' faces and writes the faces of
+    q' = A d' (face_pass_bytes).  Launches are timed with hipEvents on the
+    library's stream and averaged over `reps`; its screening kernels
+    (screening_ms) run before it.  The same step with q stored (the FP64
+    solve's form: r, q, d read and written, 24N + 12P in FP32), the two-pass
+    form's direction pass, the plain apply and their FP64 forms are reported
+    beside it, with the brick-by-brick copy ceilings of their access
+    patterns (k streams in, k out, one 4096-node brick per workgroup)."""
+    T, M, C = screening_sizes(L, tree)
+
     def timed(prec, op):
         g, k = ctypes.c_double(0), ctypes.c_double(0)
         L.check(L.lib().prx_time_stencil_op(tree._h, prec, op, reps, ctypes.byref(g), ctypes.byref(k)))
         return g.value, k.value
-    gc, kc = timed(L.PRX_PREC_MIXED, 2)
     gf, kf = timed(L.PRX_PREC_MIXED, 3)
+    gc, kc = timed(L.PRX_PREC_MIXED, 2)
     gd, kd = timed(L.PRX_PREC_MIXED, 1)
     ga, ka = timed(L.PRX_PREC_MIXED, 0)
     gc64, kc64 = timed(L.PRX_PREC_FP64, 2)
@@ -506,15 +540,16 @@ def operator_roofline(L, tree, N, P, reps):
     copy_bytes = 2 * 4 * N
     L.check(L.lib().prx_copy_bandwidth(copy_bytes, reps, 0, ctypes.byref(cms)))
     copy_gbs = copy_bytes / (cms.value * 1e-3) / 1e9
-    # the fused step's access pattern without arithmetic: 3 vectors in, 3 out, one brick per workgroup
     pattern = {}
     for prec, name, esz in ((L.PRX_PREC_MIXED, "fp32", 4), (L.PRX_PREC_FP64, "fp64", 8)):
-        for streams in (1, 3):
+        for streams in (1, 2, 3):
             bms = ctypes.c_double(0)
             L.check(L.lib().prx_brick_copy_bandwidth(N // 4096, streams, prec, reps, 0, ctypes.byref(bms)))
             pattern[f"{name}_{streams}x{streams}_GBs"] = round(2 * streams * N * esz / (bms.value * 1e-3) / 1e9, 1)
+    face_bytes = face_pass_bytes(N, T)
     cg_bytes, dir_bytes, apply_bytes = 24 * N + 12 * P, 16 * N + 12 * P, 8 * N + 12 * P
-    achieved = cg_bytes / (kc * 1e-3) / 1e9
+    achieved = face_bytes / (kf * 1e-3) / 1e9
+
     def measured_traffic(stem):
         # HBM-side bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE at depth 9
         tpath = os.path.join(ROOT, "profiles", f"{stem}_depth9_torus1M_traffic.json")
@@ -523,25 +558,25 @@ def operator_roofline(L, tree, N, P, reps):
         with open(tpath) as fh:
             return int(json.load(fh)["traffic_bytes_per_launch"])
 
-    traffic = measured_traffic("stencil_cg_step")
     gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
     sub = lambda kern, b, ms, g: {"kernel": kern, "bytes": b, "ms": round(ms, 4), "GBs": gbs(b, ms),
                                   "frac": round(gbs(b, ms) / HBM_PEAK_GBS, 4), "gather_ms": round(g, 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_stencil_tile<float,16,OP_CG>", "bytes_per_launch": cg_bytes,
-                "ms_per_launch": round(kc, 4), "gather_ms": round(gc, 4),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("cg_faces"),
+                "kernel": "k_cg_faces<float>", "bytes_per_launch": face_bytes,
+                "ms_per_launch": round(kf, 4), "screening_ms": round(gf, 4),
+                "screening_bytes": screening_bytes(P, T, C),
+                "sizes": {"nodes": N, "bricks": N // 4096, "points": P, "touched_nodes": T, "entries": M,
+                          "occupied_cells": C},
                 "frac_of_copy": round(achieved / copy_gbs, 4),
                 "brick_copy": pattern,
-                "frac_of_pattern_ceiling": round(achieved / pattern["fp32_3x3_GBs"], 4),
+                "frac_of_pattern_ceiling": round(achieved / pattern["fp32_2x2_GBs"], 4),
+                "cg_step_stored_q_fp32": dict(sub("k_stencil_tile<float,16,OP_CG>", cg_bytes, kc, gc),
+                                              traffic=measured_traffic("stencil_cg_step"),
+                                              frac_of_pattern_ceiling=round(gbs(cg_bytes, kc) / pattern["fp32_3x3_GBs"], 4)),
                 "cg_step_fp64": dict(sub("k_stencil_tile<double,16,OP_CG>", 48 * N + 24 * P, kc64, gc64),
                                      traffic=measured_traffic("stencil_cg_step_fp64"),
-                                     frac_of_pattern_ceiling=round((48 * N + 24 * P) / (kc64 * 1e-3) / 1e9
-                                                                   / pattern["fp64_3x3_GBs"], 4)),
-                # the face form: r, d_old in and r', d' out (16N), the bricks' faces of q in and out
-                # (2 x 1536 values per 4096 nodes = 3N); gather_ms = its per-point gather and
-                # per-touched-node screening sums
-                "cg_step_faces_fp32": sub("k_cg_faces<float>", 19 * N, kf, gf),
+                                     frac_of_pattern_ceiling=round(gbs(48 * N + 24 * P, kc64) / pattern["fp64_3x3_GBs"], 4)),
                 "dir_apply_fp32": sub("k_stencil_tile<float,16,OP_DIR_APPLY>", dir_bytes, kd, gd),
                 "dir_apply_fp64": sub("k_stencil_tile<double,16,OP_DIR_APPLY>", 32 * N + 24 * P, kd64, gd64),
                 "apply_fp32": sub("k_stencil_tile<float,16,OP_APPLY>", apply_bytes, ka, ga),

@@ -288,6 +288,10 @@ int prx_brick_copy_bandwidth(int64_t nbricks, int streams, int precision, int re
  * reference pushes in assemble_system (src/fem_tree.rs:199-227); they are
  * built once per point set and reused by every assembly and solve. */
 int prx_splat_times(prx_tree* t, double* ms);
+/* Sizes of the screening data of the last splat (measurement; no
+ * counterpart in the reference): out[3] = {touched nodes, (point, corner)
+ * entries, occupied trunc cells}. */
+int prx_screening_sizes(prx_tree* t, int64_t* out);
 /* Measurement: evaluate the current solution at every marching-cubes corner
  * (as prx_evaluate_corners, src/fem_tree.rs:596-605) into a device buffer
  * `reps` times; *ms = average ms per evaluation, *sumsq = sum of the squared

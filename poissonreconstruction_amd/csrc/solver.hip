@@ -1487,6 +1487,16 @@ int prx_splat_normal_field(prx_tree* t, const prx_point* pts, size_t n) {
     PRX_API_END
 }
 
+int prx_screening_sizes(prx_tree* t, int64_t* out) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t && out, PRX_ERR_INVALID, "null pointer");
+    PRX_REQUIRE(t->splatted, PRX_ERR_STATE, "splat_normal_field not called");
+    out[0] = t->trunc_list.T;
+    out[1] = t->trunc_list.M;
+    out[2] = t->cells_ok ? t->cells.C : 0;
+    PRX_API_END
+}
+
 int prx_splat_times(prx_tree* t, double* ms) {
     PRX_API_BEGIN
     PRX_REQUIRE(t && ms, PRX_ERR_INVALID, "null pointer");
