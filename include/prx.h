@@ -123,13 +123,14 @@ typedef struct prx_solve_opts {
      * iteration): PRX_CG_FUSED (default) one fused pass per iteration on
      * grids of depth >= 3 (the residual update r -= alpha q joins the next
      * operator pass, whose |r|^2 for the stopping test and beta is expanded
-     * from the pass's d.q, r.q, q.q and r.r), in FP32 on grids of depth >= 4
+     * from the pass's d.q, r.q, q.q and r.r), in FP32 on grids of depth >= 8
      * without storing q = A d (the pass recomputes it inside each 16^3 brick
-     * and reads it on the brick's boundary from the neighbours' faces);
+     * and reads it on the brick's boundary from the neighbours' faces; batched
+     * solves keep q stored);
      * PRX_CG_TWO_PASS the two-pass form (operator pass, then the vector
      * update with its own |r|^2); PRX_CG_FUSED_STORED_Q the fused pass with
      * q stored as a full vector in every precision; PRX_CG_FUSED_FACES the
-     * face form in every precision where it applies */
+     * face form in every precision on grids of depth >= 4, batches included */
     int32_t cg_form;
 } prx_solve_opts;
 
@@ -166,6 +167,11 @@ void prx_mesh_free(prx_mesh* mesh);
  * tree to `depth`, nodes numbered in Morton order. */
 int prx_tree_create(uint32_t depth, int device, prx_tree** out);
 void prx_tree_destroy(prx_tree* t);
+/* Frees the solver workspace a tree keeps between solves (CG vectors,
+ * multigrid levels); the solution, points and lists stay and the next solve
+ * allocates again.  No counterpart in the reference (for callers holding
+ * many trees). */
+int prx_tree_release_workspace(prx_tree* t);
 /* fem_node_count (src/fem_tree.rs:54) */
 int64_t prx_tree_node_count(const prx_tree* t);
 /* idx_to_offset [3N] and grid_to_idx [res^3], row-major z*res*res+y*res+x

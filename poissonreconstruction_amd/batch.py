@@ -63,7 +63,7 @@ class Batch:
 
     def solve(self, gs: int = 20, cg_iters: int = 800000, eps: float = 1e-6, residual_mode: int = L.PRX_RESID_TRUE,
               refresh_period: int = 10, autotune: bool = True, precond: int = L.PRX_PRECOND_NONE,
-              precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False):
+              precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False, cg_form: int = None):
         """Returns (batch report dict, list of per-instance report dicts).
         The batched cascadic solve runs in `precision` (FP64, or mixed with the
         fused CG step); precond PRX_PRECOND_MG solves every instance with
@@ -74,7 +74,7 @@ class Batch:
         o.residual_mode = int(residual_mode)
         o.refresh_period = int(refresh_period)
         o.precond = int(precond)
-        o.cg_form = L.PRX_CG_TWO_PASS if cg_two_pass else L.PRX_CG_FUSED
+        o.cg_form = int(cg_form) if cg_form is not None else (L.PRX_CG_TWO_PASS if cg_two_pass else L.PRX_CG_FUSED)
         rep = L.BatchReport()
         per = (L.SolveReport * max(len(self._meta), 1))()
         check(L.lib().prx_solve_batched(self._h, int(gs), int(cg_iters), float(eps), ctypes.byref(o), 1 if autotune else 0,

@@ -75,8 +75,17 @@ struct BLevel {
     DArr<uint32_t> cell, pinst;
     DArr<double> t64;
     DArr<float> t32;
+    // the face-form FP32 pass: the instances' occupied cells and (cell, corner)
+    // pairs concatenated (global offsets), and [N] q at the touched nodes,
+    // [T] screening sums per touched node, [8C] sums per cell corner
+    CellLists cells;
+    bool has_cells = false;
+    DArr<float> qt32;
     BLevel() { scr.mode = 1; }
-    ~BLevel() { free_list(scr); }
+    ~BLevel() {
+        free_list(scr);
+        free_cell_lists(cells);
+    }
     BPoints pts() const {
         BPoints p;
         p.P = P;
@@ -92,7 +101,7 @@ struct BLevel {
 struct GraphEntry {
     const BLevel* L;
     int kind, chunk, refresh, tile, gsb, prec;
-    bool fused;
+    bool fused, faces;
     hipGraphExec_t exec;
 };
 
@@ -122,6 +131,7 @@ struct prx_batch {
     std::vector<int32_t> flags_h;
     int tile = 16, chunk = 16, gs_block = 16;  // gs_block: brick edge of the GS passes
     bool fused = true;                         // the fused CG step (prx_solve_opts.cg_form != PRX_CG_TWO_PASS)
+    bool faces = false;                        // FP32 fused step in face form (cg_form PRX_CG_FUSED_FACES)
     bool tuned = false;
     double tune_ms = 0.0;
     uint64_t graph_launches = 0;
@@ -206,13 +216,19 @@ static void release_cg(BLevel& L) {
     L.q.release();
 }
 // the mixed fine solve's vectors (the FP64 CG vectors of the level are not needed then)
-static void ensure_cg32(BLevel& L) {
+static void ensure_cg32(BLevel& L, hipStream_t s) {
     L.e32.ensure(L.N);
     L.r32.ensure(2 * L.N);
     L.dd32.ensure(CG_XM * L.N);
     L.q32.ensure(2 * L.N);
     L.x2.ensure(L.N);
     L.sbuf32.ensure(std::max<int64_t>(L.P, 1));
+    if (L.has_cells && !L.qt32.p) {
+        // q at the touched nodes (zero elsewhere), the sums per touched node and per cell corner
+        const int64_t n = L.N + std::max<int64_t>(L.scr.T, 1) + 8 * std::max<int64_t>(L.cells.C, 1);
+        L.qt32.ensure(n);
+        launch_fill<float>(L.qt32.p, 0.0f, n, s);
+    }
 }
 static void release_cg32(BLevel& L) {
     L.e32.release();
@@ -221,6 +237,51 @@ static void release_cg32(BLevel& L) {
     L.q32.release();
     L.x2.release();
     L.sbuf32.release();
+    L.qt32.release();
+}
+
+// Concatenate the instances' occupied-cell lists (face-form screening) in the
+// order of the screening lists: cell starts offset by the instance's first
+// sorted point, pair ranges by the pairs before it, pairs by 8 x its first cell.
+static void build_cells(prx_batch* B, BLevel& L) {
+    const hipStream_t s = B->stream;
+    free_cell_lists(L.cells);
+    L.has_cells = false;
+    int64_t C = 0, MC = 0;
+    std::vector<int64_t> mc(L.K, 0);
+    for (int k = 0; k < L.K; ++k) {
+        prx_tree* t = B->inst[k];
+        if (!(t->pw > 0.0 && t->trunc_list.T > 0)) continue;
+        if (!t->cells_ok) return;  // no cell lists for this instance: the batch keeps the stored-q pass
+        uint32_t m = 0;
+        PRX_HIP(hipMemcpy(&m, t->cells.coff + t->trunc_list.T, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        mc[k] = m;
+        C += t->cells.C;
+        MC += m;
+    }
+    if (C == 0 || C >= ((int64_t)1 << 28) || MC >= ((int64_t)1 << 32)) return;
+    CellLists& G = L.cells;
+    G.C = C;
+    dmalloc_n(&G.cstart, C + 1);
+    dmalloc_n(&G.coff, L.scr.T + 1);
+    dmalloc_n(&G.cval, MC);
+    int64_t Co = 0, MCo = 0, Po = 0, To = 0;
+    for (int k = 0; k < L.K; ++k) {
+        prx_tree* t = B->inst[k];
+        if (!(t->pw > 0.0 && t->trunc_list.T > 0)) continue;
+        const CellLists& S = t->cells;
+        copy_add_u32(G.cstart + Co, S.cstart, S.C, (uint32_t)Po, s);
+        copy_add_u32(G.coff + To, S.coff, t->trunc_list.T, (uint32_t)MCo, s);
+        copy_add_u32(G.cval + MCo, S.cval, mc[k], (uint32_t)(8 * Co), s);
+        Co += S.C;
+        MCo += mc[k];
+        Po += t->P;
+        To += t->trunc_list.T;
+    }
+    fill_u32(G.cstart + C, 1, (uint32_t)Po, s);
+    fill_u32(G.coff + To, 1, (uint32_t)MCo, s);
+    PRX_HIP(hipStreamSynchronize(s));
+    L.has_cells = true;
 }
 
 // Concatenate the per-instance screening lists and point data (global node
@@ -286,6 +347,7 @@ static void build_screening(prx_batch* B, BLevel& L) {
     fill_u32(G.brick + (L.N >> 12), 1, (uint32_t)T, s);
     fill_u32(G.brick8 + (L.N >> 9), 1, (uint32_t)T, s);
     PRX_HIP(hipStreamSynchronize(s));
+    build_cells(B, L);
 }
 
 static void build(prx_batch* B) {
@@ -379,6 +441,13 @@ static void gs_enqueue(prx_batch* B, BLevel& L, int sweeps) {
         }
 }
 
+// The FP32 fused step runs in face form on 16^3 bricks when the level has its
+// cell lists (or no screening) and the options ask for it.
+template <typename T>
+static bool uses_faces(const prx_batch* B, const BLevel& L) {
+    return sizeof(T) == 4 && B->faces && B->tile == 16 && (!L.has_scr || (L.has_cells && L.qt32.p));
+}
+
 // One fused CG step (pass index k) of every running instance, as
 // Cg::enqueue_fused: pass k reads r, q from slot k&1 and the direction from
 // ring slot k % CG_XM and writes slot (k+1)&1 and ring slot (k+1) % CG_XM.
@@ -406,11 +475,25 @@ static void cg_enqueue_fused(prx_batch* B, BLevel& L, uint64_t k, int refresh, i
     io.dnew = dnew;
     io.y = V.q + (int64_t)((k + 1) & 1) * N;
     io.partial = B->partial.p;
-    if (L.has_scr) {
-        launch_bscreen_gather<T>(tab, L.pts(), nullptr, nullptr, nullptr, V.sbuf, io.r, dold, s, io.q);
-        io.s = V.sbuf;
+    if (uses_faces<T>(B, L)) {
+        // the face form: io.q / io.y hold the bricks' faces of q, q at the touched nodes lives in qt32
+        if (L.has_scr) {
+            float* qt = L.qt32.p;
+            float* sc = qt + N;
+            float* sig = sc + std::max<int64_t>(L.scr.T, 1);
+            launch_bcell_screening<float>(tab, L.cells, L.pts(), L.scr.T, L.scr.node, (const float*)io.r, (const float*)dold, qt, sig,
+                                          sc, s);
+            io.sc = (const T*)sc;
+            io.qt = (T*)qt;
+        }
+        launch_bcg_faces<T>(tab, N, L.scr_ptr(), io, s);
+    } else {
+        if (L.has_scr) {
+            launch_bscreen_gather<T>(tab, L.pts(), nullptr, nullptr, nullptr, V.sbuf, io.r, dold, s, io.q);
+            io.s = V.sbuf;
+        }
+        launch_bstencil<T>(tab, N, L.scr_ptr(), OP_CG, io, B->tile, s);
     }
-    launch_bstencil<T>(tab, N, L.scr_ptr(), OP_CG, io, B->tile, s);
     const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
     launch_bfin_cg(tab, B->partial.p, ulog_tile(B->tile), B->fold.p, fresh ? 1 : 0, s);
     xn = xn == CG_XM ? 1 : xn + 1;
@@ -463,7 +546,7 @@ static void cg_enqueue(prx_batch* B, BLevel& L, uint64_t k, int refresh) {
 static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int refresh, int prec = 0) {
     for (auto& g : B->graphs)
         if (g.L == &L && g.kind == kind && g.chunk == n && g.refresh == refresh && g.tile == B->tile && g.gsb == B->gs_block &&
-            g.fused == B->fused && g.prec == prec)
+            g.fused == B->fused && g.faces == B->faces && g.prec == prec)
             return g.exec;
     hipGraph_t graph;
     PRX_HIP(hipStreamSynchronize(B->stream));
@@ -496,7 +579,7 @@ static hipGraphExec_t graph_for(prx_batch* B, BLevel& L, int kind, int n, int re
     const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     PRX_HIP(e);
-    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, prec, B->fused, exec});
+    B->graphs.push_back({&L, kind, n, refresh, B->tile, B->gs_block, prec, B->fused, B->faces, exec});
     return exec;
 }
 
@@ -1000,6 +1083,9 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     }
     build(B);
     B->fused = o.cg_form != PRX_CG_TWO_PASS;
+    // measured on the batching workload (depth 6-8 instances), the face form is 3% slower than the stored-q
+    // step, so batches use it only when asked for explicitly
+    B->faces = o.cg_form == PRX_CG_FUSED_FACES;
     const hipStream_t s = B->stream;
     BLevel& F = B->fine;
     BLevel& C = B->coarse;
@@ -1011,7 +1097,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     auto fine_vectors = [&](bool want64, bool want32) {
         bool changed = false;
         if (want64 != (F.r.p != nullptr)) { want64 ? ensure_cg(F) : release_cg(F); changed = true; }
-        if (want32 != (F.e32.p != nullptr)) { want32 ? ensure_cg32(F) : release_cg32(F); changed = true; }
+        if (want32 != (F.e32.p != nullptr)) { want32 ? ensure_cg32(F, B->stream) : release_cg32(F); changed = true; }
         if (changed) B->drop_graphs();
     };
     if (!C.r.p) {

@@ -741,6 +741,89 @@ void launch_cell_screening(const LevelView& L, const CellLists& CL, const T* r, 
 }
 template void launch_cell_screening<double>(const LevelView&, const CellLists&, const double*, const double*, const double*,
                                             const CgState*, const int32_t*, double*, double*, hipStream_t);
+
+// The same over a batch: cell c belongs to the instance of its first point;
+// the instance's resolution, first node, wh3 and CG state apply, and cells of
+// instances that do not run are skipped.
+template <typename T>
+__global__ void k_bcell_gather(BatchTab tab, int64_t C, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cell,
+                               const T* __restrict__ tt, const uint32_t* __restrict__ pinst, int64_t P, const T* __restrict__ r,
+                               const T* __restrict__ dd, const T* __restrict__ qq, T* sig) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t p0 = cstart[c], p1 = cstart[c + 1], k = pinst[p0];
+    if (tab.run[(int64_t)k * tab.run_stride] != tab.run_when) return;
+    const BCoef cf = tab.coef[k];
+    const CgState& st = tab.st[k];
+    const double bt = st.beta, al = st.alpha;
+    const int rp = st.rpend;
+    const uint64_t ib = tab.off[k];
+    T wh3;
+    if constexpr (sizeof(T) == 8) wh3 = (T)cf.wh3_64;
+    else wh3 = (T)cf.wh3_32;
+    const uint32_t res = cf.res;
+    const uint32_t cx = cell[p0], cy = cell[P + p0], cz = cell[2 * P + p0];
+    bool in[8];
+    uint64_t m[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t ox = cx + (j & 1), oy = cy + ((j >> 1) & 1), oz = cz + (j >> 2);
+        in[j] = ox < res && oy < res && oz < res;
+        m[j] = in[j] ? ib + morton3(ox, oy, oz) : 0u;
+    }
+    T v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = in[j] ? cg_form(bt, al, rp, dd[m[j]], r[m[j]], qq[m[j]]) : T(0);
+    T sg[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sg[j] = T(0);
+    for (uint32_t p = p0; p < p1; ++p) {
+        const T t0 = tt[p], t1 = tt[P + p], t2 = tt[2 * P + p];
+        T w[8], acc = T(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const T bx = (j & 1) ? t0 : T(1) - t0;
+            const T by = ((j >> 1) & 1) ? t1 : T(1) - t1;
+            const T bz = (j >> 2) ? t2 : T(1) - t2;
+            w[j] = in[j] ? bx * by * bz : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += w[j] * v[j];
+        const T sp = wh3 * acc;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sg[j] += w[j] * sp;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sig[8 * c + j] = sg[j];
+}
+// k_cell_sums over a batch: touched nodes of instances that do not run are skipped
+template <typename T>
+__global__ void k_bcell_sums(BatchTab tab, int64_t Tn, const uint32_t* __restrict__ node, const uint32_t* __restrict__ coff,
+                             const uint32_t* __restrict__ cval, const T* __restrict__ sig, T* sc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Tn) return;
+    const uint32_t k = tab.brick_inst[node[t] >> 12];
+    if (tab.run[(int64_t)k * tab.run_stride] != tab.run_when) return;
+    T acc = T(0);
+    for (uint32_t j = coff[t]; j < coff[t + 1]; ++j) acc += sig[cval[j]];
+    sc[t] = acc;
+}
+template <typename T>
+void launch_bcell_screening(const BatchTab& tab, const CellLists& CL, const BPoints& pts, int64_t T_touched, const uint32_t* node,
+                            const T* r, const T* d, const T* q, T* sig, T* sc, hipStream_t s) {
+    if (CL.C <= 0 || T_touched <= 0) return;
+    const T* tt;
+    if constexpr (sizeof(T) == 8) tt = (const T*)pts.t64;
+    else tt = (const T*)pts.t32;
+    hipLaunchKernelGGL((k_bcell_gather<T>), dim3(nblk(CL.C)), dim3(256), 0, s, tab, CL.C, CL.cstart, pts.cell, tt, pts.inst, pts.P,
+                       r, d, q, sig);
+    hipLaunchKernelGGL((k_bcell_sums<T>), dim3(nblk(T_touched)), dim3(256), 0, s, tab, T_touched, node, CL.coff, CL.cval, sig, sc);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bcell_screening<double>(const BatchTab&, const CellLists&, const BPoints&, int64_t, const uint32_t*,
+                                             const double*, const double*, const double*, double*, double*, hipStream_t);
+template void launch_bcell_screening<float>(const BatchTab&, const CellLists&, const BPoints&, int64_t, const uint32_t*,
+                                            const float*, const float*, const float*, float*, float*, hipStream_t);
 template void launch_cell_screening<float>(const LevelView&, const CellLists&, const float*, const float*, const float*,
                                            const CgState*, const int32_t*, float*, float*, hipStream_t);
 

@@ -1430,6 +1430,17 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
     PRX_LAUNCH_CHECK();
 }
 template void launch_bstencil<double>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<double>&, int, hipStream_t);
+
+template <typename T>
+void launch_bcg_faces(const BatchTab& tab, int64_t Ntot, const NodeList* scr, const StencilIO<T>& io, hipStream_t s) {
+    const LevelArgs<T> a = batch_args<T>(scr, 12);
+    if (a.has_screen && !(io.sc && io.qt)) throw Error(PRX_ERR_STATE, "the face form needs sc and qt");
+    const uint32_t nbricks = (uint32_t)(Ntot >> 12);
+    hipLaunchKernelGGL((k_cg_faces<T, true>), dim3(nbricks), dim3(256), 0, s, a, io, nbricks, xcd_chunk(nbricks), tab);
+    PRX_LAUNCH_CHECK();
+}
+template void launch_bcg_faces<double>(const BatchTab&, int64_t, const NodeList*, const StencilIO<double>&, hipStream_t);
+template void launch_bcg_faces<float>(const BatchTab&, int64_t, const NodeList*, const StencilIO<float>&, hipStream_t);
 template void launch_bstencil<float>(const BatchTab&, int64_t, const NodeList*, int, const StencilIO<float>&, int, hipStream_t);
 
 

@@ -259,6 +259,14 @@ void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int
 template <typename T>
 void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
                            const T* r, const T* d, hipStream_t s, const T* q = nullptr);
+// The face form over a batch: the concatenated cell lists (cell indices and
+// (cell, corner) pairs with global offsets, instance of a cell = instance of
+// its first point) and the batched face-form pass (16^3 bricks).
+template <typename T>
+void launch_bcell_screening(const BatchTab& tab, const CellLists& CL, const BPoints& pts, int64_t T_touched, const uint32_t* node,
+                            const T* r, const T* d, const T* q, T* sig, T* sc, hipStream_t s);
+template <typename T>
+void launch_bcg_faces(const BatchTab& tab, int64_t Ntot, const NodeList* scr, const StencilIO<T>& io, hipStream_t s);
 // Per instance: fold its partials [off[k] >> ulog, off[k+1] >> ulog) in the
 // order k_fin uses and apply the same scalar step to st[k].
 void launch_bfin(const BatchTab& tab, const double* partial, int ulog, int op, int slot, hipStream_t s);

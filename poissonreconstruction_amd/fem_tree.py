@@ -80,6 +80,10 @@ class FEMTree:
             L.lib().prx_tree_destroy(self._h)
             self._h = None
 
+    def release_workspace(self) -> None:
+        """Free the solver workspace kept between solves (the next solve allocates again)."""
+        check(L.lib().prx_tree_release_workspace(self._h))
+
     def __del__(self):
         try:
             self.close()

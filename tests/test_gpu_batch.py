@@ -108,12 +108,14 @@ def single_mixed(D, pts, pw, eps):
     return t.solution, rep
 
 
-@pytest.mark.parametrize("autotune", [False, True])
-def test_batch_mixed_matches_single_tree_mixed_solves(gpu, autotune):
+@pytest.mark.parametrize("autotune,form", [(False, pr.PRX_CG_FUSED), (True, pr.PRX_CG_FUSED),
+                                           (False, pr.PRX_CG_FUSED_FACES)])
+def test_batch_mixed_matches_single_tree_mixed_solves(gpu, autotune, form):
     # the batched mixed cascadic solve (FP32 correction CG with FP64 residual
-    # replacements per instance) against each cloud solved alone by the
-    # single-tree mixed solve: both reach the FP64 true residual, and the
-    # solutions agree to the accuracy that residual fixes
+    # replacements per instance; its FP32 step with q stored, or in face form)
+    # against each cloud solved alone by the single-tree mixed solve: both
+    # reach the FP64 true residual, and the solutions agree to the accuracy
+    # that residual fixes
     cs = clouds()
     B = Batch()
     for D, pts, pw in cs:
@@ -121,7 +123,7 @@ def test_batch_mixed_matches_single_tree_mixed_solves(gpu, autotune):
     eps = 1e-9
     for rnd in range(2):  # the second solve replays the captured graphs
         rep, per = B.solve(eps=eps, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=autotune,
-                           precision=pr.PRX_PREC_MIXED)
+                           precision=pr.PRX_PREC_MIXED, cg_form=form)
         assert rep["instances"] == len(cs) and rep["converged"] == len(cs) and rep["status"] == pr.PRX_OK
         assert rep["max_true_rel"] <= eps * 1.0001
         for k, (D, pts, pw) in enumerate(cs):
