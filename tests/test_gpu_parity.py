@@ -375,3 +375,53 @@ def test_reconstruct_end_to_end(gpu):
     assert st.matrix_nnz == rep_o["nnz"]
     mesh0, st0 = pr.reconstruct(np.zeros((0, 6)), params)
     assert st0.fem_nodes == 0 and len(mesh0.vertices) == 0
+
+
+def test_screening_sizes_and_occupied_cells(gpu):
+    # prx_screening_sizes against numpy: the touched nodes are the in-grid corners of the points' trunc cells
+    # (src/fem_tree.rs:207-213), the entries one per (point, in-grid corner); the face-form screening groups the
+    # points, in their spatial order (stable, by the Morton code of the trunc cell clamped to the grid), into runs of
+    # one trunc cell.  Points of the unit-cube transform at scale 1.1 reach 0.05 beyond the cube, so cells outside the
+    # grid share a clamped code and one such cell can form several runs; every in-grid cell forms exactly one.
+    import ctypes
+    D = 6
+    pts = unit_points("torus", 20000, seed=13)
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    out = (ctypes.c_int64 * 3)()
+    L.check(L.lib().prx_screening_sizes(t._h, out))
+    res = 1 << D
+    c = np.trunc(np.stack([pts["px"], pts["py"], pts["pz"]], axis=1) * res).astype(np.int64)
+    corners = np.concatenate([c + np.array([k & 1, (k >> 1) & 1, k >> 2]) for k in range(8)])
+    inside = np.all((corners >= 0) & (corners < res), axis=1)
+    assert out[0] == len(np.unique(corners[inside], axis=0))
+    assert out[1] == int(inside.sum())
+
+    def spread(v):
+        r = np.zeros_like(v)
+        for b in range(10):
+            r = r | (((v >> b) & 1) << (3 * b))
+        return r
+    cl = np.clip(c, 0, res - 1)
+    cs = c[np.argsort(spread(cl[:, 0]) | (spread(cl[:, 1]) << 1) | (spread(cl[:, 2]) << 2), kind="stable")]
+    runs = 1 + int(np.count_nonzero(np.any(cs[1:] != cs[:-1], axis=1)))
+    in_grid = np.all((c >= 0) & (c < res), axis=1)
+    assert out[2] == runs
+    assert runs - len(np.unique(c, axis=0)) <= len(np.unique(c[~in_grid], axis=0)) * 8
+
+
+def test_release_workspace_then_solve_again(gpu):
+    # freeing the solver workspace between solves changes nothing: the next solve allocates again and gives the
+    # same solution bit for bit (the face-form arrays and cell lists included, depth 8)
+    D = 8
+    pts = unit_points("sphere", 30000, seed=14)
+    t = make_tree(D, pts, 4.0)
+    sols = []
+    for release in (False, True):
+        if release:
+            t.release_workspace()
+        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 800000, 1e-8, precision=pr.PRX_PREC_MIXED,
+                               residual_mode=pr.PRX_RESID_TRUE, refresh_period=0)
+        assert rep["status"] == 0
+        sols.append(t.solution)
+    assert np.array_equal(sols[0].view(np.uint64), sols[1].view(np.uint64))
