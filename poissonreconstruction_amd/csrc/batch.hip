@@ -1134,7 +1134,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
             const double hc = 1.0 / (double)(1u << cd);
             const double epw = t->pw * (double)((uint64_t)1 << (2 * (t->D - cd)));
             const double wh3c = epw > 0.0 ? epw * (hc * hc * hc) / 1.5 : 0.0;
-            launch_coarse_rhs(t->D, cd, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, C.b.p + C.off_h[k], s);
+            launch_coarse_rhs(t->D, cd, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, C.b.p + C.off_h[k], &t->floor_list, s);
         }
         launch_fill<double>(C.x.p, 0.0, C.N, s);
         if (gs > 0) launch_graph(B, graph_for(B, C, 1, (int)gs, 0));

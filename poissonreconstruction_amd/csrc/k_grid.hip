@@ -953,9 +953,50 @@ __global__ void k_coarse_rhs(uint32_t D, uint32_t d, const double* __restrict__ 
     if (has_add) s += add;
     rhs[c] = s;
 }
-void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs, hipStream_t s) {
+// The same sums over the splat list's nodes only.  The normal field is +0.0
+// at every other node (k_splat writes only the list's nodes), such a node
+// adds (0 + 0) + 0 = +0 to a sum that is never -0, so leaving it out changes
+// no bit; the list is sorted by node, so its nodes under one coarse cell come
+// in FEM index order.  brick8[b .. b+1] is the list's range of 8^3 brick b.
+__global__ void k_coarse_rhs_list(uint32_t D, uint32_t d, const double* __restrict__ nf, const uint32_t* __restrict__ node,
+                                  const uint32_t* __restrict__ brick8, double add, int has_add, double* rhs) {
+    const uint64_t nc = 1ull << (3 * d), nfine = 1ull << (3 * D);
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int sh = 3 * (int)(D - d);  // log2 of fine nodes per coarse cell
+    const uint64_t f0 = c << sh, f1 = (c + 1) << sh;
+    uint32_t lo = brick8[f0 >> 9], hi = brick8[(f1 + 511) >> 9];
+    if (sh < 9) {  // several coarse cells share the 8^3 brick: narrow to [f0, f1)
+        uint32_t a = lo, b = hi;
+        while (a < b) {
+            const uint32_t m = (a + b) >> 1;
+            if ((uint64_t)node[m] < f0) a = m + 1;
+            else b = m;
+        }
+        lo = a;
+        b = hi;
+        while (a < b) {
+            const uint32_t m = (a + b) >> 1;
+            if ((uint64_t)node[m] < f1) a = m + 1;
+            else b = m;
+        }
+        hi = a;
+    }
+    double s = 0.0;
+    for (uint32_t t = lo; t < hi; ++t) {
+        const uint64_t i = node[t];
+        s += nf[i] + nf[nfine + i] + nf[2 * nfine + i];
+    }
+    if (has_add) s += add;
+    rhs[c] = s;
+}
+void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs, const NodeList* splat, hipStream_t s) {
     int64_t nc = (int64_t)1 << (3 * d);
-    hipLaunchKernelGGL(k_coarse_rhs, dim3(nblocks_for(nc)), dim3(256), 0, s, D, d, nf, add, add > 0.0 ? 1 : 0, rhs);
+    if (splat && splat->brick8 && D >= 3 && d < D)
+        hipLaunchKernelGGL(k_coarse_rhs_list, dim3(nblocks_for(nc)), dim3(256), 0, s, D, d, nf, splat->node, splat->brick8, add,
+                           add > 0.0 ? 1 : 0, rhs);
+    else
+        hipLaunchKernelGGL(k_coarse_rhs, dim3(nblocks_for(nc)), dim3(256), 0, s, D, d, nf, add, add > 0.0 ? 1 : 0, rhs);
     PRX_LAUNCH_CHECK();
 }
 

@@ -347,7 +347,16 @@ __global__ void k_rhs_laplacian(uint32_t D, const double* __restrict__ nf, doubl
 // part instead of 4, no bounds tests, and, since the neighbour order is a
 // sweep along z, each thread forms two z-adjacent nodes of one (x, y) column
 // from the 4 planes they share (21 LDS reads per node instead of 81).
-__global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const double* __restrict__ nf, double* rhs) {
+//
+// The normal field is nonzero only at the splat list's nodes (k_splat writes
+// nothing else), so a brick whose 27 neighbour bricks (itself included) hold
+// no splat node reads only zeros: every part is +-0, the running sum stays
+// +0.0 (the reference's sum starts at +0.0, and +0 + -0 = +0), and the brick
+// is written as +0.0 without reading the field.  act8[b .. b+1] is the splat
+// list's node range of 8^3 brick b (null: no skipping).  On the 1M-point
+// torus 13.7% of the depth-9 bricks have such a neighbour (6.8% at depth 10).
+__global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const double* __restrict__ nf, double* rhs,
+                                                            const uint32_t* __restrict__ act8) {
     constexpr int E = 10, E2 = E * E, E3 = E * E * E;
     __shared__ double v[3][E3];
     const uint32_t res = 1u << D;
@@ -361,6 +370,21 @@ __global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const do
     typedef double d2 __attribute__((ext_vector_type(2)));
     // interior: thread t loads local Morton nodes 2t and 2t+1 (an x pair)
     const uint32_t m0 = 2u * (uint32_t)tid;
+    if (act8) {
+        int near = 0;
+        if (tid < 27) {
+            const int cx = (int)bx + tid % 3 - 1, cy = (int)by + (tid / 3) % 3 - 1, cz = (int)bz + tid / 9 - 1;
+            const int nb = (int)(res >> 3);
+            if (cx >= 0 && cy >= 0 && cz >= 0 && cx < nb && cy < nb && cz < nb) {
+                const uint32_t b = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+                near = act8[b + 1] > act8[b] ? 1 : 0;
+            }
+        }
+        if (!__syncthreads_or(near)) {
+            *(d2*)(rhs + base + m0) = d2{0.0, 0.0};
+            return;
+        }
+    }
     {
         uint32_t lx, ly, lz;
         demorton3(m0, lx, ly, lz);
@@ -458,9 +482,10 @@ __global__ __launch_bounds__(256) void k_rhs_laplacian_tile(uint32_t D, const do
     *(d2*)(rhs + base + m0) = out;
 }
 
-void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s) {
+void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, const NodeList* splat, hipStream_t s) {
     const int64_t N = (int64_t)1 << (3 * D);
-    if (D >= 3) hipLaunchKernelGGL(k_rhs_laplacian_tile, dim3((unsigned)(N >> 9)), dim3(256), 0, s, D, nf, rhs);
+    const uint32_t* act8 = splat && splat->brick8 ? splat->brick8 : nullptr;
+    if (D >= 3) hipLaunchKernelGGL(k_rhs_laplacian_tile, dim3((unsigned)(N >> 9)), dim3(256), 0, s, D, nf, rhs, act8);
     else hipLaunchKernelGGL(k_rhs_laplacian, dim3(nblk(N)), dim3(256), 0, s, D, nf, rhs);
     PRX_LAUNCH_CHECK();
 }

@@ -123,7 +123,7 @@ template <typename T> void launch_xpby_beta(T* d, const T* r, const CgState* st,
 template <typename T> void launch_sub(T* r, const T* b, const T* q, int64_t n, double* partial, int* nblk, const CgState* st, hipStream_t s);
 void launch_cast_scale_d2f(float* dst, const double* src, double scale, int64_t n, hipStream_t s);
 void launch_add_f2d(double* x, const float* e, int64_t n, hipStream_t s);
-void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs_morton, hipStream_t s);
+void launch_coarse_rhs(uint32_t D, uint32_t d, const double* nf, double add, double* rhs_morton, const NodeList* splat, hipStream_t s);
 void launch_prolongate(uint32_t D, uint32_t d, const double* coarse_morton, double* fine, hipStream_t s);
 void launch_morton_to_rowmajor(uint32_t d, const double* src, double* dst, hipStream_t s);
 void launch_rowmajor_to_morton(uint32_t d, const double* src, double* dst, hipStream_t s);
@@ -147,7 +147,7 @@ void build_node_list(NodeList& L, uint32_t D, const uint32_t* sorted_key, const 
                      const DevPoints& pts, const uint32_t* inv_perm, void* scratch, size_t scratch_bytes, hipStream_t s);
 void launch_invert_perm(const uint32_t* perm, int64_t P, uint32_t* inv, hipStream_t s);
 void launch_splat(uint32_t D, const NodeList& L, const DevPoints& pts, double* nf, double* w, hipStream_t s);
-void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, hipStream_t s);
+void launch_rhs_laplacian(uint32_t D, const double* nf, double* rhs, const NodeList* splat, hipStream_t s);
 void launch_rhs_screening(uint32_t D, const NodeList& L, const DevPoints& pts, double wh3, double* rhs, hipStream_t s);
 // s_p over the point's corners of x (x64 + e32 when x64 is given; r + beta*d
 // when r is given).

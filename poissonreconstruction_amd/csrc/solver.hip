@@ -446,7 +446,7 @@ static void coarse_solve(prx_tree* t, uint32_t cdepth, int gs, uint64_t iters, d
     L.N = nc;
     L.k64 = {stiffness_entry(hc, 0, 0, 0), stiffness_entry(hc, 1, 1, 0), stiffness_entry(hc, 1, 1, 1), epw > 0.0 ? wh3c * 1.005 : 0.0, 0.0};
     t->cb.ensure(nc); t->cx.ensure(nc); t->cr.ensure(2 * nc); t->cdd.ensure(CG_XM * nc); t->cq.ensure(2 * nc);
-    launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, s);
+    launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, t->cb.p, &t->floor_list, s);
     launch_fill<double>(t->cx.p, 0.0, nc, s);
     gs_sweeps<double>(t, L, t->cx.p, t->cb.p, gs);
     const bool tru = mode == PRX_RESID_TRUE;
@@ -1144,7 +1144,7 @@ void assemble(prx_tree* t, double pw) {
     t->set_coefficients(pw);
     t->b.ensure(t->N);
     t->x.ensure(t->N);
-    launch_rhs_laplacian(t->D, t->nf.p, t->b.p, s);
+    launch_rhs_laplacian(t->D, t->nf.p, t->b.p, &t->floor_list, s);
     if (pw > 0.0) launch_rhs_screening(t->D, t->trunc_list, t->pts, t->wh3, t->b.p, s);
     launch_fill<double>(t->x.p, 0.0, t->N, s);
     t->ensure_state();
@@ -1698,7 +1698,7 @@ int prx_coarse_system(prx_tree* t, uint32_t cdepth, double* rhs_rowmajor, double
         DArr<double> m, r;
         m.ensure(nc);
         r.ensure(nc);
-        launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, m.p, t->stream);
+        launch_coarse_rhs(t->D, cdepth, t->nf.p, epw > 0.0 ? wh3c * 0.5 : 0.0, m.p, &t->floor_list, t->stream);
         launch_morton_to_rowmajor(cdepth, m.p, r.p, t->stream);
         PRX_HIP(hipMemcpyAsync(rhs_rowmajor, r.p, sizeof(double) * nc, hipMemcpyDeviceToHost, t->stream));
         PRX_HIP(hipStreamSynchronize(t->stream));

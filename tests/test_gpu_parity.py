@@ -65,6 +65,55 @@ def test_rhs_bit_exact(gpu, D, pw):
     assert np.array_equal(bits(t.assemble_rhs(pts, pw)), bits(b))
 
 
+def corner_cloud(n, seed, res):
+    """A small sphere packed into one corner of the cube, plus points whose
+    grid coordinates sit exactly on 8^3 brick faces: most bricks then hold no
+    splat node, and the bricks that do border ones that do not."""
+    raw = synthetic.sphere(n, seed)
+    base = oracle.apply_transform(raw, oracle.unit_transform(raw, 1.1)[0])
+    pts = np.zeros(n + 64, dtype=base.dtype)
+    pts[:n] = base
+    for c in ("px", "py", "pz"):
+        pts[c][:n] = 0.05 + 0.2 * base[c]
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, res // 8, size=(64, 3)) * 8
+    for j, c in enumerate(("px", "py", "pz")):
+        pts[c][n:] = k[:, j] / res
+    for c in ("nx", "ny", "nz"):
+        pts[c][n:] = rng.normal(size=64)
+    nrm = np.sqrt(pts["nx"][n:] ** 2 + pts["ny"][n:] ** 2 + pts["nz"][n:] ** 2)
+    for c in ("nx", "ny", "nz"):
+        pts[c][n:] /= nrm
+    return pts
+
+
+@pytest.mark.parametrize("D,pw", [(7, 4.0), (7, 0.0)])
+def test_rhs_bit_exact_where_most_bricks_hold_no_splat_node(gpu, D, pw):
+    # the RHS pass writes +0.0 without reading the field on bricks with no
+    # splat node among their 27 neighbour bricks; the rest is the same sum
+    pts = corner_cloud(4000, D, 1 << D)
+    g2i, i2o = oracle.tree(D)
+    nf, _ = oracle.splat(D, g2i, pts)
+    b = oracle.rhs(D, g2i, i2o, nf, pts, pw)
+    t = pr.FEMTree(D)
+    t.splat_normal_field(pts)
+    assert np.array_equal(bits(t.assemble_rhs(pts, pw)), bits(b))
+    assert (b.reshape(-1, 512) == 0).all(axis=1).mean() > 0.5  # the skip path is exercised
+
+
+@pytest.mark.parametrize("D,cd", [(6, 3), (7, 3), (7, 5)])
+def test_coarse_rhs_bit_exact_over_the_splat_list(gpu, D, cd):
+    # the coarse RHS sums only the splat list's nodes under each coarse cell:
+    # 8^(D-cd) fine nodes per cell, below, at and above one 8^3 brick
+    pts = corner_cloud(3000, D + cd, 1 << D)
+    g2i, i2o = oracle.tree(D)
+    nf, _ = oracle.splat(D, g2i, pts)
+    _, cb = oracle.coarse_system(D, cd, i2o, nf, 4.0)
+    t = make_tree(D, pts, 4.0)
+    rhs, _ = t.build_system_at_depth(cd)
+    assert np.array_equal(bits(rhs), bits(cb))
+
+
 @pytest.mark.parametrize("D,n,pw", [(1, 20, 4.0), (2, 100, 4.0), (3, 400, 4.0), (4, 1500, 4.0), (3, 400, 0.0)])
 def test_csr_pattern_and_values_bit_exact(gpu, D, n, pw):
     pts = unit_points("sphere", n, seed=D + 10)
