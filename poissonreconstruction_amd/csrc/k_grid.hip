@@ -9,7 +9,7 @@
 
 #include <algorithm>
 
-#include "kernels.hpp"
+#include "cg_dev.hpp"
 
 namespace prx {
 
@@ -33,18 +33,6 @@ int partial_capacity(int64_t N) {
     int64_t m = b > r ? b : r;
     if (nv > m) m = nv;
     return (int)(m < 2048 ? 2048 : m);
-}
-
-// Deterministic block sum: 64-lane butterfly, then waves in order.
-__device__ inline double block_sum_256(double v, double* wsum) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) wsum[wave] = v;
-    __syncthreads();
-    double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wsum[w];
-    return s;
 }
 
 // ------------------------------------------------------------------ tree export
@@ -141,209 +129,6 @@ __device__ __forceinline__ T x_pair(T x, T dp, T d, double ap, double a) {
     return (T)(((double)x + ap * (double)dp) + a * (double)d);
 }
 
-// beta of the next direction, with the breakdown and iteration-cap tests
-__device__ inline void fin_beta(CgState* st, double beta) {
-    if (beta != beta || beta == INFINITY || beta == -INFINITY) {
-        st->done = 1;
-        st->status = PRX_ERR_BREAKDOWN;
-        st->iters = st->max_iters;
-        return;
-    }
-    st->beta = beta;
-    if (st->it >= st->max_iters) {
-        st->done = 1;
-        st->iters = st->max_iters;
-        st->status = PRX_ERR_NOT_CONVERGED;
-    }
-}
-
-// The scalar step of solve_cg (src/solvers.rs:36-64) on the folded sum.
-// the vector update of an iteration has run: account for the lazy x term
-// (dir: ping-pong index of the iteration's direction)
-__device__ inline void lazy_step(CgState* st, bool full, int dir) {
-    st->dir_buf = dir;
-    if (full || !st->lazy || st->xpend) {
-        st->xpend = 0;
-    } else {
-        st->xpend = 1;
-        st->alpha_pend = st->alpha;
-    }
-}
-
-__device__ inline void fin_step(CgState* st, double sum, int op, int slot) {
-    switch (op) {
-        case FIN_SLOT: st->slot[slot] = sum; break;
-        case FIN_INIT_REF:  // delta0 = r.r ; stop when delta <= delta0*eps^2
-        case FIN_INIT_TRUE: // stop when delta <= ||b||^2 eps^2
-            st->delta = sum;
-            st->delta0 = sum;
-            st->it = 0;
-            st->iters = 0;
-            st->status = 0;
-            st->refresh_due = 0;
-            st->xpend = 0;
-            st->thr = ((op == FIN_INIT_REF ? sum : st->bnorm2) * st->eps) * st->eps;
-            st->done = (sum == 0.0 || sum <= st->thr) ? 1 : 0;
-            break;
-        case FIN_ALPHA:
-            st->dq = sum;
-            if (sum == 0.0) {
-                st->done = 1;
-                st->iters = st->it + 1;
-            } else {
-                st->alpha = st->delta / sum;
-            }
-            break;
-        case FIN_BETA:
-        case FIN_BETA_FULL: {
-            lazy_step(st, op == FIN_BETA_FULL, slot);
-            double dold = st->delta;
-            st->delta_prev = dold;
-            st->delta = sum;
-            st->it += 1;
-            if (sum <= st->thr) {
-                st->done = 1;
-                st->iters = st->it;
-                break;
-            }
-            double beta = sum / dold;
-            if (beta != beta || beta == INFINITY || beta == -INFINITY) {
-                st->done = 1;
-                st->status = PRX_ERR_BREAKDOWN;
-                st->iters = st->max_iters;
-                break;
-            }
-            st->beta = beta;
-            if (st->it >= st->max_iters) {
-                st->done = 1;
-                st->iters = st->max_iters;
-                st->status = PRX_ERR_NOT_CONVERGED;
-            }
-            break;
-        }
-        case FIN_PINIT:  // sum = r0.z0 ; slot[0] holds |r0|^2 ; stop when |r|^2 <= ||b||^2 eps^2
-            st->delta = sum;
-            st->delta0 = sum;
-            st->delta_prev = sum;
-            st->beta = 0.0;
-            st->it = 0;
-            st->iters = 0;
-            st->status = 0;
-            st->refresh_due = 0;
-            st->xpend = 0;
-            st->thr = (st->bnorm2 * st->eps) * st->eps;
-            st->slot[1] = st->slot[0];
-            st->done = (sum == 0.0 || st->slot[0] <= st->thr) ? 1 : 0;
-            break;
-        case FIN_PRES:  // sum = |r|^2 after the update of iteration it
-            lazy_step(st, false, slot);
-            st->slot[1] = sum;
-            st->it += 1;
-            if (sum != sum) {
-                st->done = 1;
-                st->status = PRX_ERR_BREAKDOWN;
-                st->iters = st->max_iters;
-            } else if (sum <= st->thr) {
-                st->done = 1;
-                st->iters = st->it;
-            } else if (st->it >= st->max_iters) {
-                st->done = 1;
-                st->iters = st->max_iters;
-                st->status = PRX_ERR_NOT_CONVERGED;
-            }
-            break;
-        case FIN_PBETA: {  // sum = r.z of the new residual
-            const double beta = sum / st->delta;
-            if (beta != beta || beta == INFINITY || beta == -INFINITY) {
-                st->done = 1;
-                st->status = PRX_ERR_BREAKDOWN;
-                st->iters = st->max_iters;
-                break;
-            }
-            st->beta = beta;
-            st->delta_prev = st->delta;
-            st->delta = sum;
-            break;
-        }
-        case FIN_FRESH:  // sum = |b - A x|^2 of the refreshed residual (x complete)
-            st->rpend = 0;
-            st->xn = 0;
-            st->delta = sum;
-            if (sum <= st->thr) {
-                st->done = 1;
-                st->iters = st->it;
-                break;
-            }
-            fin_beta(st, sum / st->delta_prev);
-            break;
-    }
-}
-
-// The fused CG step (src/solvers.rs:36-64 with the two reductions of an
-// iteration merged): the pass formed r_k (the pending update of the previous
-// iteration), d_k = r_k + beta d_{k-1} and q_k = A d_k and summed d_k.q_k,
-// r_k.q_k, q_k.q_k and r_k.r_k.  delta_k = r_k.r_k and alpha_k = delta_k /
-// d_k.q_k as in the two-pass form; the update r_k - alpha_k q_k is left to
-// the next pass, so the stopping test and beta_k take its |r|^2 from the
-// expansion delta_k - 2 alpha_k r_k.q_k + alpha_k^2 q_k.q_k.  The next pass
-// sums its own r.r again, so the expansion's rounding never accumulates over
-// iterations (carried from one to the next, it would: its absolute error,
-// about eps * the early |r|^2, swamps |r|^2 once the residual has fallen by
-// ~1e-6).  x: CG_XM pending terms were added before the pass (xn == CG_XM
-// -> 0), then alpha_k d_k is pending.  refresh: the caller recomputes r = b -
-// A x next (FIN_FRESH decides).
-__device__ inline void fin_cg(CgState* st, double dq, double rq, double qq, double rr, int refresh) {
-    if (st->xn == CG_XM) st->xn = 0;
-    st->dq = dq;
-    st->delta = rr;
-    if (dq == 0.0) {
-        st->done = 1;
-        st->iters = st->it + 1;
-        return;
-    }
-    const double a = st->delta / dq;
-    st->alpha = a;
-    st->xa[st->xn] = a;
-    st->xn += 1;
-    st->rpend = 1;
-    const double g = (st->delta - 2.0 * a * rq) + a * a * qq;
-    st->delta_prev = st->delta;
-    st->delta = g;
-    st->it += 1;
-    // a refresh of r follows: FIN_FRESH tests the refreshed residual first,
-    // then forms beta and applies the iteration cap, as the two-pass form does
-    // (src/solvers.rs:49-63)
-    if (refresh) return;
-    if (g <= st->thr) {
-        st->done = 1;
-        st->iters = st->it;
-        return;
-    }
-    fin_beta(st, g / st->delta_prev);
-}
-
-// 1024 threads fold np partials in a fixed order (16 strided accumulators,
-// a fixed pairwise tree, then the block sum), so a reduction is reproducible
-// run to run; 16 independent loads per thread are in flight per round.
-// STRIDE: the partials of one sum are every STRIDE-th value.
-template <int STRIDE = 1>
-__device__ inline double fin_sum(const double* partial, int64_t np, double* ws) {
-    double a[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) a[u] = 0.0;
-    int64_t i = threadIdx.x;
-    for (; i + 15 * 1024 < np; i += 16 * 1024) {
-#pragma unroll
-        for (int u = 0; u < 16; ++u) a[u] += partial[STRIDE * (i + u * 1024)];
-    }
-    for (; i < np; i += 1024) a[0] += partial[STRIDE * i];
-#pragma unroll
-    for (int w = 8; w >= 1; w >>= 1)
-#pragma unroll
-        for (int u = 0; u < w; ++u) a[u] += a[u + w];
-    return block_sum_256(a[0], ws);
-}
-
 // One block folds the partials in a fixed order and advances the CG scalars
 // exactly as solve_cg does (src/solvers.rs:36-64).
 __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial, int np, int op, int slot) {
@@ -353,57 +138,11 @@ __global__ __launch_bounds__(1024) void k_fin(CgState* st, const double* partial
     if (threadIdx.x != 0) return;
     fin_step(st, sum, op, slot);
 }
-// FIN_CG first stage: block b folds the four interleaved sums of bricks
-// [b*chunk, min(np, (b+1)*chunk)) in a fixed order (per thread in brick
-// order, then the 64-lane butterfly, then the waves in order) into
-// out[4b .. 4b+3], so the single-block fold reads 4 values per block
-// instead of 4 per brick.
-constexpr int FOLD_SPLIT = 256;  // more bricks than this: two stages (at most FOLD_BLOCKS first-stage blocks)
-__device__ __host__ inline int fold_blocks(int np) {
-    const int g = (np + 1023) / 1024;
-    return np > FOLD_SPLIT ? (g < FOLD_BLOCKS ? g : FOLD_BLOCKS) : 0;
-}
-__device__ inline void fold4_block(const double* __restrict__ partial, int np, int b, double* __restrict__ out, double* ws) {
-    typedef double nd2 __attribute__((ext_vector_type(2)));
-    const int g = fold_blocks(np), chunk = (np + g - 1) / g;
-    const int b0 = b * chunk, b1 = min(np, b0 + chunk);
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = b0 + (int)threadIdx.x; i < b1; i += 256) {
-        const nd2 u = *(const nd2*)(partial + 4 * (int64_t)i);
-        const nd2 v = *(const nd2*)(partial + 4 * (int64_t)i + 2);
-        a[0] += u[0];
-        a[1] += u[1];
-        a[2] += v[0];
-        a[3] += v[1];
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_xor(a[j], o, 64);
-        if (lane == 0) ws[j * 4 + wave] = a[j];
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double sum = 0.0;
-        for (int w = 0; w < 4; ++w) sum += ws[threadIdx.x * 4 + w];
-        out[4 * b + threadIdx.x] = sum;
-    }
-}
 __global__ __launch_bounds__(256) void k_fold4(const CgState* st, const double* __restrict__ partial, int np, double* __restrict__ out) {
     __shared__ double ws[16];
     if (st->done) return;
     fold4_block(partial, np, blockIdx.x, out, ws);
 }
-// the four sums of FIN_CG, second stage (or the only one for small grids)
-__device__ inline void fold4_final(const double* partial, int np, const double* first, double* ws, double v[4]) {
-    const int g = fold_blocks(np);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        v[j] = g ? fin_sum<4>(first + j, g, ws) : fin_sum<4>(partial + j, np, ws);
-        __syncthreads();  // ws is reused by the next sum
-    }
-}
-
 // FIN_CG: the first stage writes behind the partials (partial + 4 np)
 __global__ __launch_bounds__(1024) void k_fin_cg(CgState* st, const double* partial, int np, int slot) {
     __shared__ double ws[16];
@@ -428,8 +167,6 @@ void launch_fin(CgState* st, const double* partial, int np, int op, int slot, hi
 }
 
 // ------------------------------------------------------------------ batched CG pieces
-__device__ inline bool b_runs(const BatchTab& tab, uint32_t k) { return tab.run[(int64_t)k * tab.run_stride] == tab.run_when; }
-
 __global__ __launch_bounds__(1024) void k_bfin(BatchTab tab, const double* partial, int ulog, int op, int slot) {
     __shared__ double ws[16];
     const uint32_t k = blockIdx.x;
@@ -470,49 +207,10 @@ void launch_bfin_cg(const BatchTab& tab, const double* partial, int ulog, double
     PRX_LAUNCH_CHECK();
 }
 
-// Fused CG x terms per instance (x + xa[0] d[0] + ... in FP64, rounded once,
-// as k_x_terms).  A block covers 256 vectors inside one 4096-node brick, so
-// it belongs to one instance.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_bx_terms(BatchTab tab, T* __restrict__ x, XDirs<T> dd, const T* __restrict__ ring,
                                                   int64_t Ntot) {
-    using NV = typename NV16<T>::type;
-    constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t v0 = (int64_t)blockIdx.x * 256;
-    const uint32_t k = tab.brick_inst[(uint64_t)(v0 * VEC) >> 12];
-    if (!b_runs(tab, k)) return;
-    const CgState* st = tab.st + k;
-    const int xn = st->xn;
-    if (MODE == 0 ? xn != CG_XM : xn == 0) return;
-    const T* dp[CG_XM];
-#pragma unroll
-    for (int j = 0; j < CG_XM; ++j) {
-        if (MODE == 1) {
-            const int64_t sl = ((st->it - (int64_t)st->ring0 - xn + j + 1) % CG_XM + CG_XM) % CG_XM;
-            dp[j] = ring + sl * Ntot;
-        } else {
-            dp[j] = dd.d[j];
-        }
-    }
-    double a[CG_XM];
-#pragma unroll
-    for (int j = 0; j < CG_XM; ++j) a[j] = st->xa[j];
-    const int64_t g = (v0 + threadIdx.x) * VEC;
-    const NV xv = ld_nt(x + g);
-    NV dv[CG_XM];
-#pragma unroll
-    for (int j = 0; j < CG_XM; ++j)
-        if (j < xn && dp[j]) dv[j] = ld_nt(dp[j] + g);
-    NV xo;
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) {
-        double v = (double)xv[e];
-#pragma unroll
-        for (int j = 0; j < CG_XM; ++j)
-            if (j < xn && dp[j]) v += a[j] * (double)dv[j][e];
-        xo[e] = (T)v;
-    }
-    st_nt(x + g, xo);
+    bx_terms_block<T, MODE>(tab, x, dd, ring, Ntot, blockIdx.x);
 }
 template <typename T>
 void launch_bx_terms(const BatchTab& tab, T* x, const XDirs<T>& d, const T* ring, int64_t Ntot, int mode, hipStream_t s) {
@@ -526,19 +224,8 @@ void launch_bx_terms(const BatchTab& tab, T* x, const XDirs<T>& d, const T* ring
 template void launch_bx_terms<double>(const BatchTab&, double*, const XDirs<double>&, const double*, int64_t, int, hipStream_t);
 template void launch_bx_terms<float>(const BatchTab&, float*, const XDirs<float>&, const float*, int64_t, int, hipStream_t);
 
-// x = x2 and e = 0 on the nodes of the running instances: the fold of a
-// batched residual-update pass, which wrote x + e into x2 (a block covers
-// 1024 nodes inside one 4096-node brick, so it belongs to one instance).
 __global__ __launch_bounds__(256) void k_bcommit(BatchTab tab, double* __restrict__ x, const double* __restrict__ x2, float* __restrict__ e) {
-    typedef double nd2 __attribute__((ext_vector_type(2)));
-    typedef float nf4 __attribute__((ext_vector_type(4)));
-    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
-    const uint32_t k = tab.brick_inst[((uint64_t)blockIdx.x * 1024) >> 12];
-    if (!b_runs(tab, k)) return;
-    const nd2 a = *(const nd2*)(x2 + j), b = *(const nd2*)(x2 + j + 2);
-    __builtin_nontemporal_store(a, (nd2*)(x + j));
-    __builtin_nontemporal_store(b, (nd2*)(x + j + 2));
-    __builtin_nontemporal_store(nf4{0.f, 0.f, 0.f, 0.f}, (nf4*)(e + j));
+    bcommit_block(tab, x, x2, e, blockIdx.x);
 }
 void launch_bcommit(const BatchTab& tab, double* x, const double* x2, float* e, int64_t Ntot, hipStream_t s) {
     hipLaunchKernelGGL(k_bcommit, dim3((unsigned)(Ntot / 1024)), dim3(256), 0, s, tab, x, x2, e);
@@ -664,17 +351,7 @@ template void launch_bflush_x<float>(const BatchTab&, float*, const float*, cons
 template <typename T>
 __global__ __launch_bounds__(256) void k_bdot(BatchTab tab, const T* __restrict__ a, const T* __restrict__ b, double* partial) {
     __shared__ double ws[4];
-    using NV = typename NV16<T>::type;
-    constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) * VEC;
-    const uint32_t k = tab.brick_inst[(uint64_t)blockIdx.x * 256 * VEC >> 12];
-    if (!b_runs(tab, k)) return;
-    const NV av = ld_v(a + j), bv = ld_v(b + j);
-    double acc = 0.0;
-#pragma unroll
-    for (int e = 0; e < VEC; ++e) acc += (double)av[e] * (double)bv[e];
-    const double sm = block_sum_256(acc, ws);
-    if (threadIdx.x == 0) partial[blockIdx.x] = sm;
+    bdot_block<T>(tab, a, b, partial, blockIdx.x, ws);
 }
 template <typename T>
 int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s) {

@@ -9,7 +9,7 @@
 
 #include <hipcub/hipcub.hpp>
 
-#include "kernels.hpp"
+#include "cg_dev.hpp"
 
 namespace prx {
 
@@ -856,52 +856,12 @@ template void launch_cell_screening<float>(const LevelView&, const CellLists&, c
 template void launch_screen_gather<float>(const LevelView&, const float*, const double*, const float*, float*, hipStream_t,
                                           const float*, const float*, const double*, const float*, const CgState*);
 
-// The same gather over a batch: point i belongs to instance pts.inst[i]
-// (its grid size, wh3, first node and, in MODE 2, beta); it runs while the
-// instance's run flag says so.
 template <typename T, int MODE>
 __global__ void k_bscreen_gather(BatchTab tab, int64_t P, const uint32_t* __restrict__ cell, const T* __restrict__ tt,
                                  const uint32_t* __restrict__ pinst, const T* __restrict__ x, const double* __restrict__ x64,
                                  const float* __restrict__ e32, T* s, const T* __restrict__ r, const T* __restrict__ dd,
                                  const T* __restrict__ qq) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    const uint32_t k = pinst[i];
-    if (tab.run[(int64_t)k * tab.run_stride] != tab.run_when) return;
-    const BCoef cf = tab.coef[k];
-    const uint64_t ib = tab.off[k];
-    const double bt = MODE >= 2 ? tab.st[k].beta : 0.0;
-    const double al = MODE == 3 ? tab.st[k].alpha : 0.0;
-    const int rp = MODE == 3 ? tab.st[k].rpend : 0;
-    const uint32_t res = cf.res;
-    const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
-    const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
-    // as k_screen_gather: all loads first (out-of-grid corners read the
-    // instance's node 0 with weight 0), then the sum in corner order
-    uint64_t m[8];
-    T w[8], v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const uint32_t ox = c[0] + (q & 1), oy = c[1] + ((q >> 1) & 1), oz = c[2] + (q >> 2);
-        const bool in = ox < res && oy < res && oz < res;
-        const T bx = (q & 1) ? t[0] : T(1) - t[0];
-        const T by = ((q >> 1) & 1) ? t[1] : T(1) - t[1];
-        const T bz = (q >> 2) ? t[2] : T(1) - t[2];
-        w[q] = in ? bx * by * bz : T(0);
-        m[q] = ib + (in ? morton3(ox, oy, oz) : 0u);
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        if (MODE == 0) v[q] = x[m[q]];
-        else if (MODE == 1) v[q] = (T)(x64[m[q]] + (double)e32[m[q]]);
-        else if (MODE == 2) v[q] = dir_form(bt, dd[m[q]], r[m[q]]);
-        else v[q] = cg_form(bt, al, rp, dd[m[q]], r[m[q]], qq[m[q]]);  // q is read always and used when rp
-    }
-    T acc = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc += w[q] * v[q];
-    const T wh3 = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;
-    s[i] = wh3 * acc;
+    bscreen_gather_point<T, MODE>(tab, P, cell, tt, pinst, x, x64, e32, s, r, dd, qq, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 template <typename T>
 void launch_bscreen_gather(const BatchTab& tab, const BPoints& pts, const T* x, const double* x64, const float* e32, T* sbuf,
