@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-mg", action="store_true", help="skip the multigrid-preconditioned solves")
     ap.add_argument("--batch-clouds", type=int, default=256)
+    ap.add_argument("--batch-legs", default="graphs,persistent,sequential,multigrid",
+                    help="with --batch-only: which solves of the batching workload to run (comma-separated)")
     ap.add_argument("--op-reps", type=int, default=20)
     ap.add_argument("--large", action="store_true",
                     help="run only the large workload (depth 10, 5M points on a displaced sphere, mixed) and print its line")
@@ -83,7 +85,8 @@ def main():
         print(json.dumps(large_bench(pr, L, synthetic, args.tol, args.large_fp64)), flush=True)
         return
     if args.batch_only:
-        print(json.dumps({"batch_only": True, "batch": batch_bench(pr, L, synthetic, args.tol, args.batch_clouds)}), flush=True)
+        print(json.dumps({"batch_only": True, "batch": batch_bench(pr, L, synthetic, args.tol, args.batch_clouds,
+                                                                   set(args.batch_legs.split(",")))}), flush=True)
         return
     if args.cpu_configs:
         print(json.dumps(measured_cpu_solves(pr, L, synthetic, [
@@ -382,13 +385,16 @@ def adjoint_bench(pr, L, synthetic, tol):
     return out
 
 
-def batch_bench(pr, L, synthetic, tol, nclouds):
+def batch_bench(pr, L, synthetic, tol, nclouds, legs=("graphs", "persistent", "sequential", "multigrid")):
     """Batching workload: `nclouds` superquadrics (sizes log-uniform in
     [2,000, 200,000], exponents U[0.3,3], scales U[0.5,1], position noise
     N(0,5e-3), exact normals), depth 6 below 20,000 points, 7 below 100,000,
-    else 8, point weight 4, FP64 cascadic solves to `tol` true residual: all
-    instances in one batched, graph-replayed solve (autotuned), against the
-    same instances solved one after another with the single-tree path."""
+    else 8, point weight 4, cascadic solves to `tol` true residual in FP64 and
+    in mixed precision: all instances in one batched solve, its CG iterations
+    replayed from captured graphs (autotuned; `graphs`) or run as one
+    persistent launch per level (`persistent`), against the same instances
+    solved one after another with the single-tree path (`sequential`), and
+    every instance by multigrid-preconditioned CG (`multigrid`)."""
     from poissonreconstruction_amd.batch import Batch, depth_for
     t0 = time.time()
     sizes = synthetic.batch_sizes(nclouds, seed=0)
@@ -398,27 +404,77 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
         u, _, _ = L.unit_transform(raw, 1.1)
         B.add(depth_for(int(n)), u, 4.0)
     log(f"batch setup: {nclouds} clouds, {int(sizes.sum())} points in {time.time() - t0:.1f}s")
-    runs = []
-    for i in range(3):
-        ts = time.perf_counter()
-        rep, per = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True)
-        wall = (time.perf_counter() - ts) * 1e3
-        log(f"batch solve {i}: device {rep['ms_total']:.1f} ms wall {wall:.1f} ms converged {rep['converged']}/"
-            f"{rep['instances']} max_true_rel {rep['max_true_rel']:.3e} tile {rep['tile']} chunk {rep['chunk']} "
-            f"gs_block {rep['gs_block']}")
-        runs.append((rep, wall))
-    rep, wall = runs[-1]
-    # the same batched cascadic solve in mixed precision (FP32 correction CG, FP64 residual replacements)
-    mixed_runs = []
-    for i in range(2):
-        ts = time.perf_counter()
-        mrep, mper = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True,
-                             precision=L.PRX_PREC_MIXED)
-        mwall = (time.perf_counter() - ts) * 1e3
-        log(f"batch mixed solve {i}: device {mrep['ms_total']:.1f} ms wall {mwall:.1f} ms converged {mrep['converged']}/"
-            f"{mrep['instances']} max_true_rel {mrep['max_true_rel']:.3e} graph launches {mrep['graph_launches']}")
-        mixed_runs.append((mrep, mwall, mper))
-    mrep, mwall, mper = mixed_runs[-1]
+    depths = [depth_for(int(n)) for n in sizes]
+    out = {"workload": f"{nclouds} superquadrics, sizes log-uniform [2e3, 2e5], depth 6/7/8 by size, point weight 4, "
+                       f"to {tol:g} true residual",
+           "clouds": nclouds, "points": int(sizes.sum()), "depth_counts": {d: depths.count(d) for d in (6, 7, 8)}}
+    rep = mrep = None
+    if "graphs" in legs:
+        for i in range(3):
+            ts = time.perf_counter()
+            rep, per = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True,
+                               launch=L.PRX_BATCH_LAUNCH_GRAPHS)
+            wall = (time.perf_counter() - ts) * 1e3
+            log(f"batch solve {i}: device {rep['ms_total']:.1f} ms wall {wall:.1f} ms converged {rep['converged']}/"
+                f"{rep['instances']} max_true_rel {rep['max_true_rel']:.3e} tile {rep['tile']} chunk {rep['chunk']} "
+                f"gs_block {rep['gs_block']}")
+        # the same batched cascadic solve in mixed precision (FP32 correction CG, FP64 residual replacements)
+        for i in range(2):
+            ts = time.perf_counter()
+            mrep, mper = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True,
+                                 precision=L.PRX_PREC_MIXED, launch=L.PRX_BATCH_LAUNCH_GRAPHS)
+            mwall = (time.perf_counter() - ts) * 1e3
+            log(f"batch mixed solve {i}: device {mrep['ms_total']:.1f} ms wall {mwall:.1f} ms converged "
+                f"{mrep['converged']}/{mrep['instances']} max_true_rel {mrep['max_true_rel']:.3e} "
+                f"graph launches {mrep['graph_launches']}")
+        out.update({"nodes": int(rep["total_nodes"]),
+                    "ms_batched": round(rep["ms_total"], 2), "ms_batched_wall": round(wall, 2),
+                    "nodes_per_s": round(rep["total_nodes"] / (rep["ms_total"] * 1e-3), 1),
+                    "converged": rep["converged"], "max_true_rel": rep["max_true_rel"],
+                    "max_fine_iters": rep["max_fine_iters"], "sum_fine_iters": rep["sum_fine_iters"],
+                    "ms_coarse": round(rep["ms_coarse"], 2), "ms_fine_gs": round(rep["ms_fine_gs"], 2),
+                    "ms_fine_cg": round(rep["ms_fine_cg"], 2),
+                    "autotuned": {"tile": rep["tile"], "chunk": rep["chunk"], "gs_block": rep["gs_block"],
+                                  "ms_tune": round(rep["ms_tune"], 1)},
+                    "graph_launches": rep["graph_launches"],
+                    "mixed": {"ms_batched": round(mrep["ms_total"], 2), "ms_batched_wall": round(mwall, 2),
+                              "speedup_vs_fp64_batched": round(rep["ms_total"] / mrep["ms_total"], 3),
+                              "converged": mrep["converged"], "max_true_rel": mrep["max_true_rel"],
+                              "max_fine_iters": mrep["max_fine_iters"], "sum_fine_iters": mrep["sum_fine_iters"],
+                              "ms_fine_gs": round(mrep["ms_fine_gs"], 2), "ms_fine_cg": round(mrep["ms_fine_cg"], 2),
+                              "max_outer": int(max(p["outer_iters"] for p in mper)),
+                              "graph_launches_cumulative": mrep["graph_launches"]}})
+    if "persistent" in legs:
+        # the same solves with each level's CG of all instances in one persistent launch
+        pers = {}
+        for prec, name in ((L.PRX_PREC_FP64, "fp64"), (L.PRX_PREC_MIXED, "mixed")):
+            best = None
+            for i in range(2):
+                ts = time.perf_counter()
+                prep, pper = B.solve(eps=tol, residual_mode=L.PRX_RESID_TRUE, refresh_period=0, autotune=True,
+                                     precision=prec, launch=L.PRX_BATCH_LAUNCH_PERSISTENT)
+                pw = (time.perf_counter() - ts) * 1e3
+                log(f"batch persistent {name} solve {i}: device {prep['ms_total']:.1f} ms wall {pw:.1f} ms converged "
+                    f"{prep['converged']}/{prep['instances']} max_true_rel {prep['max_true_rel']:.3e} "
+                    f"rounds {prep['persistent_rounds']} workgroups {prep['persistent_blocks']} "
+                    f"(on XCD launch index mod 8: {prep['persistent_xcd_match']}) phase ms "
+                    f"{[round(v, 1) for v in prep['persistent_phase_ms']]}")
+                if prep["converged"] != prep["instances"] or not prep["max_true_rel"] <= tol:
+                    raise SystemExit(f"bench.py: persistent batched {name} solve did not converge")
+                best = (prep, pw) if best is None or prep["ms_total"] < best[0]["ms_total"] else best
+            prep, pw = best
+            pers[name] = {"ms_batched": round(prep["ms_total"], 2), "ms_batched_wall": round(pw, 2),
+                          "ms_coarse": round(prep["ms_coarse"], 2), "ms_fine_gs": round(prep["ms_fine_gs"], 2),
+                          "ms_fine_cg": round(prep["ms_fine_cg"], 2), "max_fine_iters": prep["max_fine_iters"],
+                          "sum_fine_iters": prep["sum_fine_iters"], "max_true_rel": prep["max_true_rel"],
+                          "persistent_launches_per_solve": 2, "workgroups": prep["persistent_blocks"],
+                          "workgroups_on_xcd_index_mod_8": prep["persistent_xcd_match"],
+                          "phase_ms": dict(zip(("replacement", "gather", "pass", "fold"),
+                                               [round(v, 2) for v in prep["persistent_phase_ms"]]))}
+        if rep is not None:
+            pers["fp64"]["vs_graphs"] = round(rep["ms_total"] / pers["fp64"]["ms_batched"], 3)
+            pers["mixed"]["vs_graphs"] = round(mrep["ms_total"] / pers["mixed"]["ms_batched"], 3)
+        out["persistent"] = pers
 
     def sequential(prec):
         # every instance solved alone through its single-tree path (in FP32 the face-form step), one after
@@ -437,50 +493,34 @@ def batch_bench(pr, L, synthetic, tol, nclouds):
             t.release_workspace()
         return (time.perf_counter() - ts) * 1e3, dev, iters
 
-    seq_ms, seq_dev, seq_iters = sequential(L.PRX_PREC_FP64)
-    log(f"sequential single-tree FP64 solves: {seq_ms:.1f} ms wall, {seq_dev:.1f} ms device ({seq_iters} fine iterations)")
-    mseq_ms, mseq_dev, mseq_iters = sequential(L.PRX_PREC_MIXED)
-    log(f"sequential single-tree mixed solves: {mseq_ms:.1f} ms wall, {mseq_dev:.1f} ms device ({mseq_iters} fine iterations)")
-    depths = [depth_for(int(n)) for n in sizes]
-    out = {"workload": f"{nclouds} superquadrics, sizes log-uniform [2e3, 2e5], depth 6/7/8 by size, point weight 4, "
-                       f"FP64 to {tol:g} true residual",
-           "clouds": nclouds, "points": int(sizes.sum()), "nodes": int(rep["total_nodes"]),
-           "depth_counts": {d: depths.count(d) for d in (6, 7, 8)},
-           "ms_batched": round(rep["ms_total"], 2), "ms_batched_wall": round(wall, 2),
-           "ms_sequential": round(seq_ms, 2), "ms_sequential_device": round(seq_dev, 2),
-           "speedup_vs_sequential": round(seq_ms / wall, 3),
-           "nodes_per_s": round(rep["total_nodes"] / (rep["ms_total"] * 1e-3), 1),
-           "converged": rep["converged"], "max_true_rel": rep["max_true_rel"],
-           "max_fine_iters": rep["max_fine_iters"], "sum_fine_iters": rep["sum_fine_iters"],
-           "ms_coarse": round(rep["ms_coarse"], 2), "ms_fine_gs": round(rep["ms_fine_gs"], 2),
-           "ms_fine_cg": round(rep["ms_fine_cg"], 2),
-           "autotuned": {"tile": rep["tile"], "chunk": rep["chunk"], "gs_block": rep["gs_block"],
-                         "ms_tune": round(rep["ms_tune"], 1)},
-           "graph_launches": rep["graph_launches"],
-           "mixed": {"ms_batched": round(mrep["ms_total"], 2), "ms_batched_wall": round(mwall, 2),
-                     "speedup_vs_fp64_batched": round(rep["ms_total"] / mrep["ms_total"], 3),
-                     "speedup_vs_sequential_fp64": round(seq_ms / mwall, 3),
-                     "ms_sequential": round(mseq_ms, 2), "ms_sequential_device": round(mseq_dev, 2),
-                     "speedup_vs_sequential": round(mseq_ms / mwall, 3),
-                     "converged": mrep["converged"], "max_true_rel": mrep["max_true_rel"],
-                     "max_fine_iters": mrep["max_fine_iters"], "sum_fine_iters": mrep["sum_fine_iters"],
-                     "ms_fine_gs": round(mrep["ms_fine_gs"], 2), "ms_fine_cg": round(mrep["ms_fine_cg"], 2),
-                     "max_outer": int(max(p["outer_iters"] for p in mper)),
-                     "graph_launches_cumulative": mrep["graph_launches"]}}
-    # every instance by multigrid-preconditioned CG, over four streams
-    mg = {}
-    for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):
-        best = None
-        for i in range(2):
-            ts = time.perf_counter()
-            r, _ = B.solve(eps=tol, precond=L.PRX_PRECOND_MG, precision=prec)
-            w = (time.perf_counter() - ts) * 1e3
-            best = (r, w) if best is None or w < best[1] else best
-        r, w = best
-        mg[name] = {"ms_wall": round(w, 2), "converged": r["converged"], "max_true_rel": r["max_true_rel"],
-                    "max_iters": r["max_fine_iters"], "sum_iters": r["sum_fine_iters"]}
-        log(f"batch multigrid {name}: {mg[name]}")
-    out["multigrid"] = mg
+    if "sequential" in legs:
+        seq_ms, seq_dev, seq_iters = sequential(L.PRX_PREC_FP64)
+        log(f"sequential single-tree FP64 solves: {seq_ms:.1f} ms wall, {seq_dev:.1f} ms device ({seq_iters} fine "
+            f"iterations)")
+        mseq_ms, mseq_dev, mseq_iters = sequential(L.PRX_PREC_MIXED)
+        log(f"sequential single-tree mixed solves: {mseq_ms:.1f} ms wall, {mseq_dev:.1f} ms device ({mseq_iters} fine "
+            f"iterations)")
+        out.update({"ms_sequential": round(seq_ms, 2), "ms_sequential_device": round(seq_dev, 2)})
+        if rep is not None:
+            out["speedup_vs_sequential"] = round(seq_ms / wall, 3)
+            out["mixed"].update({"speedup_vs_sequential_fp64": round(seq_ms / mwall, 3),
+                                 "ms_sequential": round(mseq_ms, 2), "ms_sequential_device": round(mseq_dev, 2),
+                                 "speedup_vs_sequential": round(mseq_ms / mwall, 3)})
+    if "multigrid" in legs:
+        # every instance by multigrid-preconditioned CG, over four streams
+        mg = {}
+        for prec, name in ((L.PRX_PREC_MIXED, "mixed"), (L.PRX_PREC_FP64, "fp64")):
+            best = None
+            for i in range(2):
+                ts = time.perf_counter()
+                r, _ = B.solve(eps=tol, precond=L.PRX_PRECOND_MG, precision=prec)
+                w = (time.perf_counter() - ts) * 1e3
+                best = (r, w) if best is None or w < best[1] else best
+            r, w = best
+            mg[name] = {"ms_wall": round(w, 2), "converged": r["converged"], "max_true_rel": r["max_true_rel"],
+                        "max_iters": r["max_fine_iters"], "sum_iters": r["sum_fine_iters"]}
+            log(f"batch multigrid {name}: {mg[name]}")
+        out["multigrid"] = mg
     B.close()
     log(f"batch {out}")
     return out

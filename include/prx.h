@@ -132,9 +132,20 @@ typedef struct prx_solve_opts {
      * q stored as a full vector in every precision; PRX_CG_FUSED_FACES the
      * face form in every precision on grids of depth >= 4, batches included */
     int32_t cg_form;
+    /* Batched cascadic solves only (no counterpart in the reference): how the
+     * CG iterations of all instances are launched.  PRX_BATCH_LAUNCH_PERSISTENT
+     * runs each level's CG of every instance in one persistent launch (a
+     * cooperative kernel whose workgroups iterate the instances in lockstep
+     * rounds and decide replacements, confirmations and stops on the device;
+     * PRX_RESID_TRUE with the fused CG step); PRX_BATCH_LAUNCH_GRAPHS replays
+     * captured chunks of iterations with host checks between them;
+     * PRX_BATCH_LAUNCH_AUTO (0, the default) picks the persistent launch where
+     * it applies. */
+    int32_t batch_launch;
 } prx_solve_opts;
 
 enum { PRX_CG_FUSED = 0, PRX_CG_TWO_PASS = 1, PRX_CG_FUSED_STORED_Q = 2, PRX_CG_FUSED_FACES = 3 };
+enum { PRX_BATCH_LAUNCH_AUTO = 0, PRX_BATCH_LAUNCH_GRAPHS = 1, PRX_BATCH_LAUNCH_PERSISTENT = 2 };
 
 typedef struct prx_solve_report {
     uint64_t coarse_iters, fine_iters, outer_iters, applies;
@@ -355,6 +366,12 @@ typedef struct prx_batch_report {
     double ms_total, ms_coarse, ms_fine_gs, ms_fine_cg, ms_tune;
     double max_true_rel;   /* max over instances of ||b-Ax||/||b|| */
     int32_t tile, chunk, gs_block, status; /* operator brick edge, CG iterations per graph, GS brick edge */
+    uint64_t persistent_launches;          /* CG levels run as one persistent launch each */
+    uint64_t persistent_rounds;            /* lockstep rounds over all persistent launches */
+    int32_t persistent_blocks;             /* workgroups of a persistent launch */
+    int32_t persistent_xcd_match;          /* of those, how many run on XCD (launch index mod 8) */
+    double persistent_phase_ms[4];         /* this solve's persistent launches: replacement, gather, pass and fold
+                                              phases (each to the end of its grid barrier) */
 } prx_batch_report;
 int prx_batch_create(int device, prx_batch** out);
 void prx_batch_destroy(prx_batch* b);

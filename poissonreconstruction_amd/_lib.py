@@ -32,6 +32,9 @@ PRX_CG_FUSED = 0
 PRX_CG_TWO_PASS = 1
 PRX_CG_FUSED_STORED_Q = 2
 PRX_CG_FUSED_FACES = 3
+PRX_BATCH_LAUNCH_AUTO = 0
+PRX_BATCH_LAUNCH_GRAPHS = 1
+PRX_BATCH_LAUNCH_PERSISTENT = 2
 PRX_MAX_DEPTH = 10
 
 STATUS_NAMES = {0: "ok", 1: "invalid", 2: "nonfinite", 3: "not_converged", 4: "breakdown", 5: "device", 6: "oom", 7: "state"}
@@ -72,7 +75,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("residual_mode", ctypes.c_int32), ("refresh_period", ctypes.c_int32),
                 ("gs_iters", ctypes.c_int32), ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_uint64),
                 ("coarse_eps", ctypes.c_double), ("precond", ctypes.c_int32), ("mg_smooth", ctypes.c_int32),
-                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double), ("cg_form", ctypes.c_int32)]
+                ("mg_omega", ctypes.c_double), ("mg_inner_tol", ctypes.c_double), ("cg_form", ctypes.c_int32),
+                ("batch_launch", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):
@@ -96,10 +100,14 @@ class BatchReport(ctypes.Structure):
                 ("max_coarse_iters", ctypes.c_uint64), ("graph_launches", ctypes.c_uint64), ("ms_total", ctypes.c_double),
                 ("ms_coarse", ctypes.c_double), ("ms_fine_gs", ctypes.c_double), ("ms_fine_cg", ctypes.c_double),
                 ("ms_tune", ctypes.c_double), ("max_true_rel", ctypes.c_double), ("tile", ctypes.c_int32),
-                ("chunk", ctypes.c_int32), ("gs_block", ctypes.c_int32), ("status", ctypes.c_int32)]
+                ("chunk", ctypes.c_int32), ("gs_block", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("persistent_launches", ctypes.c_uint64), ("persistent_rounds", ctypes.c_uint64),
+                ("persistent_blocks", ctypes.c_int32), ("persistent_xcd_match", ctypes.c_int32),
+                ("persistent_phase_ms", ctypes.c_double * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: (list(getattr(self, k)) if isinstance(getattr(self, k), ctypes.Array) else getattr(self, k))
+                for k, _ in self._fields_}
 
 
 _vp = ctypes.c_void_p

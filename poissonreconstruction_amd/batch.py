@@ -63,18 +63,24 @@ class Batch:
 
     def solve(self, gs: int = 20, cg_iters: int = 800000, eps: float = 1e-6, residual_mode: int = L.PRX_RESID_TRUE,
               refresh_period: int = 10, autotune: bool = True, precond: int = L.PRX_PRECOND_NONE,
-              precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False, cg_form: int = None):
+              precision: int = L.PRX_PREC_FP64, cg_two_pass: bool = False, cg_form: int = None,
+              launch: int = L.PRX_BATCH_LAUNCH_AUTO):
         """Returns (batch report dict, list of per-instance report dicts).
         The batched cascadic solve runs in `precision` (FP64, or mixed with the
         fused CG step); precond PRX_PRECOND_MG solves every instance with
         multigrid-preconditioned CG (mixed or FP64) instead; cg_two_pass
-        selects the two-pass CG iteration instead of the fused one (FP64)."""
+        selects the two-pass CG iteration instead of the fused one (FP64).
+        launch: PRX_BATCH_LAUNCH_PERSISTENT runs each level's CG of all
+        instances in one persistent launch, PRX_BATCH_LAUNCH_GRAPHS replays
+        captured chunks of iterations, AUTO picks the persistent launch where
+        it applies (true-residual stopping, fused CG step)."""
         o = L.default_solve_opts()
         o.precision = int(precision)
         o.residual_mode = int(residual_mode)
         o.refresh_period = int(refresh_period)
         o.precond = int(precond)
         o.cg_form = int(cg_form) if cg_form is not None else (L.PRX_CG_TWO_PASS if cg_two_pass else L.PRX_CG_FUSED)
+        o.batch_launch = int(launch)
         rep = L.BatchReport()
         per = (L.SolveReport * max(len(self._meta), 1))()
         check(L.lib().prx_solve_batched(self._h, int(gs), int(cg_iters), float(eps), ctypes.byref(o), 1 if autotune else 0,

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libprx.so")
 ARCH = os.environ.get("PRX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["k_grid.hip", "k_points.hip", "k_stencil.hip", "solver.hip", "reconstruct.hip", "measure.hip", "k_adjoint.hip", "batch.hip", "k_mc.hip", "k_mg.hip", "noalloc.hip"]
+SOURCES = ["k_grid.hip", "k_points.hip", "k_stencil.hip", "solver.hip", "reconstruct.hip", "measure.hip", "k_adjoint.hip", "batch.hip", "k_mc.hip", "k_mg.hip", "noalloc.hip", "persist.hip"]
 
 # -packed-fp32-ops: hipcc -O3 may form packed FP32 instructions from scalar
 # float code; they are kept out of every kernel.

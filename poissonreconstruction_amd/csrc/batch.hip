@@ -20,6 +20,8 @@
 // length by timing them on the batch itself.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <functional>
 #include <numeric>
@@ -73,6 +75,7 @@ struct BLevel {
     bool has_scr = false;
     int64_t P = 0;
     DArr<uint32_t> cell, pinst;
+    DArr<uint32_t> pt_off;  // [K+1] first sorted point of each instance (instances without screening have none)
     DArr<double> t64;
     DArr<float> t32;
     // the face-form FP32 pass: the instances' occupied cells and (cell, corner)
@@ -135,6 +138,15 @@ struct prx_batch {
     bool tuned = false;
     double tune_ms = 0.0;
     uint64_t graph_launches = 0;
+    // the persistent CG launches: control block, per-instance state, work lists
+    prx::DArr<prx::PCtl> pctl;
+    prx::DArr<prx::PInst> pinst;
+    prx::DArr<int32_t> plist_i;
+    prx::DArr<uint32_t> plist_u;
+    prx::DArr<uint8_t> pargs;  // the argument block of a persistent launch
+    uint64_t persistent_launches = 0, persistent_rounds = 0;
+    int persistent_blocks = 0, persistent_xcd_match = 0;
+    double persistent_phase_ms[4] = {0.0, 0.0, 0.0, 0.0};  // of the current solve
     std::vector<prx::GraphEntry> graphs;
 
     void drop_graphs() {
@@ -294,6 +306,16 @@ static void build_screening(prx_batch* B, BLevel& L) {
         if (t->pw > 0.0 && t->trunc_list.T > 0) { T += t->trunc_list.T; M += t->trunc_list.M; P += t->P; }
     L.has_scr = T > 0;
     L.P = P;
+    {
+        std::vector<uint32_t> po(L.K + 1, 0);
+        for (int k = 0; k < L.K; ++k) {
+            prx_tree* t = B->inst[k];
+            po[k + 1] = po[k] + (uint32_t)((t->pw > 0.0 && t->trunc_list.T > 0) ? t->P : 0);
+        }
+        L.pt_off.ensure(L.K + 1);
+        PRX_HIP(hipMemcpyAsync(L.pt_off.p, po.data(), sizeof(uint32_t) * (L.K + 1), hipMemcpyHostToDevice, s));
+        PRX_HIP(hipStreamSynchronize(s));
+    }
     if (!L.has_scr) return;
     if (M >= ((int64_t)1 << 32) || P >= ((int64_t)1 << 32)) throw Error(PRX_ERR_INVALID, "batch screening list too large");
     NodeList& G = L.scr;
@@ -386,6 +408,16 @@ static void build(prx_batch* B) {
     setup_level(B, B->fine, dF, cF);
     setup_level(B, B->coarse, dC, cC);
     build_screening(B, B->fine);
+    {
+        std::vector<uint32_t> zero(K + 1, 0);
+        B->coarse.pt_off.ensure(K + 1);
+        PRX_HIP(hipMemcpyAsync(B->coarse.pt_off.p, zero.data(), sizeof(uint32_t) * (K + 1), hipMemcpyHostToDevice, B->stream));
+        PRX_HIP(hipStreamSynchronize(B->stream));
+    }
+    B->pctl.ensure(1);
+    B->pinst.ensure(K);
+    B->plist_i.ensure(3 * (int64_t)K);
+    B->plist_u.ensure(12 * ((int64_t)K + 1) + 3 * (int64_t)K);
     B->st.ensure(K);
     if (B->st_host) (void)hipHostFree(B->st_host);
     B->st_host = nullptr;
@@ -594,9 +626,74 @@ struct BOut {
     std::vector<int> status;
 };
 
+// ------------------------------------------------------------------ persistent launches
+// The fields of a persistent CG launch common to both precisions: the level's
+// tables, points, work-list storage, control block and scratch.
+template <typename T>
+static PArgs<T> persist_args(prx_batch* B, BLevel& L) {
+    PArgs<T> a;
+    a.tab = tab_all(B, L);
+    a.N = L.N;
+    a.P = L.P;
+    a.cell = L.cell.p;
+    a.t64 = L.t64.p;
+    if constexpr (sizeof(T) == 4) a.tt = (const T*)L.t32.p;
+    else a.tt = (const T*)L.t64.p;
+    a.pinst = L.pinst.p;
+    a.pt_off = L.pt_off.p;
+    a.partial = B->partial.p;
+    a.partial2 = B->partial.p + (L.N >> 10) + 16;
+    a.flag = B->flags.p;
+    a.pi = B->pinst.p;
+    const int64_t K = L.K, K1 = K + 1;
+    PList* ls[3] = {&a.run, &a.repl, &a.xt};
+    for (int l = 0; l < 3; ++l) {
+        ls[l]->inst = B->plist_i.p + l * K;
+        ls[l]->gbrick = B->plist_u.p + 12 * K1 + l * K;
+        ls[l]->brick = B->plist_u.p + (4 * l + 0) * K1;
+        ls[l]->pts = B->plist_u.p + (4 * l + 1) * K1;
+        ls[l]->vblk = B->plist_u.p + (4 * l + 2) * K1;
+        ls[l]->kblk = B->plist_u.p + (4 * l + 3) * K1;
+    }
+    a.ctl = B->pctl.p;
+    return a;
+}
+
+// One persistent launch over level L with the per-instance state pi; the CG
+// states are read back into st_host.
+template <typename T>
+static void run_persistent(prx_batch* B, BLevel& L, const PArgs<T>& a, std::vector<PInst>& pi) {
+    const hipStream_t s = B->stream;
+    const int K = L.K;
+    std::vector<int32_t> zero(K, 0);
+    PCtl c{};
+    PRX_HIP(hipMemcpyAsync(B->flags.p, zero.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(B->pinst.p, pi.data(), sizeof(PInst) * K, hipMemcpyHostToDevice, s));
+    PRX_HIP(hipMemcpyAsync(B->pctl.p, &c, sizeof(PCtl), hipMemcpyHostToDevice, s));
+    int blocks = 0;
+    B->pargs.ensure((int64_t)sizeof(PArgs<double>));
+    launch_persistent_cg<T>(a, L.scr_ptr(), B->pargs.p, &blocks, s);
+    PRX_HIP(hipMemcpyAsync(&c, B->pctl.p, sizeof(PCtl), hipMemcpyDeviceToHost, s));
+    PRX_HIP(hipMemcpyAsync(pi.data(), B->pinst.p, sizeof(PInst) * K, hipMemcpyDeviceToHost, s));
+    read_states(B);
+    B->persistent_launches++;
+    B->persistent_rounds += (uint64_t)c.rounds;
+    B->persistent_blocks = blocks;
+    B->persistent_xcd_match = (int)c.xcd_rr;
+    int khz = 0;
+    PRX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, B->device));
+    for (int i = 0; i < 4; ++i) B->persistent_phase_ms[i] += khz > 0 ? (double)c.ticks[i] / (double)khz : 0.0;
+    if (std::getenv("PRX_PERSIST_TRACE") && khz > 0)
+        std::fprintf(stderr, "persistent: %lld rounds, pass phase %.1f ms, mean busy per workgroup %.1f ms\n", (long long)c.rounds,
+                     (double)c.ticks[2] / khz, (double)c.pass_busy / khz / blocks);
+    if (c.err == 1) throw Error(PRX_ERR_DEVICE, "persistent CG: a grid barrier wait exceeded its bound");
+    if (c.err == 2) throw Error(PRX_ERR_DEVICE, "persistent CG: the round cap was reached");
+    if (c.err == 3) throw Error(PRX_ERR_DEVICE, "persistent CG: the launch ran past its 60 s deadline");
+}
+
 // CG over every instance of level L from its current x (solve_cg,
 // src/solvers.rs:20-69, per instance), iterations graph-replayed in chunks.
-static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, int mode, int refresh) {
+static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, int mode, int refresh, bool persistent = false) {
     const hipStream_t s = B->stream;
     const int K = L.K;
     const BatchTab all = tab_all(B, L);
@@ -626,6 +723,48 @@ static BOut batch_cg(prx_batch* B, BLevel& L, uint64_t max_iters, double eps, in
     if (B->fused) {  // replays keep pass index = ring position: a multiple of CG_XM (and of the refresh)
         const int unit = refresh > 0 ? std::lcm(refresh, CG_XM) : CG_XM;
         chunk = std::max(unit, (chunk + unit / 2) / unit * unit);
+    }
+    if (persistent) {
+        // every iteration, confirmation and restart of every instance in one launch
+        if (mode != PRX_RESID_TRUE || refresh > 0 || !B->fused) throw Error(PRX_ERR_INVALID, "persistent CG: true-residual mode, fused step");
+        read_states(B);
+        PArgs<double> a = persist_args<double>(B, L);
+        a.max_iters = (int64_t)max_iters;
+        a.max_rounds = (int64_t)max_iters + 16;
+        a.x = L.x.p;
+        a.r = L.r.p;
+        a.q = L.q.p;
+        a.ring = L.dd.p;
+        a.sbuf = L.sbuf.p;
+        a.sbuf64 = L.sbuf.p;
+        a.b = L.b.p;
+        std::vector<PInst> pi(K);
+        for (int k = 0; k < K; ++k) {
+            pi[k] = PInst{};
+            pi[k].last_rt = INFINITY;
+            pi[k].fin = B->st_host[k].done ? 1 : 0;
+            pi[k].status = B->st_host[k].status;
+        }
+        run_persistent<double>(B, L, a, pi);
+        // x terms still pending (instances stopped without a confirmation)
+        launch_bx_terms<double>(all, L.x.p, XDirs<double>{}, L.dd.p, L.N, 1, s);
+        for (int k = 0; k < K; ++k) {
+            B->st_host[k].xpend = 0;
+            B->st_host[k].xn = 0;
+        }
+        write_states(B);
+        BOut o;
+        o.iters.resize(K);
+        o.rel.resize(K);
+        o.status.resize(K);
+        for (int k = 0; k < K; ++k) {
+            const CgState& h = B->st_host[k];
+            o.iters[k] = (uint64_t)h.it;
+            o.status[k] = h.done ? h.status : PRX_ERR_NOT_CONVERGED;
+            o.rel[k] = bb[k] > 0.0 ? std::sqrt(h.delta / bb[k]) : 0.0;
+        }
+        PRX_HIP(hipStreamSynchronize(s));
+        return o;
     }
     const hipGraphExec_t g = graph_for(B, L, 0, chunk, refresh);
     std::vector<char> fin(K, 0);
@@ -754,7 +893,7 @@ struct MixedOut {
 // direction (a reliable update); an instance stops on its FP64 true residual
 // ||b - A x|| <= eps ||b||.
 static MixedOut batch_fine_mixed(prx_batch* B, BLevel& F, int gs, uint64_t max_iters, double eps, int mode, double theta,
-                                 uint64_t max_outer) {
+                                 uint64_t max_outer, bool persistent = false) {
     const hipStream_t s = B->stream;
     const int K = F.K;
     const int64_t N = F.N;
@@ -826,6 +965,48 @@ static MixedOut batch_fine_mixed(prx_batch* B, BLevel& F, int gs, uint64_t max_i
         h.xn = 0;
     }
     write_states(B);
+    if (persistent) {
+        // every iteration, replacement and restart of every instance in one launch
+        if (B->faces) throw Error(PRX_ERR_INVALID, "persistent CG: the stored-q fused step");
+        PArgs<float> a = persist_args<float>(B, F);
+        a.max_iters = (int64_t)max_iters;
+        a.max_outer = (int64_t)max_outer;
+        a.max_rounds = (int64_t)max_iters + 16;
+        a.theta = theta;
+        a.x = F.e32.p;
+        a.r = F.r32.p;
+        a.q = F.q32.p;
+        a.ring = F.dd32.p;
+        a.sbuf = F.sbuf32.p;
+        a.x64 = F.x.p;
+        a.x2 = F.x2.p;
+        a.b = F.b.p;
+        a.sbuf64 = F.sbuf.p;
+        std::vector<PInst> pi(K);
+        for (int k = 0; k < K; ++k) {
+            pi[k] = PInst{};
+            pi[k].thr = thr[k];
+            pi[k].thr32 = thr32[k];
+            pi[k].last_rt = INFINITY;
+            pi[k].rr = rr[k];
+            pi[k].fin = fin[k];
+            pi[k].status = PRX_OK;
+        }
+        run_persistent<float>(B, F, a, pi);
+        out.cg.iters.resize(K);
+        out.cg.rel.resize(K);
+        out.cg.status.resize(K);
+        for (int k = 0; k < K; ++k) {
+            out.outer[k] = (uint64_t)pi[k].outer;
+            out.cg.iters[k] = (uint64_t)B->st_host[k].it;
+            out.cg.status[k] = pi[k].fin ? pi[k].status : PRX_ERR_NOT_CONVERGED;
+            out.true_rel[k] = bb[k] > 0.0 ? std::sqrt(pi[k].rr / bb[k]) : 0.0;
+            const double den = mode == PRX_RESID_TRUE ? bb[k] : r0[k];
+            out.cg.rel[k] = den > 0.0 ? std::sqrt(pi[k].rr / den) : 0.0;
+        }
+        PRX_HIP(hipStreamSynchronize(s));
+        return out;
+    }
     int chunk = B->chunk;
     chunk = std::max(CG_XM, (chunk + CG_XM / 2) / CG_XM * CG_XM);  // replays keep pass index = ring position
     const hipGraphExec_t g = graph_for(B, F, 0, chunk, 0, 1);
@@ -1123,6 +1304,13 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     if (mixed) fine_vectors(false, true);
     const int mode = o.residual_mode == PRX_RESID_TRUE ? PRX_RESID_TRUE : PRX_RESID_REFERENCE;
     const int refresh = mode == PRX_RESID_TRUE ? 0 : (o.refresh_period > 0 ? o.refresh_period : 0);
+    // one persistent launch per level's CG: true-residual stopping with the
+    // fused step (q stored); AUTO keeps the graph replays for now
+    const bool pers_ok = mode == PRX_RESID_TRUE && B->fused && !B->faces;
+    if (o.batch_launch == PRX_BATCH_LAUNCH_PERSISTENT && !pers_ok)
+        throw Error(PRX_ERR_INVALID, "the persistent batched CG needs PRX_RESID_TRUE and the fused CG step with q stored");
+    const bool persistent = o.batch_launch == PRX_BATCH_LAUNCH_PERSISTENT;
+    for (double& v : B->persistent_phase_ms) v = 0.0;
     Timer total(s);
     double ms_coarse = 0.0, ms_gs = 0.0, ms_cg = 0.0;
     BOut co, fo;
@@ -1139,7 +1327,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
         launch_fill<double>(C.x.p, 0.0, C.N, s);
         if (gs > 0) launch_graph(B, graph_for(B, C, 1, (int)gs, 0));
         const double ceps = mode == PRX_RESID_TRUE && o.coarse_eps > 0.0 ? o.coarse_eps : eps;
-        co = batch_cg(B, C, cg_iters, ceps, mode, mode == PRX_RESID_TRUE ? 0 : 10);
+        co = batch_cg(B, C, cg_iters, ceps, mode, mode == PRX_RESID_TRUE ? 0 : 10, persistent);
         for (int k = 0; k < K; ++k) {
             prx_tree* t = B->inst[k];
             launch_prolongate(t->D, t->D - 2, C.x.p + C.off_h[k], F.x.p + F.off_h[k], s);
@@ -1150,7 +1338,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     if (mixed) {
         Timer tm(s);
         const double theta = o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.05;
-        MixedOut mo = batch_fine_mixed(B, F, (int)gs, cg_iters / 4, eps, mode, theta, o.max_outer > 0 ? o.max_outer : 64);
+        MixedOut mo = batch_fine_mixed(B, F, (int)gs, cg_iters / 4, eps, mode, theta, o.max_outer > 0 ? o.max_outer : 64, persistent);
         ms_gs = mo.ms_gs;
         ms_cg = tm.ms() - mo.ms_gs;
         fo = mo.cg;
@@ -1162,7 +1350,7 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
             ms_gs = tm.ms();
         }
         Timer tm(s);
-        fo = batch_cg(B, F, cg_iters / 4, eps, mode, refresh);
+        fo = batch_cg(B, F, cg_iters / 4, eps, mode, refresh, persistent);
         ms_cg = tm.ms();
     }
     const double ms_total = total.ms();
@@ -1191,6 +1379,11 @@ static void solve_batched(prx_batch* B, uint32_t gs, uint64_t cg_iters, double e
     R.chunk = B->chunk;
     R.gs_block = B->gs_block;
     R.graph_launches = B->graph_launches;
+    R.persistent_launches = B->persistent_launches;
+    R.persistent_rounds = B->persistent_rounds;
+    R.persistent_blocks = B->persistent_blocks;
+    R.persistent_xcd_match = B->persistent_xcd_match;
+    for (int i = 0; i < 4; ++i) R.persistent_phase_ms[i] = B->persistent_phase_ms[i];
     R.status = PRX_OK;
     for (int k = 0; k < K; ++k) {
         const double bb = B->st_host[k].slot[2], rr = B->st_host[k].slot[3];

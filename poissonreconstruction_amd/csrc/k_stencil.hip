@@ -613,24 +613,6 @@ template void launch_gs_color<float>(const LevelView&, int, float*, const float*
 
 // ------------------------------------------------------------------ batched grids
 template <typename T>
-static LevelArgs<T> batch_args(const NodeList* scr, int tile_log) {
-    LevelArgs<T> a{};
-    a.has_screen = (scr && scr->T > 0) ? 1 : 0;
-    if (a.has_screen) {
-        a.brick_log = scr->brick_log;
-        a.brick = scr->brick;
-        if (tile_log == 9 && scr->brick8) { a.brick = scr->brick8; a.brick_log = 9; }
-        a.node = scr->node;
-        a.off = scr->off;
-        a.sp = scr->sp;
-        if constexpr (sizeof(T) == 8) a.phi = (const T*)scr->phi;
-        else a.phi = (const T*)scr->phi32;
-        a.diag = scr->diag;
-    }
-    return a;
-}
-
-template <typename T>
 void launch_bstencil(const BatchTab& tab, int64_t Ntot, const NodeList* scr, int op, const StencilIO<T>& io, int tile,
                      hipStream_t s) {
     const int B = tile == 8 ? 8 : 16, tl = B == 16 ? 12 : 9;

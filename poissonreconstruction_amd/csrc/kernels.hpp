@@ -297,6 +297,115 @@ void launch_bcommit(const BatchTab& tab, double* x, const double* x2, float* e, 
 template <typename T>
 int launch_bdot(const BatchTab& tab, const T* a, const T* b, int64_t Ntot, double* partial, hipStream_t s);
 
+// The operator passes' view of a level: grid size, weights (as stored in
+// the level, or per instance from a BatchTab) and the screening lists.
+template <typename T>
+struct LevelArgs {
+    uint32_t D, res;
+    T ce, cc, wg, c0;  // c0 = -(12ce + 8cc) = 32s, the diagonal of L
+    T s;               // weight scale: ce = -2s, cc = -s
+    int has_screen;
+    int brick_log;
+    const uint32_t* brick;
+    const uint32_t* node;
+    const uint32_t* off;
+    const uint32_t* sp;
+    const T* phi;
+    const double* diag;  // sum phi^2 per touched node
+    T wh3;
+    // when set, the tile kernels recompute each entry's weight from its point
+    // (spc = 8*sorted point + corner, tt = the points' in-cell coordinates)
+    const uint32_t* spc;
+    const T* tt;
+    int64_t P;
+};
+
+// ---- persist.hip: the CG of every instance of a batched level in one
+// persistent launch (all workgroups co-resident: a cooperative launch).
+// Rounds in lockstep: in round k every running instance makes fused CG pass k
+// (the ping-pong slot k & 1, ring slot k % CG_XM); instances whose CG stopped
+// first get their replacement (mixed: FP64 true residual) or confirmation
+// (FP64: the true residual) in the same round, and the device decides, per
+// instance, whether it continues.  Phases are separated by a grid barrier;
+// the last workgroup to finish a round's fold builds the next round's work
+// lists (running instances, instances to replace, instances whose x terms are
+// due), so finished instances cost nothing.
+constexpr int PBAR_GROUPS = 32;  // first level of the grid barrier: workgroup w arrives at group w % PBAR_GROUPS
+struct PCtl {
+    uint32_t bar_count, bar_gen;
+    int32_t err;                 // 1: a barrier wait exceeded its bound, 2: the round cap was reached, 3: the deadline passed
+    int32_t nrun, nrepl, nxt;    // list sizes for the next round
+    uint32_t ticket;             // completed fold-phase instances (the last one builds the lists)
+    int32_t pad;
+    int64_t rounds;
+    uint64_t ticks[4];           // wall-clock ticks per phase kind (replacement, gather, pass, fold), block 0's view
+    uint32_t xcd_count[8];       // workgroups per XCD (as its XCC_ID register reads)
+    uint32_t xcd_rr;             // workgroups w found on XCD w % 8
+    uint64_t pass_busy;          // sum over workgroups and rounds of the time from a pass phase's start to its last brick
+    uint32_t group[PBAR_GROUPS][16];  // one 64-byte line per first-level counter
+};
+struct PInst {
+    double thr, thr32, last_rt, rr;  // FP64 threshold, FP32 one (mixed), last true residual, last |b - A x|^2
+    int32_t fin, status, outer, pad;
+};
+// instances with prefix sums of their 16^3 bricks, screening points, blocks
+// of 256 16-byte vectors and 1024-node blocks ([n+1] each)
+struct PList {
+    int32_t* inst = nullptr;
+    uint32_t* gbrick = nullptr;  // [n] the instance's first 16^3 brick in the level
+    uint32_t* brick = nullptr;
+    uint32_t* pts = nullptr;
+    uint32_t* vblk = nullptr;
+    uint32_t* kblk = nullptr;
+};
+template <typename T>
+struct PArgs {
+    BatchTab tab;                // off, brick_inst, coef, st of the level
+    int64_t N = 0;
+    int64_t max_iters = 0, max_outer = 0, max_rounds = 0;
+    uint64_t deadline = 0;       // wall-clock ticks the launch may run (set by the launcher)
+    double theta = 0.0;
+    LevelArgs<T> L{};            // screening lists for the tile pass (T weights)
+    // the level's screening list (FP64 weights; the mixed residual)
+    int has_scr = 0;
+    const uint32_t* s_brick = nullptr;
+    const uint32_t* s_node = nullptr;
+    const uint32_t* s_off = nullptr;
+    const uint32_t* s_sp = nullptr;
+    const double* s_phi = nullptr;
+    // sorted points, instance by instance (pt_off[K+1])
+    int64_t P = 0;
+    const uint32_t* cell = nullptr;
+    const T* tt = nullptr;
+    const double* t64 = nullptr;
+    const uint32_t* pinst = nullptr;
+    const uint32_t* pt_off = nullptr;
+    // CG vectors of type T: x (FP32 correction e in the mixed solve), r[2N], q[2N], ring[CG_XM N], sbuf[P]
+    T* x = nullptr;
+    T* r = nullptr;
+    T* q = nullptr;
+    T* ring = nullptr;
+    T* sbuf = nullptr;
+    // FP64: the solution x64, x2 = x64 + e, b, per-point screening values
+    double* x64 = nullptr;
+    double* x2 = nullptr;
+    const double* b = nullptr;
+    double* sbuf64 = nullptr;
+    double* partial = nullptr;   // per 16^3 brick (four in a CG pass)
+    double* partial2 = nullptr;  // per 1024-node block
+    int32_t* flag = nullptr;     // [K] 1: replace (confirm) in the next round
+    PInst* pi = nullptr;
+    PList run, repl, xt;
+    PCtl* ctl = nullptr;
+};
+// Launches k_persist for the batched level (T float: the mixed solve's FP32
+// correction CG with FP64 residual replacements; T double: the FP64 CG with
+// true-residual confirmation).  *blocks receives the grid size used.
+// scr: the level's screening list (null: none); dargs: device memory of at
+// least sizeof(PArgs<T>) for the argument block.  Returns when the launch ends.
+template <typename T>
+void launch_persistent_cg(const PArgs<T>& a, const NodeList* scr, void* dargs, int* blocks, hipStream_t s);
+
 // ---- k_adjoint.hip
 // gx[node] = sum over the node's evaluate entries of gF[p] * phi (input order)
 void launch_adj_load(const NodeList& E, const double* gF, double* gx, hipStream_t s);

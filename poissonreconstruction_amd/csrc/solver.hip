@@ -1440,6 +1440,7 @@ void prx_solve_opts_default(prx_solve_opts* o) {
     o->mg_inner_tol = 1e-4;
     o->coarse_eps = 0.0;
     o->cg_form = PRX_CG_FUSED;
+    o->batch_launch = PRX_BATCH_LAUNCH_AUTO;
 }
 
 int prx_tree_create(uint32_t depth, int device, prx_tree** out) {

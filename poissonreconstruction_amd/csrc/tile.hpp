@@ -8,26 +8,6 @@
 
 namespace prx {
 
-template <typename T>
-struct LevelArgs {
-    uint32_t D, res;
-    T ce, cc, wg, c0;  // c0 = -(12ce + 8cc) = 32s, the diagonal of L
-    T s;               // weight scale: ce = -2s, cc = -s
-    int has_screen;
-    int brick_log;
-    const uint32_t* brick;
-    const uint32_t* node;
-    const uint32_t* off;
-    const uint32_t* sp;
-    const T* phi;
-    const double* diag;  // sum phi^2 per touched node
-    T wh3;
-    // when set, the tile kernels recompute each entry's weight from its point
-    // (spc = 8*sorted point + corner, tt = the points' in-cell coordinates)
-    const uint32_t* spc;
-    const T* tt;
-    int64_t P;
-};
 
 template <typename T>
 static LevelArgs<T> make_args(const LevelView& L, int tile_log) {
@@ -269,7 +249,12 @@ struct TileLds {
     double wsum[4 * ((NT + 63) / 64)];
     T gdiag[OP == OP_GS ? B3 / 8 : 1];  // screening diagonal of the colour's touched nodes
 };
-template <typename T, int B, int OP, bool BATCH = false>
+// LOOPED: the caller runs the pass over many bricks in one workgroup (a
+// persistent kernel).  The thread index is then passed through an empty asm,
+// so the compiler cannot hoist the brick-independent index arithmetic out of
+// the caller's loops and keep it live across them (measured: 173 instead of
+// 127 VGPRs for a loop over this pass).
+template <typename T, int B, int OP, bool BATCH = false, bool LOOPED = false>
 __device__ __forceinline__ void stencil_tile_brick(const LevelArgs<T>& L, const StencilIO<T>& io, int swz, const BatchTab& tab,
                                                    uint32_t vbid, TileLds<T, B, OP>& lds) {
     if constexpr (!BATCH) {
@@ -283,7 +268,8 @@ __device__ __forceinline__ void stencil_tile_brick(const LevelArgs<T>& L, const 
     T* tile = lds.tile;
     double* wsum = lds.wsum;
     T* gdiag = lds.gdiag;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    if constexpr (LOOPED) asm volatile("" : "+v"(tid));
     if constexpr (OP == OP_GS)
         for (int i = tid; i < B3 / 8; i += NT) gdiag[i] = T(0);
     // the weights and the residual scale (a batch instance's own replace the level's)
@@ -792,6 +778,25 @@ inline int xcd_chunk(uint32_t nbricks) {
     uint32_t c = nbricks >> 3;
     while (c > (uint32_t)PRX_XCD_CHUNK && (c & 1u) == 0) c >>= 1;
     return (int)c;
+}
+
+// level arguments of a batched level (weights come per instance from the BatchTab)
+template <typename T>
+static LevelArgs<T> batch_args(const NodeList* scr, int tile_log) {
+    LevelArgs<T> a{};
+    a.has_screen = (scr && scr->T > 0) ? 1 : 0;
+    if (a.has_screen) {
+        a.brick_log = scr->brick_log;
+        a.brick = scr->brick;
+        if (tile_log == 9 && scr->brick8) { a.brick = scr->brick8; a.brick_log = 9; }
+        a.node = scr->node;
+        a.off = scr->off;
+        a.sp = scr->sp;
+        if constexpr (sizeof(T) == 8) a.phi = (const T*)scr->phi;
+        else a.phi = (const T*)scr->phi32;
+        a.diag = scr->diag;
+    }
+    return a;
 }
 
 }  // namespace prx

@@ -160,3 +160,50 @@ def test_batch_multigrid_matches_single_tree_multigrid(gpu, prec):
                                   precond=pr.PRX_PRECOND_MG)
             assert per[k]["fine_iters"] == rs["fine_iters"], (rnd, k)
             assert np.array_equal(B.tree(k).solution, t.solution), (rnd, k)
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_batch_persistent_launch_matches_single_tree_solves(gpu, prec):
+    # each level's CG of every instance in one persistent launch (iterations,
+    # residual replacements or confirmations and stops decided on the device)
+    # against each cloud solved alone: both reach the FP64 true residual, the
+    # solutions agree to what that residual fixes, the iteration counts match
+    cs = clouds()
+    B = Batch()
+    for D, pts, pw in cs:
+        B.add(D, pts, pw)
+    eps = 1e-10 if prec == pr.PRX_PREC_FP64 else 1e-9
+    for rnd in range(2):
+        rep, per = B.solve(eps=eps, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False, precision=prec,
+                           launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+        print(f"  round {rnd}: {rep['persistent_launches']} persistent launches, {rep['persistent_rounds']} rounds, "
+              f"{rep['persistent_blocks']} workgroups, {rep['ms_total']:.1f} ms", flush=True)
+        assert rep["instances"] == len(cs) and rep["converged"] == len(cs) and rep["status"] == pr.PRX_OK
+        assert rep["persistent_launches"] == 2 * (rnd + 1) and rep["max_true_rel"] <= eps * 1.0001
+        for k, (D, pts, pw) in enumerate(cs):
+            t = pr.FEMTree(D)
+            t.splat_normal_field(pts)
+            rs = t.solve_cascadic(pts, pw, D - 2, 20, 800000, eps, precision=prec, residual_mode=pr.PRX_RESID_TRUE,
+                                  refresh_period=0)
+            xs, xb = t.solution, B.tree(k).solution
+            rng_f = xs.max() - xs.min()
+            print(f"    instance {k} depth {D}: {per[k]['fine_iters']} fine / {per[k]['coarse_iters']} coarse iterations "
+                  f"({per[k]['outer_iters']} replacements), alone {rs['fine_iters']} / {rs['coarse_iters']}; "
+                  f"max diff {np.abs(xb - xs).max() / rng_f:.2e} of range", flush=True)
+            assert per[k]["status"] == pr.PRX_OK and per[k]["true_rel"] <= eps * 1.0001
+            if prec == pr.PRX_PREC_FP64:
+                assert np.abs(xb - xs).max() <= 1e-9 * np.abs(xs).max(), (rnd, k)
+                assert abs(per[k]["fine_iters"] - rs["fine_iters"]) <= 2, (rnd, k)
+                assert abs(per[k]["coarse_iters"] - rs["coarse_iters"]) <= 2, (rnd, k)
+            else:
+                assert np.abs(xb - xs).max() <= 1e-6 * rng_f, (rnd, k)
+                assert abs(per[k]["fine_iters"] - rs["fine_iters"]) <= max(4, 0.1 * rs["fine_iters"]), (rnd, k)
+
+
+def test_batch_persistent_launch_rejects_what_it_does_not_run(gpu):
+    B = Batch()
+    for D, pts, pw in clouds()[:2]:
+        B.add(D, pts, pw)
+    with pytest.raises(pr.PrxError):  # the reference's stopping test with periodic refresh stays on graph replays
+        B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_REFERENCE, refresh_period=10, autotune=False,
+                launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
