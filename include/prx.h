@@ -139,8 +139,9 @@ typedef struct prx_solve_opts {
      * rounds and decide replacements, confirmations and stops on the device;
      * PRX_RESID_TRUE with the fused CG step); PRX_BATCH_LAUNCH_GRAPHS replays
      * captured chunks of iterations with host checks between them;
-     * PRX_BATCH_LAUNCH_AUTO (0, the default) picks the persistent launch where
-     * it applies. */
+     * PRX_BATCH_LAUNCH_AUTO (0, the default) picks the graph replays, the
+     * faster of the two on many-cloud batches.  An explicit persistent request
+     * that does not apply returns PRX_ERR_INVALID. */
     int32_t batch_launch;
 } prx_solve_opts;
 

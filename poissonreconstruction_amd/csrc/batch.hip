@@ -689,6 +689,10 @@ static void run_persistent(prx_batch* B, BLevel& L, const PArgs<T>& a, std::vect
     if (c.err == 1) throw Error(PRX_ERR_DEVICE, "persistent CG: a grid barrier wait exceeded its bound");
     if (c.err == 2) throw Error(PRX_ERR_DEVICE, "persistent CG: the round cap was reached");
     if (c.err == 3) throw Error(PRX_ERR_DEVICE, "persistent CG: the launch ran past its 60 s deadline");
+    if (c.err == 4)
+        throw Error(PRX_ERR_DEVICE, "persistent CG: a brick phase processed " + std::to_string(c.bad_got) + " bricks of " +
+                                        std::to_string(c.bad_want) + " (round " + std::to_string(c.bad_round / 2) +
+                                        (c.bad_round & 1 ? ", CG pass)" : ", residual pass)"));
 }
 
 // CG over every instance of level L from its current x (solve_cg,

@@ -11,15 +11,26 @@
 
 namespace prx {
 
+// The thread's index in its workgroup, passed through an empty asm statement:
+// nothing computed from it can be hoisted out of a loop around the caller, so
+// the persistent CG kernel's round loop does not carry these values (in
+// registers the tile pass needs) from one phase to the next.  The value is
+// threadIdx.x.
+__device__ __forceinline__ int wg_tid() {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // Deterministic block sum: 64-lane butterfly, then waves in order.
 __device__ inline double block_sum_256(double v, double* wsum) {
 #pragma clang fp contract(off)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = wg_tid() & 63, wave = wg_tid() >> 6;
     if (lane == 0) wsum[wave] = v;
     __syncthreads();
     double s = 0.0;
-    if (threadIdx.x == 0)
+    if (wg_tid() == 0)
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += wsum[w];
     return s;
 }
@@ -219,7 +230,7 @@ __device__ inline double fin_sum(const double* partial, int64_t np, double* ws) 
     double a[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) a[u] = 0.0;
-    int64_t i = threadIdx.x;
+    int64_t i = wg_tid();
     for (; i + 15 * NTH < np; i += 16 * NTH) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) a[u] += partial[STRIDE * (i + u * NTH)];
@@ -248,7 +259,7 @@ __device__ inline void fold4_block(const double* __restrict__ partial, int np, i
     const int g = fold_blocks(np), chunk = (np + g - 1) / g;
     const int b0 = b * chunk, b1 = min(np, b0 + chunk);
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = b0 + (int)threadIdx.x; i < b1; i += 256) {
+    for (int i = b0 + (int)wg_tid(); i < b1; i += 256) {
         const nd2 u = *(const nd2*)(partial + 4 * (int64_t)i);
         const nd2 v = *(const nd2*)(partial + 4 * (int64_t)i + 2);
         a[0] += u[0];
@@ -256,17 +267,17 @@ __device__ inline void fold4_block(const double* __restrict__ partial, int np, i
         a[2] += v[0];
         a[3] += v[1];
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = wg_tid() & 63, wave = wg_tid() >> 6;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_xor(a[j], o, 64);
         if (lane == 0) ws[j * 4 + wave] = a[j];
     }
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (wg_tid() < 4) {
         double sum = 0.0;
-        for (int w = 0; w < 4; ++w) sum += ws[threadIdx.x * 4 + w];
-        out[4 * b + threadIdx.x] = sum;
+        for (int w = 0; w < 4; ++w) sum += ws[wg_tid() * 4 + w];
+        out[4 * b + wg_tid()] = sum;
     }
 }
 
@@ -314,7 +325,7 @@ __device__ inline void bx_terms_block(const BatchTab& tab, T* __restrict__ x, co
     double a[CG_XM];
 #pragma unroll
     for (int j = 0; j < CG_XM; ++j) a[j] = st->xa[j];
-    const int64_t g = (v0 + threadIdx.x) * VEC;
+    const int64_t g = (v0 + wg_tid()) * VEC;
     const NV xv = ld_nt(x + g);
     NV dv[CG_XM];
 #pragma unroll
@@ -339,7 +350,7 @@ __device__ inline void bcommit_block(const BatchTab& tab, double* __restrict__ x
                                      uint32_t blk) {
     typedef double nd2 __attribute__((ext_vector_type(2)));
     typedef float nf4 __attribute__((ext_vector_type(4)));
-    const int64_t j = ((int64_t)blk * 256 + threadIdx.x) * 4;
+    const int64_t j = ((int64_t)blk * 256 + wg_tid()) * 4;
     const uint32_t k = tab.brick_inst[((uint64_t)blk * 1024) >> 12];
     if (!b_runs(tab, k)) return;
     const nd2 a = *(const nd2*)(x2 + j), b = *(const nd2*)(x2 + j + 2);
@@ -354,7 +365,7 @@ __device__ inline void bdot_block(const BatchTab& tab, const T* __restrict__ a, 
 #pragma clang fp contract(off)
     using NV = typename NV16<T>::type;
     constexpr int VEC = 16 / (int)sizeof(T);
-    const int64_t j = ((int64_t)blk * 256 + threadIdx.x) * VEC;
+    const int64_t j = ((int64_t)blk * 256 + wg_tid()) * VEC;
     const uint32_t k = tab.brick_inst[(uint64_t)blk * 256 * VEC >> 12];
     if (!b_runs(tab, k)) return;
     const NV av = ld_v(a + j), bv = ld_v(b + j);
@@ -362,7 +373,7 @@ __device__ inline void bdot_block(const BatchTab& tab, const T* __restrict__ a, 
 #pragma unroll
     for (int e = 0; e < VEC; ++e) acc += (double)av[e] * (double)bv[e];
     const double sm = block_sum_256(acc, ws);
-    if (threadIdx.x == 0) partial[blk] = sm;
+    if (wg_tid() == 0) partial[blk] = sm;
 }
 
 // The same gather over a batch: point i belongs to instance pts.inst[i]

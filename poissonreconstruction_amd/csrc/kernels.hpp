@@ -333,7 +333,7 @@ struct LevelArgs {
 constexpr int PBAR_GROUPS = 32;  // first level of the grid barrier: workgroup w arrives at group w % PBAR_GROUPS
 struct PCtl {
     uint32_t bar_count, bar_gen;
-    int32_t err;                 // 1: a barrier wait exceeded its bound, 2: the round cap was reached, 3: the deadline passed
+    int32_t err;                 // 1: a barrier wait exceeded its bound, 2: the round cap was reached, 3: the deadline passed, 4: see done
     int32_t nrun, nrepl, nxt;    // list sizes for the next round
     uint32_t ticket;             // completed fold-phase instances (the last one builds the lists)
     int32_t pad;
@@ -343,6 +343,14 @@ struct PCtl {
     uint32_t xcd_rr;             // workgroups w found on XCD w % 8
     uint64_t pass_busy;          // sum over workgroups and rounds of the time from a pass phase's start to its last brick
     uint32_t group[PBAR_GROUPS][16];  // one 64-byte line per first-level counter
+    // brick positions taken so far per (brick phase: replacement, CG pass;
+    // round parity; XCD), one 128-byte line each
+    uint32_t take[2][2][8][32];
+    // bricks processed per (brick phase, round parity), checked against the
+    // list after the phase (err 4: a brick phase missed or repeated bricks)
+    uint32_t done[2][2][32];
+    int64_t bad_round;
+    uint32_t bad_got, bad_want;
 };
 struct PInst {
     double thr, thr32, last_rt, rr;  // FP64 threshold, FP32 one (mixed), last true residual, last |b - A x|^2

@@ -54,11 +54,18 @@ __device__ __forceinline__ void st_ctl(int32_t* p, int32_t v) {
 // G / PBAR_GROUPS + PBAR_GROUPS instead of G.  Returns false when the launch
 // is to end (a wait exceeded its bound here or elsewhere).
 constexpr uint64_t PERSIST_SPIN_LIMIT = 1ull << 22;
+// Each wave waits for its own global stores to be acknowledged (s_waitcnt
+// vmcnt(0)) before a workgroup barrier after which one thread releases the
+// workgroup's writes at agent scope: the workgroup barrier does not wait for
+// the stores of the other waves, and a store still in flight when the L2 is
+// written back could reach another XCD's reader stale.
+__device__ __forceinline__ void own_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 __device__ inline bool gsync(PCtl* c, uint32_t& gen, int* sbad) {
+    own_stores_done();
     __syncthreads();
     const uint32_t target = gen + 1;
     gen = target;
-    if (threadIdx.x == 0) {
+    if (wg_tid() == 0) {
         int bad = 0;
         const uint32_t G = gridDim.x, w = blockIdx.x;
         const uint32_t ng = G < (uint32_t)PBAR_GROUPS ? G : (uint32_t)PBAR_GROUPS, g = w % ng;
@@ -132,7 +139,7 @@ __device__ __forceinline__ WgRange wg_range(uint32_t tot, uint32_t align) {
 
 // Exclusive block scan of v (256 threads); *total gets the sum.
 __device__ inline uint32_t block_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = wg_tid() & 63, wave = wg_tid() >> 6;
     uint32_t x = v;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(x, o, 64);
@@ -157,10 +164,10 @@ __device__ inline void build_lists(const PArgs<T>& a, uint32_t* tmp) {
     constexpr int VEC = 16 / (int)sizeof(T);
     const int K = a.tab.K;
     __shared__ uint32_t acc[3][5];
-    if (threadIdx.x < 15) (&acc[0][0])[threadIdx.x] = 0;
+    if (wg_tid() < 15) (&acc[0][0])[wg_tid()] = 0;
     __syncthreads();
     for (int k0 = 0; k0 < K; k0 += 256) {
-        const int k = k0 + (int)threadIdx.x;
+        const int k = k0 + wg_tid();
         uint32_t mem[3] = {0, 0, 0}, nb = 0, np = 0, nv = 0, n1 = 0;
         if (k < K) {
             const CgState& st = a.tab.st[k];
@@ -194,7 +201,7 @@ __device__ inline void build_lists(const PArgs<T>& a, uint32_t* tmp) {
                 pl.kblk[j] = acc[l][4] + e4;
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
+            if (wg_tid() == 0) {
                 acc[l][0] += t0;
                 acc[l][1] += t1;
                 acc[l][2] += t2;
@@ -204,8 +211,8 @@ __device__ inline void build_lists(const PArgs<T>& a, uint32_t* tmp) {
             __syncthreads();
         }
     }
-    if (threadIdx.x < 3) {
-        const int l = threadIdx.x;
+    if (wg_tid() < 3) {
+        const int l = wg_tid();
         const PList pl = l == 0 ? a.run : (l == 1 ? a.repl : a.xt);
         const uint32_t n = acc[l][0];
         pl.brick[n] = acc[l][1];
@@ -223,8 +230,9 @@ __device__ inline void build_lists(const PArgs<T>& a, uint32_t* tmp) {
 // A block of work items finished; the last of `total` builds the lists.
 template <typename T>
 __device__ inline void finish_item(const PArgs<T>& a, int total, int* slast, uint32_t* tmp) {
+    own_stores_done();
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wg_tid() == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const uint32_t t = __hip_atomic_fetch_add(&a.ctl->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *slast = (int)t == total - 1;
@@ -241,7 +249,7 @@ __device__ inline void finish_item(const PArgs<T>& a, int total, int* slast, uin
 __device__ inline double block_fold(const double* p, int64_t n, int stride, double* ws) {
 #pragma clang fp contract(off)
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) s += p[stride * i];
+    for (int64_t i = wg_tid(); i < n; i += 256) s += p[stride * i];
     return block_sum_256(s, ws);
 }
 
@@ -269,31 +277,56 @@ __device__ inline BatchTab tab_flagged(const PArgs<T>& a) {
 __device__ __forceinline__ uint32_t xcc_id() {
     return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;
 }
-// The workgroup's place among the workgroups of its XCD (set once per launch).
+// The workgroup's XCD for the brick phases (set once per launch): one of nx
+// groups (8, one per XCD; 1 when some XCD has no workgroup).
 struct XcdSlot {
-    uint32_t x, rank, count;
+    uint32_t x, nx;
 };
 
 // The 16^3 bricks of a list, XCD-aware: chunks of 64 consecutive positions go
-// to XCD chunk % 8, and the workgroups of that XCD (whatever their launch
-// order) take its positions in turn, so neighbouring bricks are processed
-// together under one L2.  f(brick) is called by the whole workgroup with a
-// uniform brick id.
+// to group chunk % nx, and the workgroups of that group take its positions one
+// at a time from the group's counter (*take), in the order they finish their
+// previous brick, as the hardware hands out the workgroups of a launch.  The
+// bricks in progress thus stay a narrow window of consecutive positions under
+// one L2, whose halos the neighbouring bricks read again; a fixed share per
+// workgroup lets the workgroups drift apart over hundreds of bricks.  The next
+// position is fetched while the current brick runs.  f(brick) is called by
+// the whole workgroup with a uniform brick id.  The counters start at zero.
 template <typename F>
-__device__ inline void for_bricks(const PList& L, int n, const XcdSlot& xs, F f) {
+__device__ inline void for_bricks(const PList& L, int n, const XcdSlot& xs, uint32_t* take, uint32_t* done, F f) {
     if (n <= 0) return;
     const uint32_t total = L.brick[n];
     constexpr uint32_t CH = 64;
+    __shared__ uint32_t snext[2];
+    uint32_t* const ctr = take + 32u * xs.x;
+    if (wg_tid() == 0) snext[0] = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int j = 0;
-    for (uint32_t p = xs.rank;; p += xs.count) {
-        const uint32_t chunk = xs.x + 8u * (p / CH);
-        const uint64_t g = (uint64_t)chunk * CH + p % CH;
-        if ((uint64_t)chunk * CH >= total) break;
-        if (g >= total) continue;
-        j = list_advance(L.brick, n, (uint32_t)g, j);
-        const uint32_t bid = L.gbrick[j] + ((uint32_t)g - L.brick[j]);
-        __syncthreads();  // the previous brick's LDS reads are over
-        f(bid);
+    for (uint32_t i = 0;; ++i) {
+        __syncthreads();  // snext[i & 1] is written; the previous brick's LDS reads are over
+        const uint32_t p = snext[i & 1];
+        const uint64_t chunk = xs.x + (uint64_t)xs.nx * (p / CH);
+        if (chunk * CH >= total) break;
+        if (wg_tid() == 0) snext[(i + 1) & 1] = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t g = chunk * CH + p % CH;
+        if (g < total) {
+            j = list_advance(L.brick, n, (uint32_t)g, j);
+            const uint32_t bid = L.gbrick[j] + ((uint32_t)g - L.brick[j]);
+            f(bid);
+            if (wg_tid() == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+// After a brick phase of round k (by one thread): every brick of the list
+// processed exactly once, else err 4; the count is cleared for round k + 2.
+__device__ inline void check_done(PCtl* c, int ph, int64_t k, uint32_t want) {
+    uint32_t* d = &c->done[ph][k & 1][0];
+    const uint32_t got = __hip_atomic_load(d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (got != want) {
+        c->bad_round = k * 2 + ph;
+        c->bad_got = got;
+        c->bad_want = want;
+        st_ctl(&c->err, 4);
     }
 }
 
@@ -399,7 +432,7 @@ __device__ inline void decide_fp64(const PArgs<double>& a, int k, double rt) {
 
 // block 0 charges the time since the last barrier to phase kind i
 __device__ __forceinline__ void tick(PCtl* c, int i, uint64_t& last) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && wg_tid() == 0) {
         const uint64_t now = wall_clock64();
         c->ticks[i] += now - last;
         last = now;
@@ -421,31 +454,24 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
     uint32_t gen = 0;
     const uint64_t t0 = wall_clock64();
     uint64_t tl = t0;
-    // the workgroup's XCD and its rank there; every XCD takes one eighth of the chunks
-    __shared__ uint32_t sx[2];
-    if (threadIdx.x == 0) {
+    // the workgroup's XCD; every XCD takes one eighth of the chunks
+    __shared__ uint32_t sx;
+    if (wg_tid() == 0) {
         uint32_t x = xcc_id();
         if (x >= 8) x = w & 7u;
-        sx[0] = x;
-        sx[1] = __hip_atomic_fetch_add(&c->xcd_count[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sx = x;
+        __hip_atomic_fetch_add(&c->xcd_count[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (x == (w & 7u)) __hip_atomic_fetch_add(&c->xcd_rr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (w == 0) build_lists<T>(a, tmp);
     if (!gsync(c, gen, &sflag[0])) return;
     tick(c, 3, tl);
-    XcdSlot xs;
-    xs.x = sx[0];
-    xs.rank = sx[1];
-    xs.count = __hip_atomic_load(&c->xcd_count[xs.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool all8 = true;  // an XCD without workgroups would leave its chunks undone: use the launch order then
+    XcdSlot xs{sx, 8u};
+    bool all8 = true;  // an XCD without workgroups would leave its chunks undone
     for (int i = 0; i < 8; ++i) all8 = all8 && __hip_atomic_load(&c->xcd_count[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0;
-    const bool rr = __hip_atomic_load(&c->xcd_rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == G;
-    if (((rr || !all8) && G >= 8 && !(G & 7u))) {
-        // workgroups reach the XCDs round-robin (as measured): the launch order keeps
-        // the workgroups of a chunk in step
-        xs.x = w & 7u;
-        xs.rank = w >> 3;
-        xs.count = G >> 3;
+    if (!all8) {
+        if (G >= 8 && !(G & 7u)) xs.x = w & 7u;  // by launch order, eight equal groups
+        else xs = XcdSlot{0u, 1u};
     }
     const BatchTab trun = tab_running(a), trep = tab_flagged(a);
     for (int64_t k = 0;; ++k) {
@@ -454,10 +480,14 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
         const int nrepl = ld_ctl(&c->nrepl);
         if (nrun == 0 && nrepl == 0) break;
         if (k >= a.max_rounds) {
-            if (w == 0 && threadIdx.x == 0) st_ctl(&c->err, 2);
+            if (w == 0 && wg_tid() == 0) st_ctl(&c->err, 2);
             break;
         }
         T* const rk = a.r + (k & 1) * N;
+        // the brick counters of round k + 1 (last used in round k - 1)
+        if (w == 0 && wg_tid() < 16)
+            __hip_atomic_store(&c->take[wg_tid() >> 3][(k + 1) & 1][wg_tid() & 7][0], 0u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         if (nrepl > 0) {
             // -- pending x terms of the instances to replace
             {
@@ -475,7 +505,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             if (a.has_scr) {
                 const WgRange rg = wg_range(a.repl.pts[nrepl], 256);
                 int j = rg.v0 < rg.v1 ? list_find(a.repl.pts, nrepl, rg.v0) : 0;
-                for (uint32_t v = rg.v0 + threadIdx.x; v < rg.v1; v += 256) {
+                for (uint32_t v = rg.v0 + wg_tid(); v < rg.v1; v += 256) {
                     j = list_advance(a.repl.pts, nrepl, v, j);
                     const int kk = a.repl.inst[j];
                     const int64_t ip = (int64_t)a.pt_off[kk] + (v - a.repl.pts[j]);
@@ -491,10 +521,9 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             tick(c, 0, tl);
             // -- the true residual into slot k & 1 (partial per brick)
             if constexpr (MIXED) {
-                for_bricks(a.repl, nrepl, xs, [&](uint32_t bid) {
+                for_bricks(a.repl, nrepl, xs, &c->take[0][k & 1][0][0], &c->done[0][k & 1][0], [&](uint32_t bid) {
                     const int kk = a.tab.brick_inst[bid];
-                    int tid = threadIdx.x;
-                    asm volatile("" : "+v"(tid));
+                    const int tid = wg_tid();
                     double s = 0.0;
                     for (int u = 0; u < 16; ++u) s += resid_upd_node(a, kk, (uint64_t)bid * 4096 + u * 256 + tid, rk);
                     const double t = block_sum_256(s, ws);
@@ -507,20 +536,22 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                 io.y = rk;
                 io.s = (const T*)a.sbuf64;
                 io.partial = a.partial;
-                for_bricks(a.repl, nrepl, xs, [&](uint32_t bid) {
+                for_bricks(a.repl, nrepl, xs, &c->take[0][k & 1][0][0], &c->done[0][k & 1][0], [&](uint32_t bid) {
                     stencil_tile_brick<T, 16, OP_RESID, true, true>(a.L, io, 0, trep, bid,
                                                                     *reinterpret_cast<TileLds<T, 16, OP_RESID>*>(&lds));
                 });
             }
             if (!gsync(c, gen, &sflag[0])) return;
             tick(c, 0, tl);
+            // (the replacement list is rebuilt after workgroup 0's first decision, which follows)
+            if (w == 0 && wg_tid() == 0) check_done(c, 0, k, a.repl.brick[nrepl]);
             if constexpr (MIXED) {
                 // -- per instance |b - A x2|^2; x = x2, e = 0; the new FP32 |r|^2 per 1024-node block
                 for (int j = (int)w; j < nrepl; j += (int)G) {
                     const int kk = a.repl.inst[j];
                     const int64_t p0 = (int64_t)(a.tab.off[kk] >> 12), np = (int64_t)(a.tab.off[kk + 1] >> 12) - p0;
                     const double t = block_fold(a.partial + p0, np, 1, ws);
-                    if (threadIdx.x == 0) a.tab.st[kk].slot[1] = t;
+                    if (wg_tid() == 0) a.tab.st[kk].slot[1] = t;
                     __syncthreads();
                 }
                 const WgRange rk1 = wg_range(a.repl.kblk[nrepl], 1);
@@ -539,7 +570,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                     const int kk = a.repl.inst[j];
                     const int64_t p0 = (int64_t)(a.tab.off[kk] >> 10), np = (int64_t)(a.tab.off[kk + 1] >> 10) - p0;
                     const double dt = block_fold(a.partial2 + p0, np, 1, ws);
-                    if (threadIdx.x == 0) {
+                    if (wg_tid() == 0) {
                         a.tab.st[kk].slot[2] = dt;
                         decide_mixed(a, kk, a.tab.st[kk].slot[1], dt);
                         a.flag[kk] = 0;
@@ -551,7 +582,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                     const int kk = a.repl.inst[j];
                     const int64_t p0 = (int64_t)(a.tab.off[kk] >> 12), np = (int64_t)(a.tab.off[kk + 1] >> 12) - p0;
                     const double rt = block_fold(a.partial + p0, np, 1, ws);
-                    if (threadIdx.x == 0) {
+                    if (wg_tid() == 0) {
                         a.tab.st[kk].slot[1] = rt;
                         decide_fp64(a, kk, rt);
                         a.flag[kk] = 0;
@@ -581,7 +612,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                 if (a.has_scr) {
                     const WgRange rg = wg_range(a.run.pts[nrun], 256);
                     int j = rg.v0 < rg.v1 ? list_find(a.run.pts, nrun, rg.v0) : 0;
-                    for (uint32_t v = rg.v0 + threadIdx.x; v < rg.v1; v += 256) {
+                    for (uint32_t v = rg.v0 + wg_tid(); v < rg.v1; v += 256) {
                         j = list_advance(a.run.pts, nrun, v, j);
                         const int kk = a.run.inst[j];
                         const int64_t ip = (int64_t)a.pt_off[kk] + (v - a.run.pts[j]);
@@ -603,13 +634,15 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                 io.y = a.q + ((k + 1) & 1) * N;
                 io.partial = a.partial;
                 io.s = a.sbuf;
-                for_bricks(a.run, nrun, xs,
+                for_bricks(a.run, nrun, xs, &c->take[1][k & 1][0][0], &c->done[1][k & 1][0],
                            [&](uint32_t bid) { stencil_tile_brick<T, 16, OP_CG, true, true>(a.L, io, 0, trun, bid, lds); });
             }
-            if (threadIdx.x == 0)
+            if (wg_tid() == 0)
                 __hip_atomic_fetch_add(&c->pass_busy, wall_clock64() - tp0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (!gsync(c, gen, &sflag[0])) return;
             tick(c, 2, tl);
+            // (the run list is rebuilt after workgroup 0's first fold, which follows)
+            if (w == 0 && wg_tid() == 0) check_done(c, 1, k, a.run.brick[nrun]);
             // -- fold and scalar step per instance; the last one builds the lists
             for (int j = (int)w; j < nrun; j += (int)G) {
                 const int kk = a.run.inst[j];
@@ -619,7 +652,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                     v4[q] = block_fold(a.partial + 4 * p0 + q, np, 4, ws);
                     __syncthreads();
                 }
-                if (threadIdx.x == 0) {
+                if (wg_tid() == 0) {
                     CgState* st = a.tab.st + kk;
                     fin_cg(st, v4[0], v4[1], v4[2], v4[3], 0);
                     // the pending terms' directions sit in the ring slots of passes k - xn + 1 .. k
@@ -638,24 +671,23 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             if (!gsync(c, gen, &sflag[0])) return;
             tick(c, 3, tl);
         }
-        if (w == 0 && threadIdx.x == 0) {
+        if (w == 0 && wg_tid() == 0) {
             c->rounds = k + 1;
             if (wall_clock64() - t0 > a.deadline) st_ctl(&c->err, 3);  // the others leave at their next barrier
         }
     }
 }
 
-// FP32 (the mixed solve): held to 128 VGPRs, so that four workgroups share a
-// compute unit as in the one-pass-per-launch kernel; left to itself the
-// compiler keeps values live across the round loop and reaches 172 VGPRs (two
-// workgroups).  The 41 VGPRs it spills instead are spread over the phases, a
-// few scratch accesses per brick against the pass's hundreds of loads.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_persist32(const PArgs<float>* __restrict__ pa) {
+// FP32 (the mixed solve): three workgroups per compute unit, 168 VGPRs, no
+// spills.  Held to four (128 VGPRs, as the one-pass-per-launch kernel) the
+// kernel spills 57 VGPRs, some inside the tile pass, and the batched mixed
+// solve took 7.0 s against 6.8 s at three.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_persist32(const PArgs<float>* __restrict__ pa) {
     persist_main<float>(pa);
 }
 // FP64: its 46 KB tile allows three workgroups per compute unit at most; the
 // one-pass kernel of the same pass runs two (175 VGPRs), and this one is held
-// to two as well (256 VGPRs).
+// to two as well (237 VGPRs, no spills).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_persist64(const PArgs<double>* __restrict__ pa) {
     persist_main<double>(pa);
 }

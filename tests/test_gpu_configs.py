@@ -119,6 +119,27 @@ def test_batching_workload_256_superquadrics(gpu):
         # the mixed batched instance solved the same system to the same residual
         assert np.abs(mixed_x[d] - xs).max() <= 1e-4 * (xs.max() - xs.min()), (d, k)
         t.close()
+    # the same solves as one persistent launch per level: every instance takes
+    # the graph replays' FP64 iterations (same folds, same decisions); x
+    # differs only in the rounding of its deferred updates
+    graph_x = {d: B.tree(depths.index(d)).solution for d in (6, 7, 8)}
+    prep, pper = B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0,
+                         launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+    print(f"  cascadic FP64 persistent: converged {prep['converged']}/{prep['instances']} max_true_rel "
+          f"{prep['max_true_rel']:.3e} {prep['ms_total']:.1f} ms, {prep['persistent_rounds']} rounds", flush=True)
+    assert prep["converged"] == 256 and prep["max_true_rel"] <= 1e-6
+    assert [p["fine_iters"] for p in pper] == [p["fine_iters"] for p in per]
+    for d in (6, 7, 8):
+        xp, xg = B.tree(depths.index(d)).solution, graph_x[d]
+        assert np.abs(xp - xg).max() <= 1e-9 * np.abs(xg).max(), d
+    mprep, _ = B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, precision=pr.PRX_PREC_MIXED,
+                       launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+    print(f"  cascadic mixed persistent: converged {mprep['converged']}/{mprep['instances']} max_true_rel "
+          f"{mprep['max_true_rel']:.3e} {mprep['ms_total']:.1f} ms", flush=True)
+    assert mprep["converged"] == 256 and mprep["max_true_rel"] <= 1e-6
+    for d in (6, 7, 8):
+        xm, xg = B.tree(depths.index(d)).solution, graph_x[d]
+        assert np.abs(xm - xg).max() <= 1e-4 * (xg.max() - xg.min()), d
     B.close()
 
 
