@@ -684,8 +684,9 @@ static void run_persistent(prx_batch* B, BLevel& L, const PArgs<T>& a, std::vect
     PRX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, B->device));
     for (int i = 0; i < 4; ++i) B->persistent_phase_ms[i] += khz > 0 ? (double)c.ticks[i] / (double)khz : 0.0;
     if (std::getenv("PRX_PERSIST_TRACE") && khz > 0)
-        std::fprintf(stderr, "persistent: %lld rounds, pass phase %.1f ms, mean busy per workgroup %.1f ms\n", (long long)c.rounds,
-                     (double)c.ticks[2] / khz, (double)c.pass_busy / khz / blocks);
+        std::fprintf(stderr, "persistent: %lld rounds (%lld with replacements), phase ms %.1f %.1f %.1f %.1f, mean pass busy per workgroup %.1f ms\n",
+                     (long long)c.rounds, (long long)c.repl_rounds, (double)c.ticks[0] / khz, (double)c.ticks[1] / khz,
+                     (double)c.ticks[2] / khz, (double)c.ticks[3] / khz, (double)c.pass_busy / khz / blocks);
     if (c.err == 1) throw Error(PRX_ERR_DEVICE, "persistent CG: a grid barrier wait exceeded its bound");
     if (c.err == 2) throw Error(PRX_ERR_DEVICE, "persistent CG: the round cap was reached");
     if (c.err == 3) throw Error(PRX_ERR_DEVICE, "persistent CG: the launch ran past its 60 s deadline");

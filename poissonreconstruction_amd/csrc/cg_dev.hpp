@@ -343,6 +343,45 @@ __device__ inline void bx_terms_block(const BatchTab& tab, T* __restrict__ x, co
     st_nt(x + g, xo);
 }
 
+// The due x terms (CG_XM of them) of NB blocks at once, for a loop that owns
+// few waves per compute unit: every block's x and direction loads are issued
+// before any sum, so NB times as many bytes are in flight per wave.  Block b
+// (b < nb) is blk[b], of instance inst[b], which runs with xn == CG_XM; the
+// directions are the same ring slots for every instance (dd).  Each block's
+// result equals bx_terms_block<T, 0>'s bit for bit.
+template <typename T, int NB>
+__device__ inline void bx_terms_due(const CgState* __restrict__ st, T* __restrict__ x, const XDirs<T>& dd,
+                                    const uint32_t (&blk)[NB], const uint32_t (&inst)[NB], int nb) {
+#pragma clang fp contract(off)
+    using NV = typename NV16<T>::type;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    NV xv[NB], dv[NB][CG_XM];
+    int64_t g[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        g[b] = ((int64_t)blk[b] * 256 + wg_tid()) * VEC;
+        if (b < nb) {
+            xv[b] = ld_nt(x + g[b]);
+#pragma unroll
+            for (int j = 0; j < CG_XM; ++j) dv[b][j] = ld_nt(dd.d[j] + g[b]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (b >= nb) break;
+        const CgState* sk = st + inst[b];
+        NV xo;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+            double v = (double)xv[b][e];
+#pragma unroll
+            for (int j = 0; j < CG_XM; ++j) v += sk->xa[j] * (double)dv[b][j][e];
+            xo[e] = (T)v;
+        }
+        st_nt(x + g[b], xo);
+    }
+}
+
 // x = x2 and e = 0 on the nodes of the running instances: the fold of a
 // batched residual-update pass, which wrote x + e into x2 (a block covers
 // 1024 nodes inside one 4096-node brick, so it belongs to one instance).
@@ -422,6 +461,60 @@ __device__ inline void bscreen_gather_point(const BatchTab& tab, int64_t P, cons
     for (int q = 0; q < 8; ++q) acc += w[q] * v[q];
     const T wh3 = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;
     s[i] = wh3 * acc;
+}
+
+// bscreen_gather_point<T, 3> for NP points at once whose instances are known
+// to run (kk[n] of point ip[n], n < np): all points' loads are issued before
+// any sum, for a loop that owns few waves per compute unit.  Each s[ip] equals
+// bscreen_gather_point<T, 3>'s bit for bit.
+template <typename T, int NP>
+__device__ inline void bscreen_gather_run(const BatchTab& tab, int64_t P, const uint32_t* __restrict__ cell,
+                                          const T* __restrict__ tt, T* s, const T* __restrict__ r, const T* __restrict__ dd,
+                                          const T* __restrict__ qq, const int64_t (&ip)[NP], const uint32_t (&kk)[NP], int np) {
+#pragma clang fp contract(off)
+    uint64_t m[NP][8];
+    T w[NP][8], v[NP][8], wh3[NP];
+    double bt[NP], al[NP];
+    int rp[NP];
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        if (n >= np) break;
+        const uint32_t k = kk[n];
+        const int64_t i = ip[n];
+        const BCoef cf = tab.coef[k];
+        const uint64_t ib = tab.off[k];
+        bt[n] = tab.st[k].beta;
+        al[n] = tab.st[k].alpha;
+        rp[n] = tab.st[k].rpend;
+        wh3[n] = sizeof(T) == 8 ? (T)cf.wh3_64 : (T)cf.wh3_32;
+        const uint32_t res = cf.res;
+        const uint32_t c[3] = {cell[i], cell[P + i], cell[2 * P + i]};
+        const T t[3] = {tt[i], tt[P + i], tt[2 * P + i]};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t ox = c[0] + (q & 1), oy = c[1] + ((q >> 1) & 1), oz = c[2] + (q >> 2);
+            const bool in = ox < res && oy < res && oz < res;
+            const T bx = (q & 1) ? t[0] : T(1) - t[0];
+            const T by = ((q >> 1) & 1) ? t[1] : T(1) - t[1];
+            const T bz = (q >> 2) ? t[2] : T(1) - t[2];
+            w[n][q] = in ? bx * by * bz : T(0);
+            m[n][q] = ib + (in ? morton3(ox, oy, oz) : 0u);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        if (n >= np) break;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[n][q] = cg_form(bt[n], al[n], rp[n], dd[m[n][q]], r[m[n][q]], qq[m[n][q]]);
+    }
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        if (n >= np) break;
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += w[n][q] * v[n][q];
+        s[ip[n]] = wh3[n] * acc;
+    }
 }
 
 }  // namespace prx

@@ -338,6 +338,7 @@ struct PCtl {
     uint32_t ticket;             // completed fold-phase instances (the last one builds the lists)
     int32_t pad;
     int64_t rounds;
+    int64_t repl_rounds;         // rounds with a replacement (confirmation) phase
     uint64_t ticks[4];           // wall-clock ticks per phase kind (replacement, gather, pass, fold), block 0's view
     uint32_t xcd_count[8];       // workgroups per XCD (as its XCC_ID register reads)
     uint32_t xcd_rr;             // workgroups w found on XCD w % 8
@@ -374,13 +375,8 @@ struct PArgs {
     uint64_t deadline = 0;       // wall-clock ticks the launch may run (set by the launcher)
     double theta = 0.0;
     LevelArgs<T> L{};            // screening lists for the tile pass (T weights)
-    // the level's screening list (FP64 weights; the mixed residual)
+    LevelArgs<double> L64{};     // the same with FP64 weights (the mixed solve's FP64 residual pass)
     int has_scr = 0;
-    const uint32_t* s_brick = nullptr;
-    const uint32_t* s_node = nullptr;
-    const uint32_t* s_off = nullptr;
-    const uint32_t* s_sp = nullptr;
-    const double* s_phi = nullptr;
     // sorted points, instance by instance (pt_off[K+1])
     int64_t P = 0;
     const uint32_t* cell = nullptr;

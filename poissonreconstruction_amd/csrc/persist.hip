@@ -120,23 +120,6 @@ __device__ __forceinline__ int list_advance(const uint32_t* pref, int n, uint32_
     return j;
 }
 
-// A contiguous range of the work items [0, tot) per workgroup, in chunks
-// of 256 (one item per thread per chunk), so that the list entry found by one
-// search at the range start only moves forward a little.
-struct WgRange {
-    uint32_t v0, v1;
-};
-__device__ __forceinline__ WgRange wg_range(uint32_t tot, uint32_t align) {
-    const uint32_t G = gridDim.x;
-    uint32_t per = (tot + G - 1) / G;
-    per = (per + align - 1) / align * align;
-    const uint64_t a = (uint64_t)blockIdx.x * per;
-    WgRange r;
-    r.v0 = (uint32_t)(a < tot ? a : tot);
-    r.v1 = (uint32_t)(a + per < tot ? a + per : tot);
-    return r;
-}
-
 // Exclusive block scan of v (256 threads); *total gets the sum.
 __device__ inline uint32_t block_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
     const int lane = wg_tid() & 63, wave = wg_tid() >> 6;
@@ -330,53 +313,6 @@ __device__ inline void check_done(PCtl* c, int ph, int64_t k, uint32_t want) {
     }
 }
 
-// mixed solve: per node x2 = x + e and the FP64 residual b - A x2 (the
-// operator of the single-tree FP64 pass: the instance's weights, the
-// truncated Laplacian rows, wg on the diagonal and the per-point screening
-// blocks through sbuf64); r32 = (b - A x2) * res
-__device__ inline double resid_upd_node(const PArgs<float>& a, int k, uint64_t i, float* r32) {
-#pragma clang fp contract(off)
-    const BCoef cf = a.tab.coef[k];
-    const uint64_t ib = a.tab.off[k];
-    const uint32_t res = cf.res;
-    uint32_t ix, iy, iz;
-    demorton3((uint32_t)(i - ib), ix, iy, iz);
-    auto X = [&](int jx, int jy, int jz) -> double {
-        if ((unsigned)jx >= res || (unsigned)jy >= res || (unsigned)jz >= res) return 0.0;
-        const uint64_t j = ib + morton3((uint32_t)jx, (uint32_t)jy, (uint32_t)jz);
-        return a.x64[j] + (double)a.x[j];
-    };
-    const double xc = X((int)ix, (int)iy, (int)iz);
-    double se = 0.0, sc = 0.0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int nz = (dx != 0) + (dy != 0) + (dz != 0);
-                if (nz < 2) continue;
-                const double d = X((int)ix + dx, (int)iy + dy, (int)iz + dz) - xc;
-                if (nz == 2) se += d;
-                else sc += d;
-            }
-    double ax = (-2.0 * cf.s64) * se + (-cf.s64) * sc + cf.wg64 * xc;
-    if (a.has_scr) {
-        int64_t lo = a.s_brick[i >> 12], hi = a.s_brick[(i >> 12) + 1];
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if ((uint64_t)a.s_node[mid] < i) lo = mid + 1;
-            else hi = mid;
-        }
-        if (lo < (int64_t)a.s_brick[(i >> 12) + 1] && a.s_node[lo] == (uint32_t)i) {
-            double s = 0.0;
-            for (uint32_t j = a.s_off[lo]; j < a.s_off[lo + 1]; ++j) s += a.s_phi[j] * a.sbuf64[a.s_sp[j]];
-            ax += s;
-        }
-    }
-    const double r = a.b[i] - ax;
-    a.x2[i] = xc;
-    r32[i] = (float)(r * (double)res);
-    return r * r;
-}
-
 // The decision after a replacement (mixed): as the host loop of the per-pass
 // batched mixed solve.  rt = |b - A x|^2, dt = the new FP32 |r|^2.
 __device__ inline void decide_mixed(const PArgs<float>& a, int k, double rt, double dt) {
@@ -444,7 +380,11 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
     const PArgs<T>& a = *pa;
     constexpr bool MIXED = sizeof(T) == 4;
     constexpr int VEC = 16 / (int)sizeof(T);
-    __shared__ TileLds<T, 16, OP_CG> lds;
+    // the tile of the CG pass, shared with the residual passes (FP64 in the mixed solve)
+    constexpr size_t LDS_CG = sizeof(TileLds<T, 16, OP_CG>);
+    constexpr size_t LDS_RES = MIXED ? sizeof(TileLds<double, 16, OP_RESID_UPD>) : sizeof(TileLds<T, 16, OP_RESID>);
+    __shared__ __align__(16) unsigned char lds_raw[LDS_CG > LDS_RES ? LDS_CG : LDS_RES];
+    TileLds<T, 16, OP_CG>& lds = *reinterpret_cast<TileLds<T, 16, OP_CG>*>(lds_raw);
     __shared__ double ws[16];
     __shared__ uint32_t tmp[8];
     __shared__ int sflag[2];
@@ -489,10 +429,11 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             __hip_atomic_store(&c->take[wg_tid() >> 3][(k + 1) & 1][wg_tid() & 7][0], 0u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (nrepl > 0) {
+            if (w == 0 && wg_tid() == 0) c->repl_rounds++;
             // -- pending x terms of the instances to replace
             {
-                const WgRange rv = wg_range(a.repl.vblk[nrepl], 1);
-                for (uint32_t v = rv.v0, j = rv.v0 < rv.v1 ? (uint32_t)list_find(a.repl.vblk, nrepl, rv.v0) : 0u; v < rv.v1; ++v) {
+                const uint32_t tv = a.repl.vblk[nrepl];
+                for (uint32_t v = w, j = w < tv ? (uint32_t)list_find(a.repl.vblk, nrepl, w) : 0u; v < tv; v += G) {
                     j = (uint32_t)list_advance(a.repl.vblk, nrepl, v, (int)j);
                     const int kk = a.repl.inst[j];
                     const uint32_t blk = (uint32_t)(a.tab.off[kk] / (256 * VEC)) + (v - a.repl.vblk[j]);
@@ -503,9 +444,9 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             tick(c, 0, tl);
             // -- per-point screening values of the solution (mixed: x + e)
             if (a.has_scr) {
-                const WgRange rg = wg_range(a.repl.pts[nrepl], 256);
-                int j = rg.v0 < rg.v1 ? list_find(a.repl.pts, nrepl, rg.v0) : 0;
-                for (uint32_t v = rg.v0 + wg_tid(); v < rg.v1; v += 256) {
+                const uint32_t tp = a.repl.pts[nrepl], v0 = w * 256u + (uint32_t)wg_tid();
+                int j = v0 < tp ? list_find(a.repl.pts, nrepl, v0) : 0;
+                for (uint32_t v = v0; v < tp; v += G * 256u) {
                     j = list_advance(a.repl.pts, nrepl, v, j);
                     const int kk = a.repl.inst[j];
                     const int64_t ip = (int64_t)a.pt_off[kk] + (v - a.repl.pts[j]);
@@ -521,13 +462,19 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
             tick(c, 0, tl);
             // -- the true residual into slot k & 1 (partial per brick)
             if constexpr (MIXED) {
+                // x2 = x + e, r = (b - A x2) / h into slot k & 1, |b - A x2|^2 per brick: the per-pass
+                // solve's FP64 residual update pass
+                StencilIO<double> io;
+                io.x64 = a.x64;
+                io.e32 = (const float*)a.x;
+                io.x_out = a.x2;
+                io.b = a.b;
+                io.r32 = (float*)rk;
+                io.s = a.sbuf64;
+                io.partial = a.partial;
                 for_bricks(a.repl, nrepl, xs, &c->take[0][k & 1][0][0], &c->done[0][k & 1][0], [&](uint32_t bid) {
-                    const int kk = a.tab.brick_inst[bid];
-                    const int tid = wg_tid();
-                    double s = 0.0;
-                    for (int u = 0; u < 16; ++u) s += resid_upd_node(a, kk, (uint64_t)bid * 4096 + u * 256 + tid, rk);
-                    const double t = block_sum_256(s, ws);
-                    if (tid == 0) a.partial[bid] = t;
+                    stencil_tile_brick<double, 16, OP_RESID_UPD, true, true>(
+                        a.L64, io, 0, trep, bid, *reinterpret_cast<TileLds<double, 16, OP_RESID_UPD>*>(lds_raw));
                 });
             } else {
                 StencilIO<T> io;
@@ -538,7 +485,7 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                 io.partial = a.partial;
                 for_bricks(a.repl, nrepl, xs, &c->take[0][k & 1][0][0], &c->done[0][k & 1][0], [&](uint32_t bid) {
                     stencil_tile_brick<T, 16, OP_RESID, true, true>(a.L, io, 0, trep, bid,
-                                                                    *reinterpret_cast<TileLds<T, 16, OP_RESID>*>(&lds));
+                                                                    *reinterpret_cast<TileLds<T, 16, OP_RESID>*>(lds_raw));
                 });
             }
             if (!gsync(c, gen, &sflag[0])) return;
@@ -554,8 +501,8 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                     if (wg_tid() == 0) a.tab.st[kk].slot[1] = t;
                     __syncthreads();
                 }
-                const WgRange rk1 = wg_range(a.repl.kblk[nrepl], 1);
-                for (uint32_t v = rk1.v0, j = rk1.v0 < rk1.v1 ? (uint32_t)list_find(a.repl.kblk, nrepl, rk1.v0) : 0u; v < rk1.v1; ++v) {
+                const uint32_t tk = a.repl.kblk[nrepl];
+                for (uint32_t v = w, j = w < tk ? (uint32_t)list_find(a.repl.kblk, nrepl, w) : 0u; v < tk; v += G) {
                     j = (uint32_t)list_advance(a.repl.kblk, nrepl, v, (int)j);
                     const int kk = a.repl.inst[j];
                     const uint32_t blk = (uint32_t)(a.tab.off[kk] >> 10) + (v - a.repl.kblk[j]);
@@ -602,21 +549,47 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
                 const int nxt = ld_ctl(&c->nxt);
                 XDirs<T> dirs;
                 for (int i = 0; i < CG_XM; ++i) dirs.d[i] = a.ring + ((k + 1 + i) % CG_XM) * N;
-                const WgRange rx = wg_range(nxt > 0 ? a.xt.vblk[nxt] : 0u, 1);
-                for (uint32_t v = rx.v0, j = rx.v0 < rx.v1 ? (uint32_t)list_find(a.xt.vblk, nxt, rx.v0) : 0u; v < rx.v1; ++v) {
-                    j = (uint32_t)list_advance(a.xt.vblk, nxt, v, (int)j);
-                    const int kk = a.xt.inst[j];
-                    const uint32_t blk = (uint32_t)(a.tab.off[kk] / (256 * VEC)) + (v - a.xt.vblk[j]);
-                    bx_terms_block<T, 0>(trun, a.x, dirs, a.ring, N, blk);
+                // XB consecutive blocks per workgroup and step, the steps interleaved over the workgroups
+                constexpr int XB = sizeof(T) == 8 ? 4 : 2;
+                const uint32_t tv = nxt > 0 ? a.xt.vblk[nxt] : 0u;
+                for (uint32_t v0 = w * XB, j = v0 < tv ? (uint32_t)list_find(a.xt.vblk, nxt, v0) : 0u; v0 < tv; v0 += G * XB) {
+                    uint32_t blk[XB], ins[XB];
+                    int nb = 0;
+#pragma unroll
+                    for (int b = 0; b < XB; ++b) {
+                        const uint32_t v = v0 + b;
+                        blk[b] = ins[b] = 0;
+                        if (v < tv) {
+                            j = (uint32_t)list_advance(a.xt.vblk, nxt, v, (int)j);
+                            ins[b] = (uint32_t)a.xt.inst[j];
+                            blk[b] = (uint32_t)(a.tab.off[ins[b]] / (256 * VEC)) + (v - a.xt.vblk[j]);
+                            nb = b + 1;
+                        }
+                    }
+                    bx_terms_due<T, XB>(a.tab.st, a.x, dirs, blk, ins, nb);
                 }
                 if (a.has_scr) {
-                    const WgRange rg = wg_range(a.run.pts[nrun], 256);
-                    int j = rg.v0 < rg.v1 ? list_find(a.run.pts, nrun, rg.v0) : 0;
-                    for (uint32_t v = rg.v0 + wg_tid(); v < rg.v1; v += 256) {
-                        j = list_advance(a.run.pts, nrun, v, j);
-                        const int kk = a.run.inst[j];
-                        const int64_t ip = (int64_t)a.pt_off[kk] + (v - a.run.pts[j]);
-                        bscreen_gather_point<T, 3>(trun, a.P, a.cell, a.tt, a.pinst, nullptr, nullptr, nullptr, a.sbuf, rk, dold, qk, ip);
+                    // SP points per thread and step (the steps interleaved over the workgroups)
+                    constexpr int SP = 2;
+                    const uint32_t tp = a.run.pts[nrun], v0 = w * (256u * SP) + (uint32_t)wg_tid();
+                    int j = v0 < tp ? list_find(a.run.pts, nrun, v0) : 0;
+                    for (uint32_t v = v0; v < tp; v += G * 256u * SP) {
+                        int64_t ip[SP];
+                        uint32_t ik[SP];
+                        int np = 0;
+#pragma unroll
+                        for (int n = 0; n < SP; ++n) {
+                            const uint32_t vn = v + 256u * n;
+                            ip[n] = 0;
+                            ik[n] = 0;
+                            if (vn < tp) {
+                                j = list_advance(a.run.pts, nrun, vn, j);
+                                ik[n] = (uint32_t)a.run.inst[j];
+                                ip[n] = (int64_t)a.pt_off[ik[n]] + (vn - a.run.pts[j]);
+                                np = n + 1;
+                            }
+                        }
+                        bscreen_gather_run<T, SP>(trun, a.P, a.cell, a.tt, a.sbuf, rk, dold, qk, ip, ik, np);
                     }
                 }
             }
@@ -678,10 +651,11 @@ __device__ __forceinline__ void persist_main(const PArgs<T>* __restrict__ pa) {
     }
 }
 
-// FP32 (the mixed solve): three workgroups per compute unit, 168 VGPRs, no
-// spills.  Held to four (128 VGPRs, as the one-pass-per-launch kernel) the
-// kernel spills 57 VGPRs, some inside the tile pass, and the batched mixed
-// solve took 7.0 s against 6.8 s at three.
+// FP32 (the mixed solve): three workgroups per compute unit (its LDS holds
+// the FP64 tile of the residual pass, 47 KB), 168 VGPRs with 42 spilled.  At
+// two per compute unit nothing spills (199 VGPRs), but the CG pass phase of
+// the batched mixed solve took 5.3 s against 4.3 s at three: the pass needs
+// the waves to hide memory latency more than it needs the registers.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_persist32(const PArgs<float>* __restrict__ pa) {
     persist_main<float>(pa);
 }
@@ -696,14 +670,8 @@ template <typename T>
 void launch_persistent_cg(const PArgs<T>& a0, const NodeList* scr, void* dargs, int* blocks, hipStream_t s) {
     PArgs<T> a = a0;
     a.L = batch_args<T>(scr, 12);
+    a.L64 = batch_args<double>(scr, 12);
     a.has_scr = (scr && scr->T > 0 && a.P > 0) ? 1 : 0;
-    if (a.has_scr) {
-        a.s_brick = scr->brick;
-        a.s_node = scr->node;
-        a.s_off = scr->off;
-        a.s_sp = scr->sp;
-        a.s_phi = scr->phi;
-    }
     int dev = 0, ncu = 0, per = 0;
     PRX_HIP(hipGetDevice(&dev));
     PRX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
