@@ -77,10 +77,14 @@ __device__ inline bool gsync(PCtl* c, uint32_t& gen, int* sbad) {
             // the group's writes happen before this one's next release
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // each counter's reset is performed before the barrier can open, so that no
+            // workgroup's count for the next barrier lands before it and is lost
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             const uint32_t t = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t == ng - 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 __hip_atomic_store(&c->bar_gen, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
