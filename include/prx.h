@@ -306,6 +306,15 @@ int prx_brick_copy_bandwidth(int64_t nbricks, int streams, int precision, int re
  * reference pushes in assemble_system (src/fem_tree.rs:199-227); they are
  * built once per point set and reused by every assembly and solve. */
 int prx_splat_times(prx_tree* t, double* ms);
+/* Measurement (no counterpart in the reference): the duration of every
+ * launch of the finest level's fused CG pass (the face-form pass in the FP32
+ * inner solves at depth >= 8, else the stored-q pass) in the solves that run
+ * between prx_pass_timing(t, 1, ...) and prx_pass_timing(t, 0, ms, cap, &n),
+ * timed with a hipEvent pair on the tree's stream around each launch.  The
+ * second call returns n durations (ms, launch order; the first min(n, cap)
+ * are written).  Passes enqueued after the solve converged within a chunk of
+ * iterations return at once and appear as durations of a few microseconds. */
+int prx_pass_timing(prx_tree* t, int on, float* ms, size_t cap, size_t* n);
 /* Sizes of the screening data of the last splat (measurement; no
  * counterpart in the reference): out[3] = {touched nodes, (point, corner)
  * entries, occupied trunc cells}. */

@@ -153,6 +153,7 @@ SIGNATURES = {
     "prx_time_stencil_op": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp]),
     "prx_copy_bandwidth": (_i32, [_sz, _i32, _i32, _vp]),
     "prx_splat_times": (_i32, [_vp, _vp]),
+    "prx_pass_timing": (_i32, [_vp, _i32, _vp, ctypes.c_size_t, _vp]),
     "prx_tree_release_workspace": (_i32, [_vp]),
     "prx_screening_sizes": (_i32, [_vp, _vp]),
     "prx_splat_normal_field_device": (_i32, [_vp, _vp, _vp, _sz]),

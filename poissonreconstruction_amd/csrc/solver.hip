@@ -118,6 +118,33 @@ static void gs_sweeps(prx_tree* t, const LevelView& L, T* x, const T* b, int swe
         }
 }
 
+// In-solve timing of the fused CG pass: the durations of the pairs recorded
+// since the last read (their launches have completed: the caller has waited
+// on the stream, or waits on each end event here).
+static void ptime_read(prx_tree* t) {
+    auto& P = t->ptime;
+    for (int i = 0; i < P.used; ++i) {
+        PRX_HIP(hipEventSynchronize(P.ev[2 * i + 1]));
+        float ms = 0.f;
+        PRX_HIP(hipEventElapsedTime(&ms, P.ev[2 * i], P.ev[2 * i + 1]));
+        P.ms.push_back(ms);
+    }
+    P.used = 0;
+}
+// the event pair for the next timed launch (64 pairs in flight at most)
+static hipEvent_t* ptime_pair(prx_tree* t) {
+    auto& P = t->ptime;
+    if (P.used == 64) ptime_read(t);
+    if ((int)P.ev.size() < 2 * (P.used + 1)) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e;
+            PRX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            P.ev.push_back(e);
+        }
+    }
+    return &P.ev[2 * P.used++];
+}
+
 void read_state(prx_tree* t) {
     PRX_HIP(hipMemcpyAsync(t->st_host, t->st, sizeof(CgState), hipMemcpyDeviceToHost, t->stream));
     PRX_HIP(hipStreamSynchronize(t->stream));
@@ -240,12 +267,15 @@ struct Cg {
         io.partial = t->partial.p;
         io.done = done;
         int nb = 0;
+        // the finest level's pass, timed on request
+        hipEvent_t* tev = (t->ptime.on && L.D == t->D) ? ptime_pair(t) : nullptr;
         if (faces) {
             if (L.scr) {
                 launch_cell_screening<T>(L, *cells, io.r, dold, qt, t->st, done, sig, sc, s);
                 io.sc = sc;
                 io.qt = qt;
             }
+            if (tev) PRX_HIP(hipEventRecord(tev[0], s));
             launch_cg_faces<T>(L, io, &nb, s);
         } else {
             if (L.scr) {
@@ -253,8 +283,10 @@ struct Cg {
                 launch_screen_gather<T>(L, nullptr, nullptr, nullptr, sb, s, io.r, dold, nullptr, io.q, t->st);
                 io.s = sb;
             }
+            if (tev) PRX_HIP(hipEventRecord(tev[0], s));
             launch_stencil<T>(L, OP_CG, io, &nb, t->tile, s);
         }
+        if (tev) PRX_HIP(hipEventRecord(tev[1], s));
         applies++;
         const bool fresh = refresh > 0 && (k + 1) % (uint64_t)refresh == 0;
         launch_fin(t->st, t->partial.p, nb, FIN_CG, fresh ? 1 : 0, s);
@@ -312,6 +344,7 @@ struct Cg {
         for (;;) {
             for (int c = 0; c < chunk && k < cap; ++c) enqueue(k++);
             read_state(t);
+            if (t->ptime.used) ptime_read(t);
             if (t->st_host->done || k >= cap) break;
         }
         it = (uint64_t)t->st_host->it;
@@ -1506,6 +1539,26 @@ int prx_screening_sizes(prx_tree* t, int64_t* out) {
     out[0] = t->trunc_list.T;
     out[1] = t->trunc_list.M;
     out[2] = t->cells_ok ? t->cells.C : 0;
+    PRX_API_END
+}
+
+int prx_pass_timing(prx_tree* t, int on, float* ms, size_t cap, size_t* n) {
+    PRX_API_BEGIN
+    PRX_REQUIRE(t, PRX_ERR_INVALID, "null tree");
+    set_device(t);
+    auto& P = t->ptime;
+    if (on) {
+        P.on = true;
+        P.used = 0;
+        P.ms.clear();
+    } else {
+        PRX_REQUIRE(n, PRX_ERR_INVALID, "null count");
+        prx::ptime_read(t);
+        P.on = false;
+        *n = P.ms.size();
+        if (ms) for (size_t i = 0; i < P.ms.size() && i < cap; ++i) ms[i] = P.ms[i];
+        P.ms.clear();
+    }
     PRX_API_END
 }
 

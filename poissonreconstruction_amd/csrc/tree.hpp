@@ -144,8 +144,19 @@ struct prx_tree {
     double setup_ms[5] = {0, 0, 0, 0, 0};
     int cg_form = PRX_CG_FUSED;  // CG iteration form of the last solve (prx_solve_opts.cg_form)
 
+    // in-solve timing of the finest level's fused CG pass (prx_pass_timing):
+    // a hipEvent pair around each launch, read back after each chunk of
+    // iterations (the chunk ends with a host wait anyway)
+    struct PassTimer {
+        bool on = false;
+        std::vector<hipEvent_t> ev;  // 2 per launch in flight
+        int used = 0;                // pairs recorded and not yet read
+        std::vector<float> ms;       // durations read so far
+    } ptime;
+
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
+        for (hipEvent_t e : ptime.ev) (void)hipEventDestroy(e);
         free_list(floor_list);
         free_list(trunc_list);
         free_list(eval_list);

@@ -474,3 +474,40 @@ def test_release_workspace_then_solve_again(gpu):
         assert rep["status"] == 0
         sols.append(t.solution)
     assert np.array_equal(sols[0].view(np.uint64), sols[1].view(np.uint64))
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_pass_timing_counts_every_fine_pass(gpu, prec):
+    # prx_pass_timing records one duration per launch of the finest level's fused CG pass (the face form in the
+    # FP32 inner solves at depth 8): every iteration's pass, plus the few enqueued after an inner solve converged
+    # within a chunk, which return at once; nothing is recorded once timing is off
+    import ctypes
+    D = 8
+    pts = unit_points("sphere", 30000, seed=15)
+    t = make_tree(D, pts, 4.0)
+
+    def timed_solve(on):
+        if on:
+            L.check(L.lib().prx_pass_timing(t._h, 1, None, 0, None))
+        rep = t.solve_cascadic(pts, 4.0, D - 2, 20, 800000, 1e-6, precision=prec,
+                               residual_mode=pr.PRX_RESID_TRUE, refresh_period=0)
+        assert rep["status"] == 0
+        n = ctypes.c_size_t(0)
+        buf = np.zeros(1 << 14, dtype=np.float32)
+        if on:
+            L.check(L.lib().prx_pass_timing(t._h, 0, buf.ctypes.data, buf.size, ctypes.byref(n)))
+        return rep, buf[:n.value]
+
+    rep, ms = timed_solve(True)
+    worked = ms[ms >= 0.1 * np.median(ms)]
+    assert len(worked) == rep["fine_iters"], (len(ms), len(worked), rep["fine_iters"])
+    assert len(ms) - len(worked) <= 8 * max(1, rep["outer_iters"])
+    assert np.all(ms > 0) and np.all(np.isfinite(ms))
+    # timing off: the next solve records nothing and the counter reads zero
+    L.check(L.lib().prx_pass_timing(t._h, 1, None, 0, None))
+    n = ctypes.c_size_t(7)
+    L.check(L.lib().prx_pass_timing(t._h, 0, None, 0, ctypes.byref(n)))
+    assert n.value == 0
+    rep2, ms2 = timed_solve(False)
+    assert len(ms2) == 0 and rep2["fine_iters"] == rep["fine_iters"]
+    t.close()
