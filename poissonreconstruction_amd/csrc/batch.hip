@@ -689,7 +689,7 @@ static void run_persistent(prx_batch* B, BLevel& L, const PArgs<T>& a, std::vect
                      (double)c.ticks[2] / khz, (double)c.ticks[3] / khz, (double)c.pass_busy / khz / blocks);
     if (c.err == 1) throw Error(PRX_ERR_DEVICE, "persistent CG: a grid barrier wait exceeded its bound");
     if (c.err == 2) throw Error(PRX_ERR_DEVICE, "persistent CG: the round cap was reached");
-    if (c.err == 3) throw Error(PRX_ERR_DEVICE, "persistent CG: the launch ran past its 60 s deadline");
+    if (c.err == 3) throw Error(PRX_ERR_DEVICE, "persistent CG: the launch ran past its deadline (60 s unless PRX_PERSIST_DEADLINE_MS sets another)");
     if (c.err == 4)
         throw Error(PRX_ERR_DEVICE, "persistent CG: a brick phase processed " + std::to_string(c.bad_got) + " bricks of " +
                                         std::to_string(c.bad_want) + " (round " + std::to_string(c.bad_round / 2) +

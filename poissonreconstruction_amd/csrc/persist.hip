@@ -28,6 +28,8 @@
 // A grid barrier separates the phases.  Its wait is bounded: a wait that
 // exceeds about four seconds sets an error flag, and every workgroup leaves
 // at its next barrier, so the launch always ends.
+#include <cstdlib>
+
 #include "tile.hpp"
 #include "cg_dev.hpp"
 
@@ -685,7 +687,10 @@ void launch_persistent_cg(const PArgs<T>& a0, const NodeList* scr, void* dargs, 
     const int grid = per * ncu;
     int khz = 0;
     PRX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    a.deadline = (uint64_t)(khz > 0 ? khz : 100000) * 1000ull * 60ull;  // 60 s
+    // 60 s; PRX_PERSIST_DEADLINE_MS sets another (tests force the deadline exit with it)
+    uint64_t deadline_ms = 60000;
+    if (const char* e = std::getenv("PRX_PERSIST_DEADLINE_MS")) deadline_ms = std::strtoull(e, nullptr, 10);
+    a.deadline = (uint64_t)(khz > 0 ? khz : 100000) * deadline_ms;
     if (grid % 8) throw Error(PRX_ERR_DEVICE, "persistent CG: workgroup count not a multiple of the XCD count");
     PRX_HIP(hipMemcpyAsync(dargs, &a, sizeof(PArgs<T>), hipMemcpyHostToDevice, s));
     const PArgs<T>* pa = (const PArgs<T>*)dargs;

@@ -207,3 +207,20 @@ def test_batch_persistent_launch_rejects_what_it_does_not_run(gpu):
     with pytest.raises(pr.PrxError):  # the reference's stopping test with periodic refresh stays on graph replays
         B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_REFERENCE, refresh_period=10, autotune=False,
                 launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+
+
+def test_batch_persistent_launch_ends_at_its_deadline(gpu, monkeypatch):
+    # a launch past its deadline sets the error word after its round; every workgroup leaves at its next
+    # barrier, the host raises, and the batch solves again afterwards
+    B = Batch()
+    for D, pts, pw in clouds()[:3]:
+        B.add(D, pts, pw)
+    monkeypatch.setenv("PRX_PERSIST_DEADLINE_MS", "0")
+    with pytest.raises(pr.PrxError, match="deadline"):
+        B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False,
+                launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+    monkeypatch.delenv("PRX_PERSIST_DEADLINE_MS")
+    rep, _ = B.solve(eps=1e-6, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False,
+                     launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+    assert rep["converged"] == rep["instances"] and rep["max_true_rel"] <= 1e-6
+    B.close()
