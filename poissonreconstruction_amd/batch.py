@@ -71,9 +71,10 @@ class Batch:
         multigrid-preconditioned CG (mixed or FP64) instead; cg_two_pass
         selects the two-pass CG iteration instead of the fused one (FP64).
         launch: PRX_BATCH_LAUNCH_PERSISTENT runs each level's CG of all
-        instances in one persistent launch, PRX_BATCH_LAUNCH_GRAPHS replays
-        captured chunks of iterations, AUTO picks the persistent launch where
-        it applies (true-residual stopping, fused CG step)."""
+        instances in one persistent launch (true-residual stopping, the fused
+        CG step with q stored; anything else raises), PRX_BATCH_LAUNCH_GRAPHS
+        replays captured chunks of iterations, AUTO picks the graph replays
+        (the faster of the two on many-cloud batches)."""
         o = L.default_solve_opts()
         o.precision = int(precision)
         o.residual_mode = int(residual_mode)
