@@ -262,10 +262,12 @@ def main():
                                            "builds inside assemble_system (src/fem_tree.rs:199-227)"),
         "ms_per_solve_with_screening_lists": {"mixed": round(ms_mixed + setup["screening_lists"], 2),
                                               "fp64": round(ms_fp64 + setup["screening_lists"], 2) if ms_fp64 else None},
-        "mixed": {k2: rep_mixed[k2] for k2 in ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse",
-                                             "ms_fine_gs", "ms_fine_cg", "ms_total", "applies")},
-        "fp64": ({k2: fp64[-1][1][k2] for k2 in ("fine_iters", "coarse_iters", "true_rel", "ms_coarse", "ms_fine_gs",
-                                                 "ms_fine_cg", "ms_total", "applies")} if fp64 else None),
+        "mixed": dict({k2: rep_mixed[k2] for k2 in ("fine_iters", "outer_iters", "coarse_iters", "true_rel", "ms_coarse",
+                                                  "ms_fine_gs", "ms_fine_cg", "ms_total", "applies")},
+                      ref_style_rel=ref_style(rep_mixed)),
+        "fp64": (dict({k2: fp64[-1][1][k2] for k2 in ("fine_iters", "coarse_iters", "true_rel", "ms_coarse", "ms_fine_gs",
+                                                      "ms_fine_cg", "ms_total", "applies")},
+                      ref_style_rel=ref_style(fp64[-1][1])) if fp64 else None),
         "field_error": field or None,
         "multigrid": mg,
         "nodes_per_s": round(N / (ms_mixed * 1e-3), 1),
@@ -580,6 +582,14 @@ def screening_sizes(L, tree):
     out = (ctypes.c_int64 * 3)()
     L.check(L.lib().prx_screening_sizes(tree._h, out))
     return int(out[0]), int(out[1]), int(out[2])
+
+
+def ref_style(rep):
+    """The reference's stopping quantity ||r|| / ||r0|| of the returned
+    solution (r0 the fine CG's starting residual after smoothing; Q13,
+    src/solvers.rs:36-58), beside the headline's true residual ||b - Ax|| / ||b||."""
+    r0 = rep.get("fine_r0_rel", 0.0)
+    return rep["true_rel"] / r0 if r0 > 0.0 else None
 
 
 def in_solve_launches(ds):

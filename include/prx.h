@@ -155,6 +155,11 @@ typedef struct prx_solve_report {
     double true_rel;        /* FP64 ||b-Ax||/||b|| of the final solution */
     double ms_assemble, ms_coarse, ms_fine_gs, ms_fine_cg, ms_total;
     int32_t status;
+    /* ||r0||/||b||, r0 the fine CG's starting residual after smoothing (single-tree
+     * cascadic solves with PRX_RESID_TRUE; 0 otherwise).  The reference's stopping
+     * quantity ||r||/||r0|| (src/solvers.rs:36-58) of the returned solution is
+     * true_rel / fine_r0_rel. */
+    double fine_r0_rel;
 } prx_solve_report;
 
 typedef struct prx_tree prx_tree;

@@ -84,7 +84,7 @@ class SolveReport(ctypes.Structure):
                 ("applies", ctypes.c_uint64), ("coarse_rel", ctypes.c_double), ("fine_rel", ctypes.c_double),
                 ("true_rel", ctypes.c_double), ("ms_assemble", ctypes.c_double), ("ms_coarse", ctypes.c_double),
                 ("ms_fine_gs", ctypes.c_double), ("ms_fine_cg", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("status", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("fine_r0_rel", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -522,6 +522,8 @@ static void fine_solve_fp64(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_so
     rep.ms_fine_cg = tm.ms();
     rep.fine_iters = o.iters;
     rep.fine_rel = o.rel;
+    // delta0: |b - A x|^2 where the CG started (after smoothing); replacements keep it
+    if (bb > 0.0) rep.fine_r0_rel = std::sqrt(t->st_host->delta0 / bb);
     rep.applies += o.applies;
     rep.outer_iters = 1;
     rep.status = o.status;
@@ -600,6 +602,7 @@ static void fine_solve_mixed(prx_tree* t, const SolveCfg& c, uint64_t cap, prx_s
     rep.applies += cg.applies;
     const double den = c.o.residual_mode == PRX_RESID_TRUE ? bb : r0;
     rep.fine_rel = den > 0.0 ? std::sqrt(rr / den) : 0.0;
+    if (c.o.residual_mode == PRX_RESID_TRUE && bb > 0.0) rep.fine_r0_rel = std::sqrt(r0 / bb);
     rep.status = status;
 }
 

@@ -499,6 +499,9 @@ def test_pass_timing_counts_every_fine_pass(gpu, prec):
         return rep, buf[:n.value]
 
     rep, ms = timed_solve(True)
+    # the fine CG's starting residual after smoothing, relative to b (far above 1: the prolongated coarse solution
+    # covers only fine offsets <= res/2, src/fem_tree.rs:310-343), and the reference's ||r||/||r0|| far below 1
+    assert rep["fine_r0_rel"] > 1.0 and rep["true_rel"] < 1e-3 * rep["fine_r0_rel"]
     worked = ms[ms >= 0.1 * np.median(ms)]
     assert len(worked) == rep["fine_iters"], (len(ms), len(worked), rep["fine_iters"])
     assert len(ms) - len(worked) <= 8 * max(1, rep["outer_iters"])
