@@ -61,6 +61,8 @@ constexpr uint64_t PERSIST_SPIN_LIMIT = 1ull << 22;
 // workgroup's writes at agent scope: the workgroup barrier does not wait for
 // the stores of the other waves, and a store still in flight when the L2 is
 // written back could reach another XCD's reader stale.
+// (0x0F70 on gfx9-family targets: vmcnt = 0, expcnt = 7 and lgkmcnt = 15, i.e.
+// no wait on those two counters.)
 __device__ __forceinline__ void own_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 __device__ inline bool gsync(PCtl* c, uint32_t& gen, int* sbad) {
     own_stores_done();
@@ -689,7 +691,11 @@ void launch_persistent_cg(const PArgs<T>& a0, const NodeList* scr, void* dargs, 
     PRX_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     // 60 s; PRX_PERSIST_DEADLINE_MS sets another (tests force the deadline exit with it)
     uint64_t deadline_ms = 60000;
-    if (const char* e = std::getenv("PRX_PERSIST_DEADLINE_MS")) deadline_ms = std::strtoull(e, nullptr, 10);
+    if (const char* e = std::getenv("PRX_PERSIST_DEADLINE_MS")) {
+        char* end = nullptr;
+        const unsigned long long v = std::strtoull(e, &end, 10);
+        if (end != e && *end == '\0') deadline_ms = v;
+    }
     a.deadline = (uint64_t)(khz > 0 ? khz : 100000) * deadline_ms;
     if (grid % 8) throw Error(PRX_ERR_DEVICE, "persistent CG: workgroup count not a multiple of the XCD count");
     PRX_HIP(hipMemcpyAsync(dargs, &a, sizeof(PArgs<T>), hipMemcpyHostToDevice, s));
