@@ -341,12 +341,25 @@ struct Cg {
     void run(uint64_t cap) {
         const int chunk = L.N >= (1 << 21) ? 8 : 32;
         uint64_t k = it;
-        for (;;) {
+        // The host enqueues chunk n + 1 before it waits for the state behind
+        // chunk n (a pinned snapshot copied after each chunk), so the device
+        // does not idle while the host looks.  The chunk enqueued past
+        // convergence returns at once: every kernel of an iteration tests the
+        // done flag.
+        CgState* snap = t->state_snapshots();
+        hipEvent_t* ev = t->snapshot_events();
+        for (int n = 0;; ++n) {
             for (int c = 0; c < chunk && k < cap; ++c) enqueue(k++);
-            read_state(t);
-            if (t->ptime.used) ptime_read(t);
-            if (t->st_host->done || k >= cap) break;
+            PRX_HIP(hipMemcpyAsync(snap + (n & 1), t->st, sizeof(CgState), hipMemcpyDeviceToHost, t->stream));
+            PRX_HIP(hipEventRecord(ev[n & 1], t->stream));
+            if (k >= cap) break;
+            if (n >= 1) {
+                PRX_HIP(hipEventSynchronize(ev[(n - 1) & 1]));
+                if (snap[(n - 1) & 1].done) break;
+            }
         }
+        read_state(t);
+        if (t->ptime.used) ptime_read(t);
         it = (uint64_t)t->st_host->it;
         if (fused && t->st_host->xn) {
             launch_x_flush<T>(x, pending_dirs(it, t->st_host->xn), t->st, L.N, false, t->stream);

@@ -129,6 +129,12 @@ struct prx_tree {
         if (!st_snap) PRX_HIP(hipHostMalloc((void**)&st_snap, 2 * sizeof(CgState), hipHostMallocDefault));
         return st_snap;
     }
+    hipEvent_t snap_ev[2] = {nullptr, nullptr};  // behind each snapshot copy
+    hipEvent_t* snapshot_events() {
+        for (auto& e : snap_ev)
+            if (!e) PRX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return snap_ev;
+    }
 
     // coarse level (Morton order inside)
     uint32_t cd = 0;
@@ -157,6 +163,8 @@ struct prx_tree {
     prx_tree() { floor_list.mode = 0; trunc_list.mode = 1; eval_list.mode = 2; }
     ~prx_tree() {
         for (hipEvent_t e : ptime.ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : snap_ev)
+            if (e) (void)hipEventDestroy(e);
         free_list(floor_list);
         free_list(trunc_list);
         free_list(eval_list);

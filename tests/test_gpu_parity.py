@@ -479,8 +479,9 @@ def test_release_workspace_then_solve_again(gpu):
 @pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
 def test_pass_timing_counts_every_fine_pass(gpu, prec):
     # prx_pass_timing records one duration per launch of the finest level's fused CG pass (the face form in the
-    # FP32 inner solves at depth 8): every iteration's pass, plus the few enqueued after an inner solve converged
-    # within a chunk, which return at once; nothing is recorded once timing is off
+    # FP32 inner solves at depth 8): every iteration's pass, plus those enqueued after an inner solve converged (the
+    # rest of its chunk of 8 and the chunk queued behind it), which return at once; nothing is recorded once timing
+    # is off
     import ctypes
     D = 8
     pts = unit_points("sphere", 30000, seed=15)
@@ -504,7 +505,7 @@ def test_pass_timing_counts_every_fine_pass(gpu, prec):
     assert rep["fine_r0_rel"] > 1.0 and rep["true_rel"] < 1e-3 * rep["fine_r0_rel"]
     worked = ms[ms >= 0.1 * np.median(ms)]
     assert len(worked) == rep["fine_iters"], (len(ms), len(worked), rep["fine_iters"])
-    assert len(ms) - len(worked) <= 8 * max(1, rep["outer_iters"])
+    assert len(ms) - len(worked) <= 16 * max(1, rep["outer_iters"])
     assert np.all(ms > 0) and np.all(np.isfinite(ms))
     # timing off: the next solve records nothing and the counter reads zero
     L.check(L.lib().prx_pass_timing(t._h, 1, None, 0, None))
