@@ -280,9 +280,12 @@ struct XcdSlot {
 // previous brick, as the hardware hands out the workgroups of a launch.  The
 // bricks in progress thus stay a narrow window of consecutive positions under
 // one L2, whose halos the neighbouring bricks read again; a fixed share per
-// workgroup lets the workgroups drift apart over hundreds of bricks.  The next
-// position is fetched while the current brick runs.  f(brick) is called by
-// the whole workgroup with a uniform brick id.  The counters start at zero.
+// workgroup lets the workgroups drift apart over hundreds of bricks.  Thread 0
+// takes the next position as the current brick starts (its wave waits for the
+// atomic; the other waves start on the brick), so the position is in LDS at
+// the next brick's barrier.  f(brick) is called by the whole workgroup with a
+// uniform brick id.  Every brick processed is counted in *done.  The counters
+// start at zero.
 template <typename F>
 __device__ inline void for_bricks(const PList& L, int n, const XcdSlot& xs, uint32_t* take, uint32_t* done, F f) {
     if (n <= 0) return;
