@@ -224,3 +224,26 @@ def test_batch_persistent_launch_ends_at_its_deadline(gpu, monkeypatch):
                      launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
     assert rep["converged"] == rep["instances"] and rep["max_true_rel"] <= 1e-6
     B.close()
+
+
+@pytest.mark.parametrize("prec", [pr.PRX_PREC_FP64, pr.PRX_PREC_MIXED])
+def test_batch_persistent_launch_single_instance(gpu, prec):
+    # a batch of one: the launch's work lists, brick counters and grid barrier with a single instance (many
+    # workgroups find no brick in most rounds); the result equals the graph replays' and reaches the residual
+    D, pts, pw = clouds()[0]
+    B = Batch()
+    B.add(D, pts, pw)
+    eps = 1e-10 if prec == pr.PRX_PREC_FP64 else 1e-9
+    gr, gper = B.solve(eps=eps, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False, precision=prec,
+                       launch=pr.PRX_BATCH_LAUNCH_GRAPHS)
+    xg = B.tree(0).solution
+    pr_, pper = B.solve(eps=eps, residual_mode=pr.PRX_RESID_TRUE, refresh_period=0, autotune=False, precision=prec,
+                        launch=pr.PRX_BATCH_LAUNCH_PERSISTENT)
+    xp = B.tree(0).solution
+    assert gr["converged"] == 1 and pr_["converged"] == 1 and pr_["max_true_rel"] <= eps * 1.0001
+    if prec == pr.PRX_PREC_FP64:
+        assert pper[0]["fine_iters"] == gper[0]["fine_iters"]
+        assert np.abs(xp - xg).max() <= 1e-9 * np.abs(xg).max()
+    else:
+        assert np.abs(xp - xg).max() <= 1e-6 * (xg.max() - xg.min())
+    B.close()
