@@ -505,7 +505,8 @@ def test_pass_timing_counts_every_fine_pass(gpu, prec):
     assert rep["fine_r0_rel"] > 1.0 and rep["true_rel"] < 1e-3 * rep["fine_r0_rel"]
     worked = ms[ms >= 0.1 * np.median(ms)]
     assert len(worked) == rep["fine_iters"], (len(ms), len(worked), rep["fine_iters"])
-    assert len(ms) - len(worked) <= 16 * max(1, rep["outer_iters"])
+    # (at most 15 per CG run; an FP64 run that restarts after its true-residual check counts twice)
+    assert len(ms) - len(worked) <= 32 * max(1, rep["outer_iters"])
     assert np.all(ms > 0) and np.all(np.isfinite(ms))
     # timing off: the next solve records nothing and the counter reads zero
     L.check(L.lib().prx_pass_timing(t._h, 1, None, 0, None))
